@@ -1,0 +1,155 @@
+/*
+ * dg_oracle.h -- CPU FP64 restatement of PHiLiP's explicit strong-form DG
+ * residual and explicit Runge-Kutta step, used ONLY as a test checker and as
+ * the reported CPU baseline.  Nothing in the product library links this.
+ *
+ * Reference behaviour restated (file:line in PHiLiP):
+ *   residual assembly        src/dg/dg_base.cpp:2599-2914, :453-870 (cell loop,
+ *                            face owner rule :417-449, penalty :397-413)
+ *   volume term (strong)     src/dg/strong_dg.cpp:1017-1579
+ *   face term (strong)       src/dg/strong_dg.cpp:2368-3576
+ *   auxiliary equation       src/dg/strong_dg.cpp:498-1008
+ *   inverse mass             src/dg/dg_base.cpp:4089-4260
+ *   1D operators / sum fact. src/operators/operators.cpp:294-649, :931-1196, :1217-2005
+ *   Euler physics            src/physics/euler.cpp:177-937, :1033-1062
+ *   Navier-Stokes viscous    src/physics/navier_stokes.cpp:240-385, :700-830, :1197-1350
+ *   numerical fluxes         src/numerical_flux/convective_numerical_flux.cpp:34-480,
+ *                            src/numerical_flux/viscous_numerical_flux.cpp:144-328
+ *   RK / low-storage RK      src/ode_solver/runge_kutta_ode_solver.cpp:15-110,
+ *                            src/ode_solver/low_storage_runge_kutta_ode_solver.cpp:24-222,
+ *                            src/ode_solver/runge_kutta_methods/*.cpp
+ *   TGV / vortex / dt / KE   src/physics/initial_conditions/initial_condition_function.cpp:261-336,:643-690,
+ *                            src/flow_solver/flow_solver_cases/periodic_turbulence.cpp:115-125,:439-671,
+ *                            src/flow_solver/flow_solver.cpp:554-610
+ *
+ * Vector layout is the reference's DoF order without renumbering:
+ *   u[(cell*nstate + state)*Np + node], node lexicographic (x fastest),
+ *   cells in deal.II active-cell order (Z/Morton order for power-of-two
+ *   cells per direction after refine_global, lexicographic otherwise).
+ */
+#ifndef DG_ORACLE_H
+#define DG_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGO_MAXN 16
+
+enum { DGO_GLL = 0, DGO_GL = 1 };
+enum { DGO_KG = 0, DGO_IR = 1, DGO_CH = 2, DGO_RA = 3 };
+enum {
+    DGO_LXF = 0, DGO_ROE = 1, DGO_CENTRAL = 2, DGO_TWO_POINT = 3,
+    DGO_TWO_POINT_LXF = 4, DGO_TWO_POINT_ROE = 5
+};
+enum { DGO_EULER = 0, DGO_NAVIER_STOKES = 1 };
+enum { DGO_SIPG = 0, DGO_BR2 = 1 };
+enum { DGO_ORDER_DEALII = 0, DGO_ORDER_LEXICOGRAPHIC = 1 };
+
+typedef struct {
+    int dim;                 /* 2 or 3 */
+    int poly_degree;         /* p */
+    int flux_nodes;          /* DGO_GLL / DGO_GL */
+    int use_split_form;
+    int two_point_flux;      /* DGO_KG.. */
+    int conv_flux;           /* DGO_LXF.. */
+    int pde;                 /* DGO_EULER / DGO_NAVIER_STOKES */
+    int diss_flux;           /* DGO_SIPG / DGO_BR2 */
+    int use_constant_viscosity;
+    double gamma_gas;
+    double mach_inf;
+    double reynolds_inf;
+    double prandtl;
+    double constant_viscosity;
+    double temperature_inf;
+    double sipg_penalty_factor;
+    int n_cells_1d;
+    double domain_left;
+    double domain_right;
+    int cell_order;          /* DGO_ORDER_DEALII / DGO_ORDER_LEXICOGRAPHIC */
+} dgo_params;
+
+typedef struct dgo_ctx dgo_ctx;
+
+void dgo_default_params(dgo_params *p);
+dgo_ctx *dgo_create(const dgo_params *p);
+void dgo_destroy(dgo_ctx *c);
+
+long dgo_n_dofs(const dgo_ctx *c);
+int dgo_n_cells(const dgo_ctx *c);
+
+/* 1D nodes and weights on [0,1] */
+void dgo_gauss_legendre(int n, double *x, double *w);
+void dgo_gauss_lobatto(int n, double *x, double *w);
+/* Lagrange basis on support points s (n), values/derivatives at x: out[j] */
+void dgo_lagrange_values(int n, const double *s, double x, double *val, double *der);
+
+/* operators used by the context (row-major, [rows][DGO_MAXN] packed as rows*n) */
+void dgo_get_1d_operators(const dgo_ctx *c, double *nodes, double *weights,
+                          double *chi, double *chi_grad, double *flux_grad,
+                          double *skew, double *proj, double *minv,
+                          double *chi_surf, double *flux_surf);
+
+/* sum-factorized apply y = (Az x Ay x Ax) x ; dim 2 ignores Az */
+void dgo_tensor_apply(int dim, const double *Ax, int rx, int cx,
+                      const double *Ay, int ry, int cy,
+                      const double *Az, int rz, int cz,
+                      const double *in, double *out, int adding, double factor);
+
+/* Hadamard sparsity patterns (integers): rows/cols sized n^dim*n*dim */
+void dgo_hadamard_volume_pattern(int dim, int n, unsigned *rows, unsigned *cols);
+void dgo_hadamard_surface_pattern(int dim, int n, int dim_not_zero, unsigned *rows, unsigned *cols);
+
+/* mesh */
+void dgo_cell_ijk(const dgo_ctx *c, int *ijk);            /* [ncells][3] */
+void dgo_neighbors(const dgo_ctx *c, int *nbr);            /* [ncells][2*dim] */
+void dgo_face_list(const dgo_ctx *c, int *faces);          /* [nfaces][4]: owner, iface, nbr, nbr_iface */
+int  dgo_n_faces(const dgo_ctx *c);
+void dgo_node_coords(const dgo_ctx *c, double *xyz);       /* [ncells][Np][3] */
+
+/* point physics (nstate = dim+2) */
+double dgo_pressure(const dgo_ctx *c, const double *u);
+void dgo_convective_flux(const dgo_ctx *c, const double *u, double *f /*[ns][dim]*/);
+void dgo_entropy_variables(const dgo_ctx *c, const double *u, double *v);
+void dgo_conservative_from_entropy(const dgo_ctx *c, const double *v, double *u);
+void dgo_two_point_flux(const dgo_ctx *c, int which, const double *u1, const double *u2, double *f);
+double dgo_log_mean(double a, double b);
+void dgo_numerical_flux(const dgo_ctx *c, int which, const double *ui, const double *ue,
+                        const double *n, double *f);
+void dgo_dissipative_flux(const dgo_ctx *c, const double *u, const double *grad /*[ns][dim]*/,
+                          double *f /*[ns][dim]*/);
+double dgo_max_convective_eigenvalue(const dgo_ctx *c, const double *u);
+
+/* residual, inverse mass */
+void dgo_assemble_residual(dgo_ctx *c, const double *u, double *rhs);
+void dgo_apply_inverse_mass(const dgo_ctx *c, const double *in, double *out, int aux);
+void dgo_get_auxiliary(const dgo_ctx *c, int idim, double *out);
+
+/* initial conditions */
+void dgo_ic_tgv(const dgo_ctx *c, int isothermal, double *u);
+void dgo_ic_vortex(const dgo_ctx *c, double strength, double u_inf, double v_inf, double *u);
+
+/* time integration: method ids 0 euler,1 heun,2 ssprk3,3 rk4,
+   10 RK4(3)5[3S*], 11 RK3(2)5F[3S*+], 12 RK4(3)9F[3S*+], 13 RK5(4)10F[3S*+] */
+int dgo_n_stages(int method);
+void dgo_step(dgo_ctx *c, int method, double dt, double *u);
+double dgo_constant_time_step(const dgo_ctx *c, double cfl);
+double dgo_vortex_time_step(const dgo_ctx *c, double cfl, double speed);
+
+/* integrated quantities with QGauss(p+1+overint) : out[0]=KE, [1]=enstrophy,
+   [2]=incompressible KE, [3]=vorticity-based dissipation rate */
+void dgo_integrated_quantities(const dgo_ctx *c, const double *u, int overint, double *out);
+/* numerical entropy  sum_i w_i J_i (-rho s)/(gamma-1) at volume nodes (mass-lumped, reference
+   periodic_entropy_tests style uses M-weighted v.u; see oracle source) */
+double dgo_entropy_rate(dgo_ctx *c, const double *u);
+
+/* full flow-solver loop; returns number of steps; ke_hist gets KE after every step (may be NULL) */
+int dgo_run(dgo_ctx *c, int method, double dt_const, double final_time, double *u,
+            double *ke_hist, int max_hist);
+
+void dgo_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
