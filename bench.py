@@ -1,0 +1,123 @@
+"""Benchmark of the explicit strong-form DG RK stage on one MI355X.
+
+Workload (the configuration the metric is quoted on): 3D Euler Taylor-Green
+vortex, p=3, 32^3 periodic hex cells (2,097,152 nodes x 5 states =
+10,485,760 DOF), Chandrashekar split-form volume flux with entropy-conserving
++ Roe (Harten) surface flux, collocated GLL nodes, low-storage RK3(2)5F[3S*+]
+(5 stages per step).  The state and all RK registers are resident in HBM before
+the timed region; a "step" is one RK step of the whole mesh.
+
+Metric: DOF-updates/s per RK stage = DOF x stages x steps / wall time.
+Roofline: HBM-bound; algorithmic bytes per stage launch (FP64 registers of the
+fused 3S* stage) = read S1,S2,S3 + write S1',S2 = 40 B/DOF (32 B/DOF for the
+first stage, which writes S3 = S1 and starts S2 from zero).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_params(precision):
+    import philip_amd as P
+    return dict(dim=3, poly_degree=3, n_cells_1d=32, use_split_form=1, two_point_flux=P.CH,
+                conv_flux=P.TWO_POINT_ROE, pde=P.EULER, mach_inf=0.1, precision=precision)
+
+
+def cpu_baseline(threads):
+    """Oracle (FP64 C restatement of the reference loop structure, OpenMP over cells) timed on
+    the host: one RK3(2)5F[3S*+] step of the full workload mesh (5 stages + the reference's FSAL
+    residual), credited with 5 stage updates."""
+    import oracle as O
+    O.set_threads(threads)
+    o = O.Oracle(dim=3, poly_degree=3, n_cells_1d=32, use_split_form=1, two_point_flux=O.CH,
+                 conv_flux=O.TWO_POINT_ROE, mach_inf=0.1)
+    u = o.ic_tgv()
+    dt = o.constant_time_step(0.1)
+    t0 = time.perf_counter()
+    o.step("rk3_2_5f_3sstarplus", dt, u)
+    t = time.perf_counter() - t0
+    return dict(value=o.ndofs * 5 / t, unit="DOF-updates/s", cores=threads, kind="port",
+                sample=f"1 RK3(2)5F[3S*+] step (5 stages + FSAL residual) of the 32^3 p=3 mesh, {t:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
+        sys.exit(2)
+
+    import philip_amd as P
+
+    prec = P.FP64 if args.precision == "fp64" else P.MIXED
+    d = P.DG(**workload_params(prec))
+    d.set_initial_condition_tgv()
+    method = "rk3_2_5f_3sstarplus"
+    stages = 5
+    dt = d.constant_time_step(0.1)
+    d.allocate_ode_system(method)
+    d.step_in_time(dt, args.warmup)
+    d.solution()  # synchronizes
+    t0 = time.perf_counter()
+    d.step_in_time(dt, args.steps)
+    _ = d.solution()[:1]  # device sync through the blocking copy of the context stream
+    t_total = time.perf_counter() - t0
+    # the blocking copy of the full vector is inside the window: time it separately and remove it
+    t0c = time.perf_counter()
+    d.solution()
+    t_copy = time.perf_counter() - t0c
+    t = max(t_total - t_copy, 1e-9)
+
+    ndofs = d.ndofs
+    value = ndofs * stages * args.steps / t
+    ms_per_step = 1e3 * t / args.steps
+
+    # per-launch kernel time with HIP events on the context stream
+    d.enable_timing(True)
+    d.step_in_time(dt, 2)
+    avg_ms, nlaunch = d.kernel_timing()
+    d.enable_timing(False)
+    bytes_per_launch = ndofs * (32 + 4 * 40) / 5.0
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+
+    line = {
+        "metric": "DOF-updates/s per RK stage (p=3 Euler TGV)",
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == P.FP64 else "f32-flux/f64-state",
+        "data": "synthetic (TGV initial condition, reference's closed form)",
+        "config": {"workload": "Euler TGV p=3 32^3 hex, CH split + EC/Roe surface flux, GLL, RK3(2)5F[3S*+]",
+                   "dof": ndofs, "stages_per_step": stages, "precision": args.precision},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "stage_kernel<3,4,split>", "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,155 @@
+/*
+ * philip_dg.h -- C ABI of the MI355X-native explicit strong-form DG path.
+ *
+ * The reference (PHiLiP) has no foreign-function interface: its callers are
+ * C++ (ODE solvers, flow solver, tests).  The entry points below are what a
+ * C/C++/ctypes/cgo caller binds to replace that path; each cites the
+ * reference interface it stands in for.  No torch or HIP types cross this
+ * boundary: plain pointers, sizes, doubles.  Every entry point returns 0 on
+ * success or a negative DGX_ERR_* code; dgx_last_error() gives the message.
+ *
+ * Vector layout everywhere is the reference's DoF order without
+ * renumbering: v[(cell*nstate + state)*n_nodes + node], node lexicographic
+ * with x fastest (deal.II FE_DGQ), cells in deal.II active-cell order (Z order
+ * after refine_global for 2^k cells per direction, lexicographic otherwise).
+ */
+#ifndef PHILIP_DG_H
+#define PHILIP_DG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGX_OK 0
+#define DGX_ERR_INVALID (-1)      /* invalid configuration or argument        */
+#define DGX_ERR_HIP (-2)          /* HIP runtime error                        */
+#define DGX_ERR_NONPHYSICAL (-3)  /* non-physical state detected              */
+#define DGX_ERR_UNSUPPORTED (-4)  /* configuration the device path lacks      */
+#define DGX_ERR_STATE (-5)        /* call out of order (e.g. no ODE system)   */
+
+/* enumerations mirror Parameters::AllParameters (src/parameters/all_parameters.cpp:73-399) */
+enum dgx_flux_nodes { DGX_GLL = 0, DGX_GL = 1 };
+enum dgx_two_point { DGX_KG = 0, DGX_IR = 1, DGX_CH = 2, DGX_RA = 3 };
+enum dgx_conv_flux {
+    DGX_LAX_FRIEDRICHS = 0, DGX_ROE = 1, DGX_CENTRAL = 2, DGX_TWO_POINT = 3,
+    DGX_TWO_POINT_LXF = 4, DGX_TWO_POINT_ROE = 5
+};
+enum dgx_pde { DGX_EULER = 0, DGX_NAVIER_STOKES = 1 };
+enum dgx_diss_flux { DGX_SIPG = 0, DGX_BR2 = 1 };
+enum dgx_cell_order { DGX_ORDER_DEALII = 0, DGX_ORDER_LEXICOGRAPHIC = 1 };
+enum dgx_precision { DGX_FP64 = 0, DGX_MIXED = 1 };
+/* ODE methods (Parameters::ODESolverParam, parameters_ode_solver.cpp:111-296) */
+enum dgx_ode_method {
+    DGX_EULER_EX = 0, DGX_HEUN_EX = 1, DGX_SSPRK3_EX = 2, DGX_RK4_EX = 3,
+    DGX_RK4_3_5_3SSTAR = 10, DGX_RK3_2_5F_3SSTARPLUS = 11,
+    DGX_RK4_3_9F_3SSTARPLUS = 12, DGX_RK5_4_10F_3SSTARPLUS = 13
+};
+
+/* Scheme + mesh description: the subset of AllParameters this path reads
+   (all_parameters.cpp:73-155,325-351; parameters_navier_stokes.cpp:17-32;
+   flow solver grid subsection) plus the device precision. */
+typedef struct {
+    int dim;                    /* 2 or 3                                          */
+    int poly_degree;            /* p, 1..7                                         */
+    int flux_nodes;             /* DGX_GLL (device path); DGX_GL rejected           */
+    int use_split_form;
+    int two_point_flux;         /* two_point_num_flux_type                         */
+    int conv_flux;              /* conv_num_flux                                   */
+    int pde;                    /* pde_type                                        */
+    int diss_flux;              /* diss_num_flux                                   */
+    int use_constant_viscosity;
+    double gamma_gas;
+    double mach_inf;
+    double reynolds_inf;
+    double prandtl;
+    double constant_viscosity;
+    double temperature_inf;
+    double sipg_penalty_factor;
+    int n_cells_1d;             /* straight_periodic_cube cells per direction      */
+    double domain_left;
+    double domain_right;
+    int cell_order;
+    int precision;              /* DGX_FP64 or DGX_MIXED (FP32 fluxes, FP64 state)  */
+} dgx_params;
+
+typedef struct dgx_ctx dgx_ctx;
+
+/* Fills the reference defaults (all_parameters.cpp declare_parameters). */
+void dgx_default_params(dgx_params *p);
+
+/* DGFactory::create_discontinuous_galerkin + allocate_system
+   (flow_solver.cpp:44, dg_factory.cpp, dg_base.hpp:184): builds operators,
+   connectivity and device buffers on the current HIP device. */
+int dgx_create(const dgx_params *p, dgx_ctx **out);
+int dgx_destroy(dgx_ctx *ctx);
+const char *dgx_last_error(const dgx_ctx *ctx);
+
+int64_t dgx_n_dofs(const dgx_ctx *ctx);
+int dgx_n_cells(const dgx_ctx *ctx);
+
+/* SetInitialCondition::interpolate_initial_condition (set_initial_condition.cpp:45-51)
+   of InitialConditionFunction_TaylorGreenVortex[_Isothermal]
+   (initial_condition_function.cpp:261-336); resets current_time/iteration. */
+int dgx_set_initial_condition_tgv(dgx_ctx *ctx, int isothermal);
+/* Isentropic vortex (initial_condition_function.cpp:643-690) with strength and advection
+   velocity as parameters: beta = sqrt(2/gamma) sqrt(2) strength/(4 pi) e^(1/2); strength 5
+   and (u,v) = sqrt(2/gamma)(cos pi/4, sin pi/4) is the reference's hard-coded case. */
+int dgx_set_initial_condition_vortex(dgx_ctx *ctx, double strength, double u_inf, double v_inf);
+
+/* DGBase::solution (dg_base.hpp:409): host <-> device copies. */
+int dgx_set_solution(dgx_ctx *ctx, const double *host_u);
+int dgx_get_solution(const dgx_ctx *ctx, double *host_u);
+/* DGBase::right_hand_side (dg_base.hpp:396): the un-lifted residual. */
+int dgx_get_right_hand_side(const dgx_ctx *ctx, double *host_r);
+/* DGBase::auxiliary_solution[idim] (dg_base.hpp:419), Navier-Stokes only. */
+int dgx_get_auxiliary_solution(const dgx_ctx *ctx, int idim, double *host_q);
+/* integer connectivity used by the kernels: nbr[cell][2*dim] (deal.II face numbering). */
+int dgx_get_neighbors(const dgx_ctx *ctx, int32_t *host_nbr);
+
+/* DGBase::assemble_residual(false,false,false,0.0) (dg_base.hpp:571,
+   dg_base.cpp:2599-2914): right_hand_side = R(solution). */
+int dgx_assemble_residual(dgx_ctx *ctx);
+/* DGBase::apply_inverse_global_mass_matrix(right_hand_side, out) (dg_base.hpp:267,
+   dg_base.cpp:4089-4260); out is a host array of n_dofs. */
+int dgx_apply_inverse_mass_to_rhs(dgx_ctx *ctx, double *host_out);
+
+/* Device-pointer variants (inputs/outputs already resident in HBM, same layout).
+   u_dev, r_dev: n_dofs doubles on the context's device. */
+int dgx_assemble_residual_device(dgx_ctx *ctx, const double *u_dev, double *r_dev);
+int dgx_rhs_device(dgx_ctx *ctx, const double *u_dev, double *dudt_dev);
+
+/* ODESolverFactory::create_ODESolver + allocate_ode_system (ode_solver_factory.cpp:20-35,
+   ode_solver_base.h:79): allocates the RK registers for `method`. */
+int dgx_allocate_ode_system(dgx_ctx *ctx, int method);
+/* ODESolverBase::step_in_time(dt, pseudotime=false) called n_steps times
+   (ode_solver_base.h:70, runge_kutta_base.cpp:21-38): advances solution,
+   current_time += dt, current_iteration += 1 per step. */
+int dgx_step_in_time(dgx_ctx *ctx, double dt, int n_steps);
+double dgx_current_time(const dgx_ctx *ctx);
+int64_t dgx_current_iteration(const dgx_ctx *ctx);
+/* 3S*+ methods: also track the embedded register (extra FSAL residual per step,
+   low_storage_runge_kutta_ode_solver.cpp:63-69); off by default because it does
+   not change the solution at a fixed time step. */
+int dgx_set_track_error_estimate(dgx_ctx *ctx, int enable);
+
+/* PeriodicTurbulence::get_constant_time_step (periodic_turbulence.cpp:115-125). */
+double dgx_constant_time_step(const dgx_ctx *ctx, double cfl);
+/* PeriodicTurbulence::compute_and_update_integrated_quantities (periodic_turbulence.cpp:439-671)
+   with QGauss(p+1+overint): out[0]=kinetic energy, [1]=enstrophy, [2]=incompressible KE,
+   [3]=vorticity-based dissipation rate; reductions in FP64 on the device. */
+int dgx_integrated_quantities(dgx_ctx *ctx, int overint, double *out4);
+
+/* HIP stream the context launches on (as an opaque pointer); 0 = default stream. */
+int dgx_set_stream(dgx_ctx *ctx, void *hip_stream);
+/* Average duration (ms) of the last `dgx_step_in_time` stage kernels, measured with
+   HIP events on the context stream when timing is enabled. */
+int dgx_enable_timing(dgx_ctx *ctx, int enable);
+int dgx_kernel_timing(const dgx_ctx *ctx, double *avg_stage_ms, int64_t *n_stage_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,234 @@
+"""Python binding of the MI355X device DG library (libphilip_dg.so, C ABI in
+include/philip_dg.h).  The library is built in-tree by `make -C philip_amd`;
+importing `DG` without it raises immediately -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphilip_dg.so")
+
+GLL, GL = 0, 1
+KG, IR, CH, RA = 0, 1, 2, 3
+LXF, ROE, CENTRAL, TWO_POINT, TWO_POINT_LXF, TWO_POINT_ROE = 0, 1, 2, 3, 4, 5
+EULER, NAVIER_STOKES = 0, 1
+SIPG, BR2 = 0, 1
+FP64, MIXED = 0, 1
+METHODS = {"euler": 0, "heun": 1, "ssprk3": 2, "rk4": 3,
+           "rk4_3_5_3sstar": 10, "rk3_2_5f_3sstarplus": 11,
+           "rk4_3_9f_3sstarplus": 12, "rk5_4_10f_3sstarplus": 13}
+
+EXPORTED = [
+    "dgx_default_params", "dgx_create", "dgx_destroy", "dgx_last_error", "dgx_n_dofs", "dgx_n_cells",
+    "dgx_set_solution", "dgx_get_solution", "dgx_get_right_hand_side", "dgx_get_auxiliary_solution",
+    "dgx_get_neighbors", "dgx_assemble_residual", "dgx_apply_inverse_mass_to_rhs",
+    "dgx_assemble_residual_device", "dgx_rhs_device", "dgx_allocate_ode_system", "dgx_step_in_time",
+    "dgx_current_time", "dgx_current_iteration", "dgx_set_track_error_estimate",
+    "dgx_constant_time_step", "dgx_integrated_quantities", "dgx_set_stream", "dgx_enable_timing",
+    "dgx_kernel_timing", "dgx_set_initial_condition_tgv", "dgx_set_initial_condition_vortex",
+]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int), ("poly_degree", ctypes.c_int), ("flux_nodes", ctypes.c_int),
+        ("use_split_form", ctypes.c_int), ("two_point_flux", ctypes.c_int), ("conv_flux", ctypes.c_int),
+        ("pde", ctypes.c_int), ("diss_flux", ctypes.c_int), ("use_constant_viscosity", ctypes.c_int),
+        ("gamma_gas", ctypes.c_double), ("mach_inf", ctypes.c_double), ("reynolds_inf", ctypes.c_double),
+        ("prandtl", ctypes.c_double), ("constant_viscosity", ctypes.c_double),
+        ("temperature_inf", ctypes.c_double), ("sipg_penalty_factor", ctypes.c_double),
+        ("n_cells_1d", ctypes.c_int), ("domain_left", ctypes.c_double), ("domain_right", ctypes.c_double),
+        ("cell_order", ctypes.c_int), ("precision", ctypes.c_int),
+    ]
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load the in-tree device library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DeviceError(f"{LIB_PATH} is missing: build it with `make -C philip_amd` (no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    dp = P(d)
+    i64 = ctypes.c_int64
+    lib.dgx_default_params.argtypes = [P(Params)]
+    lib.dgx_create.argtypes = [P(Params), P(vp)]
+    lib.dgx_destroy.argtypes = [vp]
+    lib.dgx_last_error.restype = ctypes.c_char_p
+    lib.dgx_last_error.argtypes = [vp]
+    lib.dgx_n_dofs.restype = i64
+    lib.dgx_n_dofs.argtypes = [vp]
+    lib.dgx_n_cells.argtypes = [vp]
+    for fn in ["dgx_set_solution", "dgx_get_solution", "dgx_get_right_hand_side", "dgx_apply_inverse_mass_to_rhs"]:
+        getattr(lib, fn).argtypes = [vp, dp]
+    lib.dgx_get_auxiliary_solution.argtypes = [vp, i, dp]
+    lib.dgx_set_initial_condition_tgv.argtypes = [vp, i]
+    lib.dgx_set_initial_condition_vortex.argtypes = [vp, d, d, d]
+    lib.dgx_get_neighbors.argtypes = [vp, P(ctypes.c_int32)]
+    lib.dgx_assemble_residual.argtypes = [vp]
+    lib.dgx_assemble_residual_device.argtypes = [vp, vp, vp]
+    lib.dgx_rhs_device.argtypes = [vp, vp, vp]
+    lib.dgx_allocate_ode_system.argtypes = [vp, i]
+    lib.dgx_step_in_time.argtypes = [vp, d, i]
+    lib.dgx_current_time.restype = d
+    lib.dgx_current_time.argtypes = [vp]
+    lib.dgx_current_iteration.restype = i64
+    lib.dgx_current_iteration.argtypes = [vp]
+    lib.dgx_set_track_error_estimate.argtypes = [vp, i]
+    lib.dgx_constant_time_step.restype = d
+    lib.dgx_constant_time_step.argtypes = [vp, d]
+    lib.dgx_integrated_quantities.argtypes = [vp, i, dp]
+    lib.dgx_set_stream.argtypes = [vp, vp]
+    lib.dgx_enable_timing.argtypes = [vp, i]
+    lib.dgx_kernel_timing.argtypes = [vp, dp, P(i64)]
+    _lib = lib
+    return lib
+
+
+def params(**kw):
+    p = Params()
+    load().dgx_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class DG:
+    """DGBase-shaped handle over the device context (one per HIP device/stream)."""
+
+    def __init__(self, **kw):
+        lib = load()
+        self.p = params(**kw)
+        h = ctypes.c_void_p()
+        rc = lib.dgx_create(ctypes.byref(self.p), ctypes.byref(h))
+        if rc != 0:
+            raise DeviceError(f"dgx_create failed with code {rc}")
+        self.h = h
+        self.dim = self.p.dim
+        self.ns = self.dim + 2
+        self.n = self.p.poly_degree + 1
+        self.Np = self.n ** self.dim
+        self.ndofs = lib.dgx_n_dofs(self.h)
+        self.ncells = lib.dgx_n_cells(self.h)
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise DeviceError(f"{what}: code {rc}: {load().dgx_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().dgx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_solution(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.size == self.ndofs
+        self._chk(load().dgx_set_solution(self.h, _dp(u)), "set_solution")
+
+    def set_initial_condition_tgv(self, isothermal=False):
+        self._chk(load().dgx_set_initial_condition_tgv(self.h, int(isothermal)), "set_initial_condition_tgv")
+
+    def set_initial_condition_vortex(self, strength=5.0, u_inf=None, v_inf=None):
+        m = np.sqrt(2 / 1.4)
+        u_inf = m * np.cos(np.pi / 4) if u_inf is None else u_inf
+        v_inf = m * np.sin(np.pi / 4) if v_inf is None else v_inf
+        self._chk(load().dgx_set_initial_condition_vortex(self.h, strength, u_inf, v_inf),
+                  "set_initial_condition_vortex")
+
+    def solution(self):
+        u = np.empty(self.ndofs)
+        self._chk(load().dgx_get_solution(self.h, _dp(u)), "get_solution")
+        return u
+
+    def assemble_residual(self):
+        self._chk(load().dgx_assemble_residual(self.h), "assemble_residual")
+        r = np.empty(self.ndofs)
+        self._chk(load().dgx_get_right_hand_side(self.h, _dp(r)), "get_right_hand_side")
+        return r
+
+    def apply_inverse_mass_to_rhs(self):
+        o = np.empty(self.ndofs)
+        self._chk(load().dgx_apply_inverse_mass_to_rhs(self.h, _dp(o)), "apply_inverse_mass")
+        return o
+
+    def auxiliary_solution(self, k):
+        q = np.empty(self.ndofs)
+        self._chk(load().dgx_get_auxiliary_solution(self.h, k, _dp(q)), "get_auxiliary_solution")
+        return q
+
+    def neighbors(self):
+        a = np.empty((self.ncells, 2 * self.dim), dtype=np.int32)
+        self._chk(load().dgx_get_neighbors(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "neighbors")
+        return a
+
+    def allocate_ode_system(self, method):
+        m = METHODS[method] if isinstance(method, str) else method
+        self._chk(load().dgx_allocate_ode_system(self.h, m), "allocate_ode_system")
+
+    def step_in_time(self, dt, n_steps=1):
+        self._chk(load().dgx_step_in_time(self.h, dt, n_steps), "step_in_time")
+
+    def set_track_error_estimate(self, on=True):
+        self._chk(load().dgx_set_track_error_estimate(self.h, int(on)), "set_track_error_estimate")
+
+    @property
+    def current_time(self):
+        return load().dgx_current_time(self.h)
+
+    @property
+    def current_iteration(self):
+        return load().dgx_current_iteration(self.h)
+
+    def constant_time_step(self, cfl):
+        return load().dgx_constant_time_step(self.h, cfl)
+
+    def integrated(self, overint=10):
+        o = np.zeros(4)
+        self._chk(load().dgx_integrated_quantities(self.h, overint, _dp(o)), "integrated_quantities")
+        return dict(kinetic_energy=o[0], enstrophy=o[1], incompressible_kinetic_energy=o[2],
+                    vorticity_dissipation=o[3])
+
+    def enable_timing(self, on=True):
+        self._chk(load().dgx_enable_timing(self.h, int(on)), "enable_timing")
+
+    def kernel_timing(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._chk(load().dgx_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)), "kernel_timing")
+        return ms.value, n.value
+
+    def run(self, method, dt, final_time):
+        """FlowSolver::run time loop (flow_solver.cpp:554-610) with a constant dt."""
+        self.allocate_ode_system(method)
+        t = self.current_time
+        steps = 0
+        while t < final_time:
+            step = dt
+            if t + step > final_time:
+                step = final_time - t
+            self.step_in_time(step, 1)
+            t = self.current_time
+            steps += 1
+        return steps

@@ -1,0 +1,774 @@
+// C ABI of the device DG path (declared in include/philip_dg.h) and the host
+// set-up it needs: operators, connectivity, buffers, RK register management.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dg_host.hpp"
+
+namespace dgx {
+
+static void legendre_pd(int n, double x, double *pn, double *dpn)
+{
+    double p0 = 1.0, p1 = x;
+    if (n == 0) { *pn = 1.0; *dpn = 0.0; return; }
+    for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    *pn = p1;
+    *dpn = (std::fabs(x) < 1.0) ? n * (x * p1 - p0) / (x * x - 1.0) : 0.5 * n * (n + 1.0);
+}
+
+void gauss_lobatto(int n, double *x, double *w)
+{
+    const int N = n - 1;
+    std::vector<double> t(n);
+    for (int i = 0; i < n; ++i) {
+        double z = std::cos(M_PI * i / N);
+        if (i > 0 && i < N) {
+            for (int it = 0; it < 100; ++it) {
+                double pn, dpn;
+                legendre_pd(N, z, &pn, &dpn);
+                const double d2 = (2.0 * z * dpn - N * (N + 1.0) * pn) / (1.0 - z * z);
+                const double dz = dpn / d2;
+                z -= dz;
+                if (std::fabs(dz) < 1e-17) break;
+            }
+        }
+        t[i] = z;
+    }
+    t[0] = 1.0;
+    t[N] = -1.0;
+    for (int i = 0; i < n; ++i) {
+        double pn, dpn;
+        legendre_pd(N, t[i], &pn, &dpn);
+        x[i] = 0.5 * (1.0 - t[i]);
+        w[i] = 1.0 / (N * (N + 1.0) * pn * pn);
+    }
+    for (int i = 0; i < n / 2; ++i) {
+        const double a = 0.5 * (x[i] + 1.0 - x[n - 1 - i]);
+        x[i] = a;
+        x[n - 1 - i] = 1.0 - a;
+        const double b = 0.5 * (w[i] + w[n - 1 - i]);
+        w[i] = b;
+        w[n - 1 - i] = b;
+    }
+    x[0] = 0.0;
+    x[n - 1] = 1.0;
+    if (n % 2 == 1) x[n / 2] = 0.5;
+}
+
+void gauss_legendre(int n, double *x, double *w)
+{
+    for (int i = 0; i < n; ++i) {
+        double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+        for (int it = 0; it < 100; ++it) {
+            double pn, dpn;
+            legendre_pd(n, z, &pn, &dpn);
+            const double dz = pn / dpn;
+            z -= dz;
+            if (std::fabs(dz) < 1e-17) break;
+        }
+        double pn, dpn;
+        legendre_pd(n, z, &pn, &dpn);
+        x[i] = 0.5 * (1.0 - z);
+        w[i] = 1.0 / ((1.0 - z * z) * dpn * dpn);
+    }
+    for (int i = 0; i < n / 2; ++i) {
+        const double a = 0.5 * (x[i] + 1.0 - x[n - 1 - i]);
+        x[i] = a;
+        x[n - 1 - i] = 1.0 - a;
+        const double b = 0.5 * (w[i] + w[n - 1 - i]);
+        w[i] = b;
+        w[n - 1 - i] = b;
+    }
+    if (n % 2 == 1) x[n / 2] = 0.5;
+}
+
+void lagrange(int n, const double *s, double x, double *val, double *der)
+{
+    for (int j = 0; j < n; ++j) {
+        double v = 1.0;
+        for (int m = 0; m < n; ++m)
+            if (m != j) v *= (x - s[m]) / (s[j] - s[m]);
+        double d = 0.0;
+        for (int k = 0; k < n; ++k) {
+            if (k == j) continue;
+            double t = 1.0 / (s[j] - s[k]);
+            for (int m = 0; m < n; ++m)
+                if (m != j && m != k) t *= (x - s[m]) / (s[j] - s[m]);
+            d += t;
+        }
+        if (val) val[j] = v;
+        if (der) der[j] = d;
+    }
+}
+
+Ops1D build_ops(int n)
+{
+    Ops1D o{};
+    double x[MAXN1D], w[MAXN1D];
+    gauss_lobatto(n, x, w);
+    for (int a = 0; a < n; ++a) {
+        double val[MAXN1D], der[MAXN1D];
+        lagrange(n, x, x[a], val, der);
+        for (int b = 0; b < n; ++b) o.D[a][b] = der[b];
+        o.w[a] = w[a];
+    }
+    // flux-basis stiffness Q = W D (operators.cpp:1440-1477), skew form scaled by 1/w_a
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) o.Sk[a][b] = (w[a] * o.D[a][b] - w[b] * o.D[b][a]) / w[a];
+    o.inv_wb = 1.0 / w[0];
+    return o;
+}
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+void cell_ijk(int dim, int N, int order, std::vector<int> &ijk)
+{
+    const int nc = (dim == 3) ? N * N * N : N * N;
+    ijk.assign((size_t)nc * 3, 0);
+    const bool morton = (order == DGX_ORDER_DEALII) && is_pow2(N);
+    int levels = 0;
+    while ((1 << levels) < N) ++levels;
+    for (int e = 0; e < nc; ++e) {
+        int c[3] = {0, 0, 0};
+        if (morton) {
+            // children of a refined cell are numbered x-fastest; coarsest level in the high bits
+            for (int bit = 0; bit < levels; ++bit)
+                for (int k = 0; k < dim; ++k) c[k] |= ((e >> (bit * dim + k)) & 1) << bit;
+        } else {
+            c[0] = e % N;
+            c[1] = (e / N) % N;
+            if (dim == 3) c[2] = e / (N * N);
+        }
+        for (int k = 0; k < 3; ++k) ijk[(size_t)e * 3 + k] = c[k];
+    }
+}
+
+void neighbors(int dim, int N, const std::vector<int> &ijk, std::vector<int> &nbr)
+{
+    const int nc = (int)(ijk.size() / 3);
+    std::vector<int> lookup((size_t)nc);
+    for (int e = 0; e < nc; ++e) {
+        const int lin = ijk[e * 3] + N * (ijk[e * 3 + 1] + (dim == 3 ? N * ijk[e * 3 + 2] : 0));
+        lookup[lin] = e;
+    }
+    nbr.assign((size_t)nc * 2 * dim, 0);
+    for (int e = 0; e < nc; ++e)
+        for (int f = 0; f < 2 * dim; ++f) {
+            int c[3] = {ijk[e * 3], ijk[e * 3 + 1], ijk[e * 3 + 2]};
+            const int k = f / 2;
+            c[k] = (c[k] + (f % 2 ? 1 : N - 1)) % N;
+            nbr[(size_t)e * 2 * dim + f] = lookup[c[0] + N * (c[1] + (dim == 3 ? N * c[2] : 0))];
+        }
+}
+
+} // namespace dgx
+
+using namespace dgx;
+
+#define HIPCHK(call)                                                                     \
+    do {                                                                                 \
+        hipError_t _e = (call);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            c->err = std::string(#call) + ": " + hipGetErrorString(_e);                   \
+            return DGX_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+static int fail(dgx_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+extern "C" {
+
+void dgx_default_params(dgx_params *p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->dim = 3;
+    p->poly_degree = 2;
+    p->flux_nodes = DGX_GLL;
+    p->use_split_form = 0;
+    p->two_point_flux = DGX_KG;
+    p->conv_flux = DGX_LAX_FRIEDRICHS;
+    p->pde = DGX_EULER;
+    p->diss_flux = DGX_SIPG;
+    p->use_constant_viscosity = 0;
+    p->gamma_gas = 1.4;
+    p->mach_inf = 0.1;
+    p->reynolds_inf = 1600.0;
+    p->prandtl = 0.72;
+    p->constant_viscosity = 1.0;
+    p->temperature_inf = 273.15;
+    p->sipg_penalty_factor = 1.0;
+    p->n_cells_1d = 4;
+    p->domain_left = 0.0;
+    p->domain_right = 6.28318530717958623200;
+    p->cell_order = DGX_ORDER_DEALII;
+    p->precision = DGX_FP64;
+}
+
+int dgx_create(const dgx_params *p, dgx_ctx **out)
+{
+    if (!p || !out) return DGX_ERR_INVALID;
+    *out = nullptr;
+    dgx_ctx *c = new dgx_ctx();
+    c->P = *p;
+    auto bad = [&](int code, const std::string &m) {
+        std::fprintf(stderr, "dgx_create: %s\n", m.c_str());
+        delete c;
+        return code;
+    };
+    if (p->dim != 2 && p->dim != 3) return bad(DGX_ERR_INVALID, "dim must be 2 or 3");
+    if (p->poly_degree < 1 || p->poly_degree > 6) return bad(DGX_ERR_UNSUPPORTED, "device path supports p = 1..6");
+    if (p->flux_nodes != DGX_GLL) return bad(DGX_ERR_UNSUPPORTED, "device path supports collocated GLL flux nodes only");
+    if (p->n_cells_1d < 2) return bad(DGX_ERR_INVALID, "n_cells_1d must be >= 2");
+    if (p->conv_flux < 0 || p->conv_flux > 5) return bad(DGX_ERR_INVALID, "unknown conv_flux");
+    if (p->conv_flux >= 3 && !p->use_split_form)
+        return bad(DGX_ERR_INVALID, "two point flux and not using split form are not compatible");
+    if (p->pde == DGX_NAVIER_STOKES && p->dim != 3) return bad(DGX_ERR_UNSUPPORTED, "Navier-Stokes device path is 3D");
+    if (p->precision != DGX_FP64 && p->precision != DGX_MIXED) return bad(DGX_ERR_INVALID, "unknown precision");
+    hipGetDevice(&c->device);
+    c->dim = p->dim;
+    c->p = p->poly_degree;
+    c->n = c->p + 1;
+    c->Np = (c->dim == 3) ? c->n * c->n * c->n : c->n * c->n;
+    c->ns = c->dim + 2;
+    const int N = p->n_cells_1d;
+    c->ncells = (c->dim == 3) ? N * N * N : N * N;
+    c->ndofs = (int64_t)c->ncells * c->ns * c->Np;
+    c->h = (p->domain_right - p->domain_left) / N;
+    c->ops = build_ops(c->n);
+    PhysConst &pc = c->pc;
+    pc.gam = p->gamma_gas;
+    pc.gamm1 = p->gamma_gas - 1.0;
+    pc.mach2 = p->mach_inf * p->mach_inf;
+    pc.reynolds = p->reynolds_inf;
+    pc.prandtl = p->prandtl;
+    pc.mu_const = p->constant_viscosity;
+    pc.temperature_ratio = 110.4 / p->temperature_inf;
+    pc.use_constant_viscosity = p->use_constant_viscosity;
+    pc.two_point = p->two_point_flux;
+    pc.conv_flux = p->conv_flux;
+    pc.diss_flux = p->diss_flux;
+    std::vector<int> ijk;
+    cell_ijk(c->dim, N, p->cell_order, ijk);
+    c->cell_lat = ijk;
+    neighbors(c->dim, N, ijk, c->nbr_host);
+    const size_t bytes = sizeof(double) * (size_t)c->ndofs;
+    if (hipMalloc(&c->nbr_dev, sizeof(int) * c->nbr_host.size()) != hipSuccess ||
+        hipMalloc(&c->u_dev, bytes) != hipSuccess || hipMalloc(&c->r_dev, bytes) != hipSuccess)
+        return bad(DGX_ERR_HIP, "hipMalloc failed");
+    if (p->pde == DGX_NAVIER_STOKES)
+        for (int k = 0; k < c->dim; ++k)
+            if (hipMalloc(&c->aux_dev[k], bytes) != hipSuccess) return bad(DGX_ERR_HIP, "hipMalloc failed");
+    if (hipMemcpy(c->nbr_dev, c->nbr_host.data(), sizeof(int) * c->nbr_host.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bad(DGX_ERR_HIP, "hipMemcpy failed");
+    hipMemset(c->u_dev, 0, bytes);
+    hipMemset(c->r_dev, 0, bytes);
+    *out = c;
+    return DGX_OK;
+}
+
+int dgx_destroy(dgx_ctx *c)
+{
+    if (!c) return DGX_OK;
+    hipDeviceSynchronize();
+    hipFree(c->nbr_dev);
+    hipFree(c->u_dev);
+    hipFree(c->r_dev);
+    for (int k = 0; k < 3; ++k) hipFree(c->aux_dev[k]);
+    for (double *r : c->regs) hipFree(r);
+    hipFree(c->iq_ops);
+    hipFree(c->iq_partial);
+    if (c->timing.a) hipEventDestroy(c->timing.a);
+    if (c->timing.b) hipEventDestroy(c->timing.b);
+    delete c;
+    return DGX_OK;
+}
+
+const char *dgx_last_error(const dgx_ctx *c) { return c ? c->err.c_str() : "null context"; }
+int64_t dgx_n_dofs(const dgx_ctx *c) { return c ? c->ndofs : 0; }
+int dgx_n_cells(const dgx_ctx *c) { return c ? c->ncells : 0; }
+
+int dgx_set_solution(dgx_ctx *c, const double *host_u)
+{
+    if (!c || !host_u) return DGX_ERR_INVALID;
+    HIPCHK(hipMemcpyAsync(c->u_dev, host_u, sizeof(double) * c->ndofs, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DGX_OK;
+}
+
+int dgx_get_solution(const dgx_ctx *cc, double *host_u)
+{
+    dgx_ctx *c = const_cast<dgx_ctx *>(cc);
+    if (!c || !host_u) return DGX_ERR_INVALID;
+    HIPCHK(hipMemcpyAsync(host_u, c->u_dev, sizeof(double) * c->ndofs, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DGX_OK;
+}
+
+int dgx_get_right_hand_side(const dgx_ctx *cc, double *host_r)
+{
+    dgx_ctx *c = const_cast<dgx_ctx *>(cc);
+    if (!c || !host_r) return DGX_ERR_INVALID;
+    HIPCHK(hipMemcpyAsync(host_r, c->r_dev, sizeof(double) * c->ndofs, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DGX_OK;
+}
+
+int dgx_get_auxiliary_solution(const dgx_ctx *cc, int idim, double *host_q)
+{
+    dgx_ctx *c = const_cast<dgx_ctx *>(cc);
+    if (!c || !host_q || idim < 0 || idim >= c->dim) return DGX_ERR_INVALID;
+    if (!c->aux_dev[idim]) return fail(c, DGX_ERR_STATE, "no auxiliary solution (Euler)");
+    HIPCHK(hipMemcpyAsync(host_q, c->aux_dev[idim], sizeof(double) * c->ndofs, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DGX_OK;
+}
+
+int dgx_get_neighbors(const dgx_ctx *cc, int32_t *host_nbr)
+{
+    dgx_ctx *c = const_cast<dgx_ctx *>(cc);
+    if (!c || !host_nbr) return DGX_ERR_INVALID;
+    // copy back what the kernels read, so integer parity is checked on the device copy
+    HIPCHK(hipMemcpy(host_nbr, c->nbr_dev, sizeof(int) * c->nbr_host.size(), hipMemcpyDeviceToHost));
+    return DGX_OK;
+}
+
+static StageArgs base_args(dgx_ctx *c, const double *u)
+{
+    StageArgs a{};
+    a.u = u;
+    for (int k = 0; k < 3; ++k) a.aux[k] = c->aux_dev[k];
+    a.nbr = c->nbr_dev;
+    a.ncells = c->ncells;
+    a.inv_h = 1.0 / c->h;
+    const unsigned deg = (unsigned)c->p;
+    const double pen = (double)(deg * (deg + 1)) / c->h * c->P.sipg_penalty_factor; // dg_base.cpp:397-413
+    a.penalty = 0.5 * (pen + pen);
+    a.resid_scale = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
+    return a;
+}
+
+// right-hand side of the stage solution u (aux solve first for Navier-Stokes)
+static int residual_into(dgx_ctx *c, const double *u, double *out, int mode)
+{
+    if (c->P.pde == DGX_NAVIER_STOKES) {
+        int rc = launch_aux(c, u);
+        if (rc) return rc;
+    }
+    StageArgs a = base_args(c, u);
+    a.mode = mode;
+    a.out = out;
+    return launch_stage(c, a);
+}
+
+int dgx_assemble_residual(dgx_ctx *c)
+{
+    if (!c) return DGX_ERR_INVALID;
+    int rc = residual_into(c, c->u_dev, c->r_dev, MODE_RESIDUAL);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DGX_OK;
+}
+
+int dgx_assemble_residual_device(dgx_ctx *c, const double *u_dev, double *r_dev)
+{
+    if (!c || !u_dev || !r_dev) return DGX_ERR_INVALID;
+    return residual_into(c, u_dev, r_dev, MODE_RESIDUAL);
+}
+
+int dgx_rhs_device(dgx_ctx *c, const double *u_dev, double *dudt_dev)
+{
+    if (!c || !u_dev || !dudt_dev) return DGX_ERR_INVALID;
+    return residual_into(c, u_dev, dudt_dev, MODE_RHS);
+}
+
+int dgx_apply_inverse_mass_to_rhs(dgx_ctx *c, double *host_out)
+{
+    // M^-1 of the collocated GLL cell is diagonal: out_i = r_i / (W_i det J)
+    if (!c || !host_out) return DGX_ERR_INVALID;
+    std::vector<double> r((size_t)c->ndofs);
+    HIPCHK(hipMemcpy(r.data(), c->r_dev, sizeof(double) * c->ndofs, hipMemcpyDeviceToHost));
+    const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
+    for (int64_t q = 0; q < c->ndofs; ++q) {
+        const int node = (int)(q % c->Np);
+        const int a = node % c->n, b = (node / c->n) % c->n, d = node / (c->n * c->n);
+        double W = c->ops.w[a] * c->ops.w[b];
+        if (c->dim == 3) W *= c->ops.w[d];
+        host_out[q] = r[q] / (W * detJ);
+    }
+    return DGX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// explicit RK (Butcher) and low-storage RK tableaus
+// runge_kutta_methods.cpp:8-110, low_storage_runge_kutta_methods.cpp:11-220
+// ---------------------------------------------------------------------------
+struct Butcher { int s; double a[4][4], b[4]; };
+struct Lsrk { int s; bool plus; double gamma[11][3], beta[11], delta[11], bhat[11]; };
+
+static bool get_butcher(int m, Butcher &B)
+{
+    std::memset(&B, 0, sizeof(B));
+    switch (m) {
+    case DGX_EULER_EX: B.s = 1; B.b[0] = 1.0; return true;
+    case DGX_HEUN_EX: B.s = 2; B.a[1][0] = 1.0; B.b[0] = 0.5; B.b[1] = 0.5; return true;
+    case DGX_SSPRK3_EX:
+        B.s = 3; B.a[1][0] = 1.0; B.a[2][0] = 0.25; B.a[2][1] = 0.25;
+        B.b[0] = 1.0 / 6.0; B.b[1] = 1.0 / 6.0; B.b[2] = 2.0 / 3.0; return true;
+    case DGX_RK4_EX:
+        B.s = 4; B.a[1][0] = 0.5; B.a[2][1] = 0.5; B.a[3][2] = 1.0;
+        B.b[0] = 1.0 / 6.0; B.b[1] = 1.0 / 3.0; B.b[2] = 1.0 / 3.0; B.b[3] = 1.0 / 6.0; return true;
+    default: return false;
+    }
+}
+
+static bool get_lsrk(int m, Lsrk &L)
+{
+    std::memset(&L, 0, sizeof(L));
+    if (m == DGX_RK4_3_5_3SSTAR) {
+        const double g[6][3] = {{0.0, 0.0, 0.0}, {0.0, 1.0, 0.0},
+            {-0.497531095840104, 1.384996869124138, 0.0}, {1.010070514199942, 3.878155713328178, 0.0},
+            {-3.196559004608766, -2.324512951813145, 1.642598936063715},
+            {1.717835630267259, -0.514633322274467, 0.188295940828347}};
+        const double be[6] = {0.0, 0.075152045700771, 0.211361016946069, 1.100713347634329, 0.728537814675568, 0.393172889823198};
+        const double de[7] = {1.0, 0.081252332929194, -1.083849060586449, -1.096110881845602, 2.859440022030827,
+                              -0.655568367959557, -0.194421504490852};
+        L.s = 5; L.plus = false;
+        std::memcpy(L.gamma, g, sizeof(g)); std::memcpy(L.beta, be, sizeof(be)); std::memcpy(L.delta, de, sizeof(de));
+        return true;
+    }
+    if (m == DGX_RK3_2_5F_3SSTARPLUS) {
+        const double g[6][3] = {{0.0, 0.0, 0.0}, {0.0, 1.0, 0.0},
+            {0.2587771979725733308135192812685323706, 0.5528354909301389892439698870483746541, 0.0},
+            {-0.1324380360140723382965420909764953437, 0.6731871608203061824849561782794643600, 0.0},
+            {0.05056033948190826045833606441415585735, 0.2803103963297672407841316576323901761, 0.2752563273304676380891217287572780582},
+            {0.5670532000739313812633197158607642990, 0.5521525447020610386070346724931300367, -0.8950526174674033822276061734289327568}};
+        const double be[6] = {0.0, 0.2300298624518076223899418286314123354, 0.3021434166948288809034402119555380003,
+                              0.8025606185416310937583009085873554681, 0.4362158943603440930655148245148766471, 0.11292725304550591};
+        const double de[5] = {1.0, 0.34076558793345252, 0.34143826550033862, 0.72292753667879872, 0.0};
+        const double bh[6] = {0.094841667050357029, 0.17263713394303537, 0.39982431890843712, 0.17180168075801786,
+                              0.058819144221557401, 0.1020760551185952388626787099944507877};
+        L.s = 5; L.plus = true;
+        std::memcpy(L.gamma, g, sizeof(g)); std::memcpy(L.beta, be, sizeof(be));
+        std::memcpy(L.delta, de, sizeof(de)); std::memcpy(L.bhat, bh, sizeof(bh));
+        return true;
+    }
+    if (m == DGX_RK4_3_9F_3SSTARPLUS) {
+        const double g[10][3] = {{0.0, 0.0, 0.0}, {0.0, 1.0, 0.0},
+            {-4.655641447335068552684422206224169103, 2.499262792574495009336242992898153462, 0.0},
+            {-0.7720265099645871829248487209517314217, 0.5866820377718875577451517985847920081, 0.0},
+            {-4.024436690519806086742256154738379161, 1.205146086523094569925592464380295241, 0.7621006678721315291614677352949377871},
+            {-0.02129676284018530966221583708648634733, 0.3474793722186732780030762737753849272, -0.1981182504339400567765766904309673119},
+            {-2.435022509790109546199372365866450709, 1.321346060965113109321230804210670518, -0.6228959218699007450469629366684127462},
+            {0.01985627297131987000579523283542615256, 0.3119636464694193615946633676950358444, -0.3752248380775956442989480369774937099},
+            {-0.2810791146791038566946663374735713961, 0.4351419539684379261368971206040518552, -0.3355438309135169811915662336248989661},
+            {0.1689434168754859644351230590422137972, 0.2359698130028753572503744518147537768, -0.04560955005031121479972862973705108039}};
+        const double be[10] = {0.0, 0.2836343005184365275160654678626695428, 0.9736500104654741223716056170419660217, 0.3382359225242515288768487569778320563,
+                               -0.3584943611106183357043212309791897386, -0.004113944068471528211627210454497620358, 1.427968894048586363415504654313371031,
+                               0.01808470948394314017665968411915568633, 0.1605770645946802213926893453819236685, 0.2952227015964591648775833803635147962};
+        const double de[9] = {1.0, 1.262923876648114432874834923838556100, 0.7574967189685911558308119415539596711, 0.5163589453140728104667573195005629833,
+                              -0.02746327421802609557034437892013640319, -0.4382673178127944142238606608356542890, 1.273587294602656522645691372699677063,
+                              -0.6294740283927400326554066998751383342, 0.0};
+        const double bh[10] = {0.02483675912451591196775756814283216443, 0.1866327774562103796990092260942180726, 0.05671080795936984495604436622517631183,
+                               -0.003447695439149287702616943808570747099, 0.003602245056516636472203469198006404016, 0.4545570622145088936800484247980581766,
+                               -0.0002434665289427612407531544765622888855, 0.06642755361103549971517945063138312147, 0.1613697079523505006226025497715177578,
+                               0.04955424859358438183052504342394102722};
+        L.s = 9; L.plus = true;
+        std::memcpy(L.gamma, g, sizeof(g)); std::memcpy(L.beta, be, sizeof(be));
+        std::memcpy(L.delta, de, sizeof(de)); std::memcpy(L.bhat, bh, sizeof(bh));
+        return true;
+    }
+    if (m == DGX_RK5_4_10F_3SSTARPLUS) {
+        const double g[11][3] = {{0.0, 0.0, 0.0}, {0.0, 1.0, 0.0},
+            {0.4043660121685749695640462197806189975, 0.6871467028161416909922221357014564412, 0.0},
+            {-0.8503427289575839690883191973980814832, 1.093024748914750833700799552463885117, 0.0},
+            {-6.950894175262117526410215315179482885, 3.225975379607193001678365742708874597, -2.393405133244194727221124311276648940},
+            {0.9238765192731084931855438934978371889, 1.041153702510101386914019859778740444, -1.902854422421760920850597670305403139},
+            {-2.563178056509891340215942413817786020, 1.292821487912164945157744726076279306, -2.820042207399977261483046412236557428},
+            {0.2545744879365226143946122067064118430, 0.7391462755788122847651304143259254381, -1.832698465277380999601896111079977378},
+            {0.3125831707411998258746812355492206137, 0.1239129251371800313941948224441873274, -0.2199094483084671192328083958346519535},
+            {-0.7007114414440507927791249989236719346, 0.1842753472370123193132193302369345580, -0.4082430635847870963724591602173546218},
+            {0.4839621016023833375810172323297465039, 0.05712788998796583446479387686662738843, -0.1377669797880289713535665985132703979}};
+        const double be[11] = {0.0, 0.2597883554788674084039539165398464630, 0.01777008889438867858759149597539211023, 0.2481636629715501931294746189266601496,
+                               0.7941736871152005775821844297293296135, 0.3885391285642019129575902994397298066, 0.1455051657916305055730603387469193768,
+                               0.1587517385964749337690916959584348979, 0.1650605617880053419242434594242509601, 0.2118093284937153836908655490906875007,
+                               0.1559392342362059886106995325687547506};
+        const double de[10] = {1.0, -0.1331778419508803397033287009506932673, 0.8260422814750207498262063505871077303,
+                               1.513700425755728332485300719652378197, -1.305810059935023735972298885749903694, 3.036678802924163246003321318996156380,
+                               -1.449458274398895177922690618003584514, 3.834313899176362315089976408899373409, 4.122293760012985409330881631526514714,
+                               0.0};
+        const double bh[11] = {-0.02019255440012066080909442770590267512, 0.02737903480959184339932730854141598275, 0.3028818636145965534365173822296811090,
+                               -0.03656843880622222190071445247906780540, 0.3982664774676767729863101188528827405, -0.05715959421140685436681459970502471634,
+                               0.09849855103848558320961101178888983150, 0.06654601552456084978615342374581437947, 0.09073479542748112726465375642050504556,
+                               0.08432289325330803924891866923939606351, 0.04529095628204896774513180907141004447};
+        L.s = 10; L.plus = true;
+        std::memcpy(L.gamma, g, sizeof(g)); std::memcpy(L.beta, be, sizeof(be));
+        std::memcpy(L.delta, de, sizeof(de)); std::memcpy(L.bhat, bh, sizeof(bh));
+        return true;
+    }
+    return false;
+}
+
+int dgx_allocate_ode_system(dgx_ctx *c, int method)
+{
+    if (!c) return DGX_ERR_INVALID;
+    Butcher B;
+    Lsrk L;
+    for (double *r : c->regs) hipFree(r);
+    c->regs.clear();
+    int nregs;
+    if (get_butcher(method, B)) nregs = 3 + B.s;   // acc, stage x2, k_j (only if referenced)
+    else if (get_lsrk(method, L)) nregs = 4;        // S1', S2, S3, S4
+    else return fail(c, DGX_ERR_INVALID, "unknown ODE method");
+    const size_t bytes = sizeof(double) * (size_t)c->ndofs;
+    for (int r = 0; r < nregs; ++r) {
+        double *p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes));
+        HIPCHK(hipMemset(p, 0, bytes));
+        c->regs.push_back(p);
+    }
+    c->method = method;
+    c->lsrk_cur = 0;
+    return DGX_OK;
+}
+
+int dgx_set_track_error_estimate(dgx_ctx *c, int enable)
+{
+    if (!c) return DGX_ERR_INVALID;
+    c->track_error = enable != 0;
+    return DGX_OK;
+}
+
+static int step_butcher(dgx_ctx *c, const Butcher &B, double dt)
+{
+    // registers: 0 acc, 1/2 stage ping-pong, 3.. stored k_j
+    double *acc = c->regs[0];
+    double *stage_buf[2] = {c->regs[1], c->regs[2]};
+    bool need_k[4] = {false, false, false, false};
+    for (int j = 0; j < B.s; ++j)
+        for (int m = j + 2; m < B.s; ++m)
+            if (B.a[m][j] != 0.0) need_k[j] = true;
+    const double *cur = c->u_dev;
+    for (int i = 0; i < B.s; ++i) {
+        if (c->P.pde == DGX_NAVIER_STOKES) {
+            int rc = launch_aux(c, cur);
+            if (rc) return rc;
+        }
+        StageArgs a = base_args(c, cur);
+        a.mode = MODE_BUTCHER;
+        a.dt = dt;
+        a.u0 = c->u_dev;
+        a.acc = acc;
+        a.first_stage = (i == 0);
+        a.last_stage = (i == B.s - 1);
+        a.stage = i;
+        a.b_dt = dt * B.b[i];
+        a.k_store = need_k[i] ? c->regs[3 + i] : nullptr;
+        if (i < B.s - 1) {
+            a.u_next = stage_buf[i & 1];
+            for (int j = 0; j <= i; ++j) {
+                a.a_next[j] = B.a[i + 1][j];
+                a.k_prev[j] = (j < i && need_k[j]) ? c->regs[3 + j] : nullptr;
+                if (j < i && B.a[i + 1][j] != 0.0 && !need_k[j])
+                    return fail(c, DGX_ERR_INVALID, "tableau register bookkeeping");
+            }
+        }
+        int rc = launch_stage(c, a);
+        if (rc) return rc;
+        cur = stage_buf[i & 1];
+    }
+    std::swap(c->u_dev, c->regs[0]); // solution = solution_update
+    return DGX_OK;
+}
+
+static int step_lsrk(dgx_ctx *c, const Lsrk &L, double dt)
+{
+    // S1 ping-pong between u_dev and regs[0]; S2 = regs[1]; S3 = regs[2]; S4 = regs[3]
+    double *s1[2] = {c->u_dev, c->regs[0]};
+    double *S2 = c->regs[1], *S3 = c->regs[2], *S4 = c->regs[3];
+    const bool track = L.plus && c->track_error;
+    int cur = 0;
+    for (int i = 0; i < L.s; ++i) {
+        if (c->P.pde == DGX_NAVIER_STOKES) {
+            int rc = launch_aux(c, s1[cur]);
+            if (rc) return rc;
+        }
+        StageArgs a = base_args(c, s1[cur]);
+        a.mode = MODE_LSRK;
+        a.s1_out = s1[cur ^ 1];
+        a.s2 = S2;
+        a.s3 = S3;
+        a.s4 = track ? S4 : nullptr;
+        a.lsrk_first = (i == 0);
+        a.delta = L.delta[i];
+        a.g0 = L.gamma[i + 1][0];
+        a.g1 = L.gamma[i + 1][1];
+        a.g2 = L.gamma[i + 1][2];
+        a.beta = L.beta[i + 1];
+        a.bhat = L.bhat[i];
+        a.dt = dt;
+        int rc = launch_stage(c, a);
+        if (rc) return rc;
+        cur ^= 1;
+    }
+    if (track) {
+        // FSAL residual for the embedded estimate (low_storage_runge_kutta_ode_solver.cpp:63-69):
+        // S4 += bhat_s * dt * M^-1 R(S1); it does not change the solution.
+        if (c->P.pde == DGX_NAVIER_STOKES) {
+            int rc = launch_aux(c, s1[cur]);
+            if (rc) return rc;
+        }
+        StageArgs a = base_args(c, s1[cur]);
+        a.mode = MODE_ACCUM;
+        a.out = S4;
+        a.bhat = L.bhat[L.s];
+        a.dt = dt;
+        int rc = launch_stage(c, a);
+        if (rc) return rc;
+    }
+    if (cur == 1) std::swap(c->u_dev, c->regs[0]);
+    return DGX_OK;
+}
+
+int dgx_step_in_time(dgx_ctx *c, double dt, int n_steps)
+{
+    if (!c) return DGX_ERR_INVALID;
+    if (c->method < 0) return fail(c, DGX_ERR_STATE, "allocate_ode_system first");
+    Butcher B;
+    Lsrk L;
+    const bool isb = get_butcher(c->method, B);
+    if (!isb) get_lsrk(c->method, L);
+    for (int s = 0; s < n_steps; ++s) {
+        int rc = isb ? step_butcher(c, B, dt) : step_lsrk(c, L, dt);
+        if (rc) return rc;
+        ++c->iteration;
+        c->current_time += dt;
+    }
+    return DGX_OK;
+}
+
+double dgx_current_time(const dgx_ctx *c) { return c ? c->current_time : 0.0; }
+int64_t dgx_current_iteration(const dgx_ctx *c) { return c ? c->iteration : 0; }
+
+double dgx_constant_time_step(const dgx_ctx *c, double cfl)
+{
+    const unsigned ndofs_per_state = (unsigned)(c->ndofs / c->ns);
+    const double dx = (c->P.domain_right - c->P.domain_left) / std::pow((double)ndofs_per_state, (1.0 / c->dim));
+    return cfl * dx;
+}
+
+// Initial conditions interpolated at the GLL support points of each cell
+// (set_initial_condition.cpp:45-51): TGV (initial_condition_function.cpp:261-336),
+// isentropic vortex with parameterised strength/advection (:643-690).
+static void node_xyz(const dgx_ctx *c, std::vector<double> &xs, int e, int node, double *x)
+{
+    const int a = node % c->n, b = (node / c->n) % c->n, d = node / (c->n * c->n);
+    const int id[3] = {a, b, d};
+    for (int k = 0; k < 3; ++k)
+        x[k] = (k < c->dim) ? c->P.domain_left + (c->cell_lat[(size_t)e * 3 + k] + xs[id[k]]) * c->h : 0.0;
+}
+
+static int upload_ic(dgx_ctx *c, const std::vector<double> &u)
+{
+    HIPCHK(hipMemcpy(c->u_dev, u.data(), sizeof(double) * c->ndofs, hipMemcpyHostToDevice));
+    c->current_time = 0.0;
+    c->iteration = 0;
+    return DGX_OK;
+}
+
+int dgx_set_initial_condition_tgv(dgx_ctx *c, int isothermal)
+{
+    if (!c) return DGX_ERR_INVALID;
+    if (c->dim != 3) return fail(c, DGX_ERR_INVALID, "Taylor-Green vortex is 3D");
+    std::vector<double> xs(c->n), ws(c->n), u((size_t)c->ndofs);
+    gauss_lobatto(c->n, xs.data(), ws.data());
+    const double gam = c->P.gamma_gas, m2 = c->P.mach_inf * c->P.mach_inf;
+    for (int e = 0; e < c->ncells; ++e)
+        for (int i = 0; i < c->Np; ++i) {
+            double x[3];
+            node_xyz(c, xs, e, i, x);
+            const double p = 1.0 / (gam * m2) + (1.0 / 16.0) * (std::cos(2.0 * x[0]) + std::cos(2.0 * x[1])) *
+                                                    (std::cos(2.0 * x[2]) + 2.0);
+            const double rho = isothermal ? p * gam * m2 : 1.0;
+            const double vx = std::sin(x[0]) * std::cos(x[1]) * std::cos(x[2]);
+            const double vy = -std::cos(x[0]) * std::sin(x[1]) * std::cos(x[2]);
+            const double vz = 0.0;
+            const double cons[5] = {rho, rho * vx, rho * vy, rho * vz,
+                                    p / (gam - 1.0) + 0.5 * rho * (vx * vx + vy * vy + vz * vz)};
+            for (int s = 0; s < 5; ++s) u[((size_t)e * 5 + s) * c->Np + i] = cons[s];
+        }
+    return upload_ic(c, u);
+}
+
+int dgx_set_initial_condition_vortex(dgx_ctx *c, double strength, double u_inf, double v_inf)
+{
+    if (!c) return DGX_ERR_INVALID;
+    if (c->dim != 2) return fail(c, DGX_ERR_INVALID, "isentropic vortex is 2D");
+    std::vector<double> xs(c->n), ws(c->n), u((size_t)c->ndofs);
+    gauss_lobatto(c->n, xs.data(), ws.data());
+    const double gam = 1.4;
+    const double M = std::sqrt(2 / gam);
+    const double beta = M * strength * std::sqrt(2.0) / 4.0 / M_PI * std::exp(1.0 / 2.0);
+    for (int e = 0; e < c->ncells; ++e)
+        for (int i = 0; i < c->Np; ++i) {
+            double x[3];
+            node_xyz(c, xs, e, i, x);
+            const double Om = beta * std::exp(-0.5 * (x[0] * x[0] + x[1] * x[1]));
+            const double dT = -(gam - 1.0) / 2.0 * Om * Om;
+            const double rho = std::pow(1 + dT, 1.0 / (gam - 1.0));
+            const double vx = u_inf - x[1] * Om, vy = v_inf + x[0] * Om;
+            const double p = 1.0 / gam * std::pow(1 + dT, gam / (gam - 1.0));
+            const double cons[4] = {rho, rho * vx, rho * vy, p / (gam - 1.0) + 0.5 * rho * (vx * vx + vy * vy)};
+            for (int s = 0; s < 4; ++s) u[((size_t)e * 4 + s) * c->Np + i] = cons[s];
+        }
+    return upload_ic(c, u);
+}
+
+int dgx_integrated_quantities(dgx_ctx *c, int overint, double *out4)
+{
+    if (!c || !out4) return DGX_ERR_INVALID;
+    return launch_integrated(c, c->u_dev, overint, out4);
+}
+
+int dgx_set_stream(dgx_ctx *c, void *s)
+{
+    if (!c) return DGX_ERR_INVALID;
+    c->stream = (hipStream_t)s;
+    return DGX_OK;
+}
+
+int dgx_enable_timing(dgx_ctx *c, int enable)
+{
+    if (!c) return DGX_ERR_INVALID;
+    c->timing.enabled = enable != 0;
+    if (enable && !c->timing.a) {
+        HIPCHK(hipEventCreate(&c->timing.a));
+        HIPCHK(hipEventCreate(&c->timing.b));
+    }
+    c->timing.sum_ms = 0.0;
+    c->timing.n = 0;
+    return DGX_OK;
+}
+
+int dgx_kernel_timing(const dgx_ctx *c, double *avg_ms, int64_t *n)
+{
+    if (!c) return DGX_ERR_INVALID;
+    if (avg_ms) *avg_ms = c->timing.n ? c->timing.sum_ms / c->timing.n : 0.0;
+    if (n) *n = c->timing.n;
+    return DGX_OK;
+}
+
+} // extern "C"

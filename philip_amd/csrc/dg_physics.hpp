@@ -1,0 +1,325 @@
+// Device physics for the strong-form DG path: Euler fluxes, entropy variables,
+// two-point split fluxes, surface numerical fluxes, Navier-Stokes viscous flux.
+// T is the arithmetic type of the flux evaluation (double, or float in the
+// mixed-precision kernels).  Formulas follow the reference:
+//   src/physics/euler.cpp:177-937,1033-1062 ; src/physics/navier_stokes.cpp:240-385,700-830,1197-1350
+//   src/numerical_flux/convective_numerical_flux.cpp:34-480
+//   src/numerical_flux/viscous_numerical_flux.cpp:144-328
+#pragma once
+#ifndef DGX_HOST_TEST
+#include <hip/hip_runtime.h>
+#endif
+
+namespace dgx {
+
+struct PhysConst {
+    double gam, gamm1, mach2, reynolds, prandtl, mu_const, temperature_ratio;
+    int use_constant_viscosity;
+    int two_point;   // KG=0 IR=1 CH=2 RA=3
+    int conv_flux;   // LXF=0 ROE=1 CENTRAL=2 TWO_POINT=3 TP+LXF=4 TP+ROE=5
+    int diss_flux;   // SIPG=0 BR2=1
+};
+
+template <typename T> __device__ __forceinline__ T dsqrt(T x) { return sqrt(x); }
+template <typename T> __device__ __forceinline__ T dlog(T x) { return log(x); }
+template <typename T> __device__ __forceinline__ T dexp(T x) { return exp(x); }
+template <typename T> __device__ __forceinline__ T dpow(T x, T y) { return pow(x, y); }
+template <typename T> __device__ __forceinline__ T dabs(T x) { return fabs(x); }
+template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
+template <> __device__ __forceinline__ float dlog<float>(float x) { return logf(x); }
+template <> __device__ __forceinline__ float dexp<float>(float x) { return expf(x); }
+template <> __device__ __forceinline__ float dpow<float>(float x, float y) { return powf(x, y); }
+template <> __device__ __forceinline__ float dabs<float>(float x) { return fabsf(x); }
+
+// compute_pressure_templated (euler.cpp:486-500)
+template <typename T, int D>
+__device__ __forceinline__ T pressure(const T *u, T gamm1)
+{
+    const T rho = u[0];
+    T vel2 = T(0);
+    for (int k = 0; k < D; ++k) { const T v = u[1 + k] / rho; vel2 = vel2 + v * v; }
+    return gamm1 * (u[D + 1] - T(0.5) * rho * vel2);
+}
+
+// convective_flux (euler.cpp:915-937), component along direction fd only
+template <typename T, int D>
+__device__ __forceinline__ void convective_flux_dir(const T *u, int fd, T gamm1, T *f)
+{
+    const T rho = u[0];
+    const T p = pressure<T, D>(u, gamm1);
+    T vel[D];
+    for (int k = 0; k < D; ++k) vel[k] = u[1 + k] / rho;
+    const T e = u[D + 1] / u[0];
+    const T H = e + p / rho;
+    f[0] = u[1 + fd];
+    for (int vd = 0; vd < D; ++vd) f[1 + vd] = rho * vel[fd] * vel[vd];
+    f[1 + fd] += p;
+    f[D + 1] = rho * vel[fd] * H;
+}
+
+// compute_entropy_variables (euler.cpp:810-829)
+template <typename T, int D>
+__device__ __forceinline__ void entropy_variables(const T *u, const PhysConst &pc, T *v)
+{
+    const T gam = T(pc.gam), gamm1 = T(pc.gamm1);
+    const T rho = u[0];
+    const T p = pressure<T, D>(u, gamm1);
+    const T s = dlog<T>(p * dpow<T>(rho, -gam));
+    const T rho_theta = p / gamm1;
+    v[0] = (rho_theta * (gam + T(1) - s) - u[D + 1]) / rho_theta;
+    for (int k = 0; k < D; ++k) v[k + 1] = u[k + 1] / rho_theta;
+    v[D + 1] = -rho / rho_theta;
+}
+
+// compute_conservative_variables_from_entropy_variables (euler.cpp:832-853)
+template <typename T, int D>
+__device__ __forceinline__ void conservative_from_entropy(const T *v, const PhysConst &pc, T *u)
+{
+    const T gam = T(pc.gam), gamm1 = T(pc.gamm1);
+    T vv = T(0);
+    for (int k = 0; k < D; ++k) vv += v[k + 1] * v[k + 1];
+    const T s = gam - v[0] + T(0.5) * vv / v[D + 1];
+    const T rho_theta = dpow<T>((gamm1 / dpow<T>(-v[D + 1], gam)), T(1) / gamm1) * dexp<T>(-s / gamm1);
+    u[0] = -rho_theta * v[D + 1];
+    for (int k = 0; k < D; ++k) u[k + 1] = rho_theta * v[k + 1];
+    u[D + 1] = rho_theta * (T(1) - T(0.5) * vv / v[D + 1]);
+}
+
+// compute_ismail_roe_logarithmic_mean (euler.cpp:629-647)
+template <typename T>
+__device__ __forceinline__ T log_mean(T a, T b)
+{
+    const T zeta = a / b;
+    const T f = (zeta - T(1)) / (zeta + T(1));
+    const T uu = f * f;
+    T F;
+    if (uu < T(1.0e-2)) F = T(1) + uu / T(3) + uu * uu / T(5) + uu * uu * uu / T(7);
+    else F = dlog<T>(zeta) / T(2) / f;
+    return (a + b) / (T(2) * F);
+}
+
+// Primitive cache used by the pair loops: rho, velocity, pressure.
+template <typename T, int D> struct Prim { T rho, vel[D], p; };
+
+template <typename T, int D>
+__device__ __forceinline__ Prim<T, D> to_prim(const T *u, T gamm1)
+{
+    Prim<T, D> q;
+    q.rho = u[0];
+    for (int k = 0; k < D; ++k) q.vel[k] = u[1 + k] / u[0];
+    q.p = pressure<T, D>(u, gamm1);
+    return q;
+}
+
+// Two-point split fluxes, direction fd component (euler.cpp:587-807).
+// u1/u2 are the conservative states, q1/q2 their primitive caches.
+template <typename T, int D>
+__device__ __forceinline__ void two_point_flux_dir(const T *u1, const Prim<T, D> &q1, const T *u2,
+                                                   const Prim<T, D> &q2, int fd, const PhysConst &pc, T *f)
+{
+    const T gam = T(pc.gam), gamm1 = T(pc.gamm1);
+    if (pc.two_point == 2 || pc.two_point == 3) {
+        const bool ranocha = (pc.two_point == 3);
+        const T rho_log = log_mean<T>(q1.rho, q2.rho);
+        const T b1 = ranocha ? q1.rho / q1.p : q1.rho / (T(2) * q1.p);
+        const T b2 = ranocha ? q2.rho / q2.p : q2.rho / (T(2) * q2.p);
+        const T beta_log = log_mean<T>(b1, b2);
+        const T p_hat = ranocha ? T(0.5) * (q1.p + q2.p)
+                                : T(0.5) * (q1.rho + q2.rho) / (T(2) * T(0.5) * (b1 + b2));
+        T va[D];
+        T vsq = T(0);
+        for (int k = 0; k < D; ++k) {
+            va[k] = T(0.5) * (q1.vel[k] + q2.vel[k]);
+            vsq += (T(0.5) * (q1.vel[k] + q2.vel[k])) * (T(0.5) * (q1.vel[k] + q2.vel[k]));
+        }
+        T H = ranocha ? T(1) / (beta_log * gamm1) + vsq + T(2) * p_hat / rho_log
+                      : T(1) / (T(2) * beta_log * gamm1) + vsq + p_hat / rho_log;
+        for (int k = 0; k < D; ++k) H -= T(0.5) * (T(0.5) * (q1.vel[k] * q1.vel[k] + q2.vel[k] * q2.vel[k]));
+        f[0] = rho_log * va[fd];
+        for (int vd = 0; vd < D; ++vd) f[1 + vd] = rho_log * va[fd] * va[vd];
+        f[1 + fd] += p_hat;
+        f[D + 1] = rho_log * va[fd] * H;
+        if (ranocha) f[D + 1] -= (T(0.5) * (q1.p * q1.vel[fd] + q2.p * q2.vel[fd]));
+    } else if (pc.two_point == 1) {
+        // Ismail-Roe (euler.cpp:614-707)
+        T z1[D + 2], z2[D + 2];
+        z1[0] = dsqrt<T>(q1.rho / q1.p);
+        z2[0] = dsqrt<T>(q2.rho / q2.p);
+        for (int k = 0; k < D; ++k) { z1[1 + k] = z1[0] * q1.vel[k]; z2[1 + k] = z2[0] * q2.vel[k]; }
+        z1[D + 1] = dsqrt<T>(q1.rho * q1.p);
+        z2[D + 1] = dsqrt<T>(q2.rho * q2.p);
+        T avg[D + 2];
+        for (int s = 0; s < D + 2; ++s) avg[s] = T(0.5) * (z1[s] + z2[s]);
+        const T lm0 = log_mean<T>(z1[0], z2[0]);
+        const T lmE = log_mean<T>(z1[D + 1], z2[D + 1]);
+        const T mrho = avg[0] * lmE;
+        T mv[D];
+        for (int k = 0; k < D; ++k) mv[k] = avg[1 + k] / avg[0];
+        const T mp = avg[D + 1] / avg[0];
+        T mh = (gam + T(1)) * (lmE / lm0) + gamm1 * mp;
+        mh /= T(2) * gam;
+        mh *= gam / (mrho * gamm1);
+        T vsq = T(0);
+        for (int k = 0; k < D; ++k) vsq += mv[k] * mv[k];
+        mh += T(0.5) * vsq;
+        f[0] = mrho * mv[fd];
+        for (int vd = 0; vd < D; ++vd) f[1 + vd] = mrho * mv[fd] * mv[vd];
+        f[1 + fd] += mp;
+        f[D + 1] = mrho * mv[fd] * mh;
+    } else {
+        // Kennedy-Gruber (euler.cpp:587-611)
+        const T mrho = (q1.rho + q2.rho) / T(2);
+        const T mp = (q1.p + q2.p) / T(2);
+        T mv[D];
+        for (int k = 0; k < D; ++k) mv[k] = T(0.5) * (q1.vel[k] + q2.vel[k]);
+        const T me = ((u1[D + 1] / u1[0]) + (u2[D + 1] / u2[0])) / T(2);
+        f[0] = mrho * mv[fd];
+        for (int vd = 0; vd < D; ++vd) f[1 + vd] = mrho * mv[fd] * mv[vd];
+        f[1 + fd] += mp;
+        f[D + 1] = mrho * mv[fd] * me + mp * mv[fd];
+    }
+}
+
+// Roe-Pike dissipation with Harten's entropy fix along the unit normal n = sgn * e_fd
+// (convective_numerical_flux.cpp:216-232, :332-480)
+template <typename T, int D>
+__device__ __forceinline__ void roe_dissipation(const T *ui, const Prim<T, D> &L, const T *ue,
+                                                const Prim<T, D> &R, int fd, T sgn, const PhysConst &pc,
+                                                T *out)
+{
+    const T gam = T(pc.gam), gamm1 = T(pc.gamm1);
+    T n[D];
+    for (int k = 0; k < D; ++k) n[k] = (k == fd) ? sgn : T(0);
+    T vnL = T(0), vnR = T(0);
+    for (int k = 0; k < D; ++k) vnL += L.vel[k] * n[k];
+    for (int k = 0; k < D; ++k) vnR += R.vel[k] * n[k];
+    const T hL = (ui[D + 1] + L.p) / L.rho;
+    const T hR = (ue[D + 1] + R.p) / R.rho;
+    const T r = dsqrt<T>(R.rho / L.rho);
+    const T rp1 = r + T(1);
+    const T rho_ravg = r * L.rho;
+    T v_ravg[D];
+    for (int k = 0; k < D; ++k) v_ravg[k] = (r * R.vel[k] + L.vel[k]) / rp1;
+    const T h_ravg = (r * hR + hL) / rp1;
+    T vel2_ravg = T(0);
+    for (int k = 0; k < D; ++k) vel2_ravg = vel2_ravg + v_ravg[k] * v_ravg[k];
+    T vn_ravg = T(0);
+    for (int k = 0; k < D; ++k) vn_ravg += v_ravg[k] * n[k];
+    const T c2 = gamm1 * (h_ravg - T(0.5) * vel2_ravg);
+    T c_ravg = T(1e10);
+    if (c2 > T(0)) c_ravg = dsqrt<T>(c2);
+    T eig[3] = {dabs<T>(vn_ravg - c_ravg), dabs<T>(vn_ravg), dabs<T>(vn_ravg + c_ravg)};
+    const T cL = dsqrt<T>(L.p * gam / L.rho), cR = dsqrt<T>(R.p * gam / R.rho);
+    const T eL[3] = {dabs<T>(vnL - cL), dabs<T>(vnL), dabs<T>(vnL + cL)};
+    const T eR[3] = {dabs<T>(vnR - cR), dabs<T>(vnR), dabs<T>(vnR + cR)};
+    const T dp = R.p - L.p, drho = R.rho - L.rho;
+    const T dVn = vnR - vnL;
+    T dVt[D];
+    for (int k = 0; k < D; ++k) dVt[k] = (R.vel[k] - L.vel[k]) - dVn * n[k];
+    for (int e = 0; e < 3; ++e) {
+        const T a1 = dabs<T>(eig[e] - eL[e]), a2 = dabs<T>(eR[e] - eig[e]);
+        const T eps = a1 > a2 ? a1 : a2;
+        if (eig[e] < eps) eig[e] = T(0.5) * (eig[e] * eig[e] / eps + eps);
+    }
+    T co0 = eig[0] * (dp - rho_ravg * c_ravg * dVn) / (T(2) * c2);
+    T co1 = eig[1] * (drho - dp / c2);
+    T co2 = eig[1] * rho_ravg;
+    T co3 = eig[2] * (dp + rho_ravg * c_ravg * dVn) / (T(2) * c2);
+    T A[D + 2];
+    A[0] = co0;
+    for (int k = 0; k < D; ++k) A[1 + k] = co0 * (v_ravg[k] - c_ravg * n[k]);
+    A[D + 1] = co0 * (h_ravg - c_ravg * vn_ravg);
+    A[0] += co1;
+    for (int k = 0; k < D; ++k) A[1 + k] += co1 * v_ravg[k];
+    A[D + 1] += co1 * vel2_ravg * T(0.5);
+    T dvt_dot = T(0);
+    for (int k = 0; k < D; ++k) { A[1 + k] += co2 * dVt[k]; dvt_dot += v_ravg[k] * dVt[k]; }
+    A[D + 1] += co2 * dvt_dot;
+    A[0] += co3;
+    for (int k = 0; k < D; ++k) A[1 + k] += co3 * (v_ravg[k] + c_ravg * n[k]);
+    A[D + 1] += co3 * (h_ravg + c_ravg * vn_ravg);
+    for (int s = 0; s < D + 2; ++s) out[s] = T(-0.5) * A[s];
+}
+
+// NumericalFluxConvective::evaluate_flux = baseline + dissipation (convective_numerical_flux.cpp:34-54)
+// with the unit normal sgn * e_fd.
+template <typename T, int D>
+__device__ __forceinline__ void numerical_flux(const T *ui, const Prim<T, D> &L, const T *ue,
+                                               const Prim<T, D> &R, int fd, T sgn, const PhysConst &pc,
+                                               T *out)
+{
+    const T gamm1 = T(pc.gamm1);
+    const int cf = pc.conv_flux;
+    T base[D + 2];
+    if (cf >= 3) {
+        T f[D + 2];
+        two_point_flux_dir<T, D>(ui, L, ue, R, fd, pc, f);
+        for (int s = 0; s < D + 2; ++s) base[s] = f[s] * sgn;
+    } else {
+        T f1[D + 2], f2[D + 2];
+        convective_flux_dir<T, D>(ui, fd, gamm1, f1);
+        convective_flux_dir<T, D>(ue, fd, gamm1, f2);
+        for (int s = 0; s < D + 2; ++s) base[s] = (T(0.5) * (f1[s] + f2[s])) * sgn;
+    }
+    if (cf == 0 || cf == 4) {
+        const T gam = T(pc.gam);
+        const T ei = dabs<T>(L.vel[fd] * sgn) + dsqrt<T>(L.p * gam / L.rho);
+        const T ee = dabs<T>(R.vel[fd] * sgn) + dsqrt<T>(R.p * gam / R.rho);
+        const T lam = (ei > ee) ? ei : ee;
+        for (int s = 0; s < D + 2; ++s) out[s] = base[s] + T(-0.5) * lam * (ue[s] - ui[s]);
+    } else if (cf == 1 || cf == 5) {
+        T diss[D + 2];
+        roe_dissipation<T, D>(ui, L, ue, R, fd, sgn, pc, diss);
+        for (int s = 0; s < D + 2; ++s) out[s] = base[s] + diss[s];
+    } else {
+        for (int s = 0; s < D + 2; ++s) out[s] = base[s];
+    }
+}
+
+// dissipative_flux_templated (navier_stokes.cpp:1197-1220), direction fd component.
+// g is the conservative gradient g[s*D + k].
+template <typename T, int D>
+__device__ __forceinline__ void viscous_flux_dir(const T *u, const T *g, int fd, const PhysConst &pc, T *f)
+{
+    const T gam = T(pc.gam), gamm1 = T(pc.gamm1), mach2 = T(pc.mach2);
+    const T rho = u[0];
+    T vel[D];
+    for (int k = 0; k < D; ++k) vel[k] = u[1 + k] / rho;
+    const T p = pressure<T, D>(u, gamm1);
+    T pg[D + 2][D];
+    for (int k = 0; k < D; ++k) pg[0][k] = g[0 * D + k];
+    for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b) pg[1 + a][b] = (g[(1 + a) * D + b] - vel[a] * g[0 * D + b]) / rho;
+    for (int a = 0; a < D; ++a) {
+        pg[D + 1][a] = g[(D + 1) * D + a];
+        for (int b = 0; b < D; ++b) pg[D + 1][a] -= T(0.5) * (vel[b] * g[(1 + b) * D + a] + u[1 + b] * pg[1 + b][a]);
+        pg[D + 1][a] *= gamm1;
+    }
+    const T T_ = gam * mach2 * (p / rho);
+    T mu;
+    if (pc.use_constant_viscosity) mu = T(1) * T(pc.mu_const);
+    else {
+        const T tr = T(pc.temperature_ratio);
+        mu = ((T(1) + tr) / (T_ + tr)) * dpow<T>(T_, T(1.5));
+    }
+    const T mu_hat = mu / T(pc.reynolds);
+    T divv = T(0);
+    for (int k = 0; k < D; ++k) divv += T(0.5) * (pg[1 + k][k] + pg[1 + k][k]);
+    const T mu2 = (T(-2) / T(3)) * mu_hat;
+    // tau[a][fd] and tau[fd][a] (symmetric)
+    T tau_fd[D];
+    for (int a = 0; a < D; ++a) {
+        tau_fd[a] = T(2) * mu_hat * (T(0.5) * (pg[1 + a][fd] + pg[1 + fd][a]));
+        if (a == fd) tau_fd[a] += mu2 * divv;
+    }
+    const T kappa = mu_hat / (gamm1 * mach2 * T(pc.prandtl));
+    const T dT = (gam * mach2 * pg[D + 1][fd] - T_ * pg[0][fd]) / rho;
+    const T q = -kappa * dT;
+    f[0] = T(0);
+    for (int a = 0; a < D; ++a) f[1 + a] = -tau_fd[a];
+    T fe = T(0);
+    for (int a = 0; a < D; ++a) fe -= vel[a] * tau_fd[a];
+    f[D + 1] = fe + q;
+}
+
+} // namespace dgx

@@ -1,0 +1,323 @@
+// Phase-structured fused stage kernel.
+//
+// Per block: EPB whole elements.  Work items change shape between phases so
+// that every lane has work:
+//   A  node-parallel   load the stage state (FP64, coalesced), entropy
+//                      round trip for the split form, primitive cache and
+//                      (conservative/viscous) volume fluxes into LDS
+//   C  face-parallel   one item per (face, face node): neighbour trace from
+//                      HBM, surface numerical flux (reference owner
+//                      orientation: the lower cell index is "interior"),
+//                      viscous numerical flux, result into LDS per direction
+//   B/D per direction  pair-parallel two-point fluxes F#(u_a,u_b), a<b, each
+//                      pair evaluated once and used by both nodes (the
+//                      flux is symmetric); node-parallel skew-Hadamard sum
+//   E  node-parallel   divergence, face lift 1/w_0, 1/h scaling and the RK
+//                      register update in FP64
+// T is the flux arithmetic type: double (FP64 path) or float (mixed path,
+// FP32 fluxes and contractions, FP64 state and RK registers).
+#pragma once
+#include "dg_kernels.hpp"
+
+namespace dgx {
+
+template <int D, int N> struct Geo {
+    static constexpr int NP = (D == 3) ? N * N * N : N * N;
+    static constexpr int NS = D + 2;
+    static constexpr int NL = (D == 3) ? N * N : N;          // lines per direction = face nodes
+    static constexpr int NPAIR = N * (N - 1) / 2;
+    static constexpr int EPB = (NP >= 256) ? 1 : (256 / NP);
+    static constexpr int BLOCK = ((EPB * NP + 63) / 64) * 64;
+};
+
+// node index of position `pos` along direction k on line `line`
+template <int D, int N>
+__device__ __forceinline__ int line_node(int k, int line, int pos)
+{
+    if (D == 3) {
+        if (k == 0) return pos + N * line;
+        if (k == 1) return (line % N) + N * pos + N * N * (line / N);
+        return line + N * N * pos;
+    } else {
+        if (k == 0) return pos + N * line;
+        return line + N * pos;
+    }
+}
+
+template <int D, int N>
+__device__ __forceinline__ int node_line(int k, int i)
+{
+    if (D == 3) {
+        if (k == 0) return i / N;
+        if (k == 1) return (i % N) + N * (i / (N * N));
+        return i % (N * N);
+    } else {
+        if (k == 0) return i / N;
+        return i % N;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void pair_nodes(int p, int &a, int &b)
+{
+    int q = p;
+    a = 0;
+    while (q >= N - 1 - a) {
+        q -= N - 1 - a;
+        ++a;
+    }
+    b = a + 1 + q;
+}
+
+template <int N>
+__device__ __forceinline__ int pair_index(int a, int b)
+{
+    return a * (2 * N - a - 1) / 2 + (b - a - 1);
+}
+
+template <typename T> struct PhysT {
+    T gam, gamm1;
+};
+
+template <int D, int N, bool SPLIT, bool NSV, bool RT, typename T>
+__global__ void __launch_bounds__((Geo<D, N>::BLOCK))
+stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
+{
+    using G = Geo<D, N>;
+    constexpr int NP = G::NP, NS = G::NS, NL = G::NL, NPAIR = G::NPAIR, EPB = G::EPB;
+    constexpr int NFLUX = (!SPLIT || NSV) ? D : 1;
+    __shared__ T sU[EPB][NS][NP];                       // state for pairs and faces
+    __shared__ T sP[EPB][SPLIT ? D + 1 : 1][NP];          // velocity, pressure (split form)
+    __shared__ T sF[EPB][NFLUX][NS][NP];                // volume fluxes for the D-divergence
+    __shared__ T sFace[EPB][D][NS][NP];                 // face terms per face direction
+    __shared__ T sPF[EPB][SPLIT ? NL * NPAIR : 1][NS];  // pair fluxes of one direction
+
+    const int tid = threadIdx.x;
+    const int blk0 = blockIdx.x * EPB;
+    const T gamm1 = T(pc.gamm1);
+
+    // ---------------- phase A: node-parallel load -----------------------
+    const int le = tid / NP, i = tid % NP;
+    const int e = blk0 + le;
+    const bool active = (le < EPB) && (e < a.ncells);
+    double ud[NS];
+    if (active) {
+        for (int s = 0; s < NS; ++s) ud[s] = a.u[((long)e * NS + s) * NP + i];
+        T ut[NS];
+        for (int s = 0; s < NS; ++s) ut[s] = T(ud[s]);
+        if (SPLIT && RT) {
+            T tmp[NS];
+            for (int s = 0; s < NS; ++s) tmp[s] = ut[s];
+            roundtrip<T, D>(tmp, pc, ut);
+        }
+        for (int s = 0; s < NS; ++s) sU[le][s][i] = ut[s];
+        if (SPLIT) {
+            const T rho = ut[0];
+            for (int k = 0; k < D; ++k) sP[le][k][i] = ut[1 + k] / rho;
+            sP[le][D][i] = pressure<T, D>(ut, gamm1);
+        }
+        if (!SPLIT || NSV) {
+            T g[NS * D];
+            if (NSV)
+                for (int s = 0; s < NS; ++s)
+                    for (int d = 0; d < D; ++d) g[s * D + d] = T(a.aux[d][((long)e * NS + s) * NP + i]);
+            for (int k = 0; k < D; ++k) {
+                T f[NS];
+                for (int s = 0; s < NS; ++s) f[s] = T(0);
+                if (!SPLIT) {
+                    T uu[NS];
+                    for (int s = 0; s < NS; ++s) uu[s] = T(ud[s]);
+                    convective_flux_dir<T, D>(uu, k, gamm1, f);
+                }
+                if (NSV) {
+                    T fv[NS];
+                    viscous_flux_dir<T, D>(ut, g, k, pc, fv);
+                    for (int s = 0; s < NS; ++s) f[s] += fv[s];
+                }
+                for (int s = 0; s < NS; ++s) sF[le][k][s][i] = f[s];
+            }
+        }
+        for (int k = 0; k < D; ++k)
+            for (int s = 0; s < NS; ++s) sFace[le][k][s][i] = T(0);
+    }
+    __syncthreads();
+
+    // ---------------- phase C: face-parallel numerical fluxes -----------
+    constexpr int FITEMS = 2 * D * NL;
+    for (int t = tid; t < EPB * FITEMS; t += G::BLOCK) {
+        const int fle = t / FITEMS, r = t % FITEMS;
+        const int fe = blk0 + fle;
+        if (fe >= a.ncells) continue;
+        const int f = r / NL, fn = r % NL;
+        const int k = f / 2, side = f % 2;
+        const int fi = line_node<D, N>(k, fn, side ? N - 1 : 0);
+        const int nb = a.nbr[fe * 2 * D + f];
+        const int fj = line_node<D, N>(k, fn, side ? 0 : N - 1);
+        const T sgn = side ? T(1) : T(-1);
+        T us[NS], un[NS], unt[NS];
+        double und[NS];
+        for (int s = 0; s < NS; ++s) {
+            us[s] = sU[fle][s][fi];
+            und[s] = a.u[((long)nb * NS + s) * NP + fj];
+            un[s] = T(und[s]);
+        }
+        if (SPLIT && RT) roundtrip<T, D>(un, pc, unt);
+        else
+            for (int s = 0; s < NS; ++s) unt[s] = un[s];
+        const Prim<T, D> pa = to_prim<T, D>(us, gamm1);
+        const Prim<T, D> pb = to_prim<T, D>(unt, gamm1);
+        T fs[NS];
+        if (fe < nb) {
+            numerical_flux<T, D>(us, pa, unt, pb, k, sgn, pc, fs);
+        } else {
+            numerical_flux<T, D>(unt, pb, us, pa, k, -sgn, pc, fs);
+            for (int s = 0; s < NS; ++s) fs[s] = -fs[s];
+        }
+        T ft[NS];
+        for (int s = 0; s < NS; ++s) ft[s] = -fs[s];
+        if (!SPLIT)
+            for (int s = 0; s < NS; ++s) ft[s] += sgn * sF[fle][k][s][fi];
+        if (NSV) {
+            T ui[NS], gi[NS * D], gn[NS * D], jump[NS * D];
+            for (int s = 0; s < NS; ++s) {
+                ui[s] = T(a.u[((long)fe * NS + s) * NP + fi]);
+                for (int d = 0; d < D; ++d) {
+                    gi[s * D + d] = T(a.aux[d][((long)fe * NS + s) * NP + fi]);
+                    gn[s * D + d] = T(a.aux[d][((long)nb * NS + s) * NP + fj]);
+                    jump[s * D + d] = (d == k) ? (ui[s] - un[s]) * sgn : (ui[s] - un[s]) * T(0);
+                }
+            }
+            T fa[NS], fb[NS];
+            viscous_flux_dir<T, D>(ui, gi, k, pc, fa);
+            viscous_flux_dir<T, D>(un, gn, k, pc, fb);
+            T fvs[NS];
+            for (int s = 0; s < NS; ++s) fvs[s] = T(0.5) * (fa[s] * sgn + fb[s] * sgn);
+            if (pc.diss_flux == 0) {
+                T ja[NS], jb[NS];
+                viscous_flux_dir<T, D>(ui, jump, k, pc, ja);
+                viscous_flux_dir<T, D>(un, jump, k, pc, jb);
+                const T pen = T(a.penalty);
+                for (int s = 0; s < NS; ++s) fvs[s] = fvs[s] - pen * (T(0.5) * (ja[s] * sgn + jb[s] * sgn));
+            }
+            for (int s = 0; s < NS; ++s) ft[s] -= fvs[s];
+            if (SPLIT)
+                for (int s = 0; s < NS; ++s) ft[s] += sgn * fa[s];
+        }
+        for (int s = 0; s < NS; ++s) sFace[fle][k][s][fi] = ft[s];
+    }
+
+    // ---------------- phases B/D: volume ---------------------------------
+    int c[3] = {0, 0, 0};
+    T acc[NS];
+    for (int s = 0; s < NS; ++s) acc[s] = T(0);
+    if (active) node_coords<D, N>(i, c);
+    if (SPLIT) {
+        for (int k = 0; k < D; ++k) {
+            __syncthreads();  // previous direction's sPF consumed; phase C done
+            constexpr int PITEMS = NL * NPAIR;
+            for (int t = tid; t < EPB * PITEMS; t += G::BLOCK) {
+                const int ple = t / PITEMS, r = t % PITEMS;
+                if (blk0 + ple >= a.ncells) continue;
+                const int line = r / NPAIR, p = r % NPAIR;
+                int pa_, pb_;
+                pair_nodes<N>(p, pa_, pb_);
+                const int na = line_node<D, N>(k, line, pa_), nbn = line_node<D, N>(k, line, pb_);
+                T u1[NS], u2[NS];
+                Prim<T, D> q1, q2;
+                for (int s = 0; s < NS; ++s) {
+                    u1[s] = sU[ple][s][na];
+                    u2[s] = sU[ple][s][nbn];
+                }
+                q1.rho = u1[0];
+                q2.rho = u2[0];
+                for (int d = 0; d < D; ++d) {
+                    q1.vel[d] = sP[ple][d][na];
+                    q2.vel[d] = sP[ple][d][nbn];
+                }
+                q1.p = sP[ple][D][na];
+                q2.p = sP[ple][D][nbn];
+                T F[NS];
+                two_point_flux_dir<T, D>(u1, q1, u2, q2, k, pc, F);
+                for (int s = 0; s < NS; ++s) sPF[ple][r][s] = F[s];
+            }
+            __syncthreads();
+            if (active) {
+                const int line = node_line<D, N>(k, i);
+                const int ck = c[k];
+                for (int l = 0; l < N; ++l) {
+                    if (l == ck) continue;
+                    const int p = (ck < l) ? pair_index<N>(ck, l) : pair_index<N>(l, ck);
+                    const T sk = T(ops.Sk[ck][l]);
+                    for (int s = 0; s < NS; ++s) acc[s] -= sk * sPF[le][line * NPAIR + p][s];
+                }
+            }
+        }
+    } else {
+        __syncthreads();  // phase C done
+    }
+    if (!active) return;
+    if (!SPLIT || NSV) {
+        for (int k = 0; k < D; ++k)
+            for (int l = 0; l < N; ++l) {
+                const int j = node_with<D, N>(i, k, l);
+                const T dk = T(ops.D[c[k]][l]);
+                for (int s = 0; s < NS; ++s) acc[s] -= dk * sF[le][k][s][j];
+            }
+    }
+    const T inv_wb = T(ops.inv_wb);
+    for (int k = 0; k < D; ++k)
+        for (int s = 0; s < NS; ++s) acc[s] += inv_wb * sFace[le][k][s][i];
+
+    // ---------------- phase E: FP64 epilogue -----------------------------
+    double dudt[NS];
+    for (int s = 0; s < NS; ++s) dudt[s] = double(acc[s]) * a.inv_h;
+    const long base = (long)e * NS * NP + i;
+    if (a.mode == MODE_RHS) {
+        for (int s = 0; s < NS; ++s) a.out[base + s * NP] = dudt[s];
+    } else if (a.mode == MODE_RESIDUAL) {
+        double W = ops.w[c[0]] * ops.w[c[1]];
+        if (D == 3) W *= ops.w[c[2]];
+        for (int s = 0; s < NS; ++s) a.out[base + s * NP] = dudt[s] * (W * a.resid_scale);
+    } else if (a.mode == MODE_ACCUM) {
+        for (int s = 0; s < NS; ++s) {
+            const double r = dudt[s] * a.dt;
+            a.out[base + s * NP] = a.out[base + s * NP] + a.bhat * r;
+        }
+    } else if (a.mode == MODE_LSRK) {
+        for (int s = 0; s < NS; ++s) {
+            const long idx = base + s * NP;
+            const double s1 = ud[s];
+            const double s2 = a.lsrk_first ? 0.0 + a.delta * s1 : a.s2[idx] + a.delta * s1;
+            const double s3 = a.lsrk_first ? s1 : a.s3[idx];
+            double n1 = s1 * a.g0;
+            n1 += a.g1 * s2;
+            n1 += a.g2 * s3;
+            const double r = dudt[s] * a.dt;
+            n1 += a.beta * r;
+            a.s2[idx] = s2;
+            if (a.lsrk_first) a.s3[idx] = s1;
+            a.s1_out[idx] = n1;
+            if (a.s4) a.s4[idx] = (a.lsrk_first ? s1 : a.s4[idx]) + a.bhat * r;
+        }
+    } else {
+        for (int s = 0; s < NS; ++s) {
+            const long idx = base + s * NP;
+            const double kk = dudt[s];
+            double accv = a.first_stage ? a.u0[idx] : a.acc[idx];
+            accv += a.b_dt * kk;
+            a.acc[idx] = accv;
+            if (a.k_store) a.k_store[idx] = kk;
+            if (a.u_next) {
+                double st = 0.0;
+                for (int j2 = 0; j2 < a.stage; ++j2)
+                    if (a.a_next[j2] != 0.0) st += a.a_next[j2] * a.k_prev[j2][idx];
+                if (a.a_next[a.stage] != 0.0) st += a.a_next[a.stage] * kk;
+                st *= a.dt;
+                st += 1.0 * a.u0[idx];
+                a.u_next[idx] = st;
+            }
+        }
+    }
+}
+
+} // namespace dgx

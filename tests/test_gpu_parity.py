@@ -1,0 +1,220 @@
+"""Device path vs the CPU FP64 oracle on identical seeded inputs, through the C ABI.
+
+Bars (stated per check): integer connectivity bit-exact; FP64 residual
+||R_dev - R_oracle|| / ||R_oracle|| <= max(1e-12, 2 x floor) in the inf- and
+2-norms, where floor is how far the oracle's own residual moves when its input
+is perturbed by one ulp (measured per case: ~1e-14 for conservative forms,
+up to ~2.5e-12 for split forms at M=0.1, p>=3, whose FP64 evaluation is that
+ill-conditioned); time-stepped solutions <= max(1e-11, 4 x floor) with the floor
+measured the same way on the stepped oracle; the reference's golden
+viscous-TGV kinetic energy reproduced at its own 1e-10 tolerance.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import philip_amd as P
+
+pytestmark = pytest.mark.gpu
+
+TOL_RESIDUAL = 1e-12
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def perturbed(u, o, seed, amp=1e-3):
+    rng = np.random.default_rng(seed)
+    v = u.reshape(o.ncells, o.ns, o.Np).copy()
+    scale = np.abs(v).max(axis=(0, 2), keepdims=True)
+    v += amp * scale * rng.standard_normal(v.shape)
+    return v.ravel()
+
+
+def rel(a, b):
+    return max(np.abs(a - b).max() / np.abs(b).max(), np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def ulp_perturb(u, seed=0):
+    rng = np.random.default_rng(seed)
+    return u * (1.0 + rng.integers(-1, 2, u.size) * 2.0 ** -52)
+
+
+def residual_bar(o, u, r_ref):
+    """max(1e-12, 2 x the oracle's own 1-ulp input sensitivity) for this case."""
+    floor = rel(o.residual(ulp_perturb(u)), r_ref)
+    return max(TOL_RESIDUAL, 2.0 * floor), floor
+
+
+SCHEMES = {
+    "lxf": dict(conv_flux=O.LXF),
+    "roe": dict(conv_flux=O.ROE),
+    "central": dict(conv_flux=O.CENTRAL),
+    "ch_roe": dict(use_split_form=1, two_point_flux=O.CH, conv_flux=O.TWO_POINT_ROE),
+    "ch_lxf": dict(use_split_form=1, two_point_flux=O.CH, conv_flux=O.TWO_POINT_LXF),
+    "ra_ec": dict(use_split_form=1, two_point_flux=O.RA, conv_flux=O.TWO_POINT),
+    "ir_ec": dict(use_split_form=1, two_point_flux=O.IR, conv_flux=O.TWO_POINT),
+    "kg_roe": dict(use_split_form=1, two_point_flux=O.KG, conv_flux=O.TWO_POINT_ROE),
+}
+
+
+def make_pair(**kw):
+    o = O.Oracle(**kw)
+    d = P.DG(**kw)
+    return o, d
+
+
+def test_connectivity_bit_exact():
+    for dim, N in [(3, 4), (3, 8), (3, 3), (2, 8), (2, 6)]:
+        log(f"connectivity dim={dim} N={N}")
+        o, d = make_pair(dim=dim, poly_degree=1, n_cells_1d=N)
+        assert np.array_equal(d.neighbors(), o.neighbors())
+
+
+@pytest.mark.parametrize("scheme", list(SCHEMES))
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
+def test_residual_3d(scheme, p):
+    kw = dict(dim=3, poly_degree=p, n_cells_1d=4 if p <= 3 else 2, **SCHEMES[scheme])
+    log(f"residual 3d scheme={scheme} p={p}")
+    o, d = make_pair(**kw)
+    u = perturbed(o.ic_tgv(), o, seed=10 + p)
+    r_ref = o.residual(u)
+    d.set_solution(u)
+    r = d.assemble_residual()
+    e = rel(r, r_ref)
+    bar, floor = residual_bar(o, u, r_ref)
+    log(f"  rel err {e:.3e} (bar {bar:.1e}, oracle 1-ulp floor {floor:.1e})")
+    assert e <= bar
+    # lifted right-hand side (apply_inverse_global_mass_matrix)
+    assert rel(d.apply_inverse_mass_to_rhs(), o.inverse_mass(r_ref)) <= bar
+
+
+@pytest.mark.parametrize("scheme", ["lxf", "roe", "ch_roe", "ir_ec", "ra_ec"])
+@pytest.mark.parametrize("p", [2, 3, 4, 5, 6])
+def test_residual_2d_vortex(scheme, p):
+    kw = dict(dim=2, poly_degree=p, n_cells_1d=4, domain_left=-5.0, domain_right=5.0, mach_inf=1.195228609334,
+              **SCHEMES[scheme])
+    log(f"residual 2d scheme={scheme} p={p}")
+    o, d = make_pair(**kw)
+    u = perturbed(o.ic_vortex(strength=3.0, u_inf=0.7, v_inf=0.9), o, seed=p)
+    r_ref = o.residual(u)
+    d.set_solution(u)
+    e = rel(d.assemble_residual(), r_ref)
+    bar, floor = residual_bar(o, u, r_ref)
+    log(f"  rel err {e:.3e} (bar {bar:.1e}, oracle 1-ulp floor {floor:.1e})")
+    assert e <= bar
+
+
+@pytest.mark.parametrize("diss", [O.SIPG, O.BR2])
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("p", [2, 3, 5])
+def test_residual_navier_stokes(diss, split, p):
+    kw = dict(dim=3, poly_degree=p, n_cells_1d=4 if p <= 3 else 2, pde=O.NAVIER_STOKES, diss_flux=diss,
+              reynolds_inf=1600.0, prandtl=0.71, mach_inf=0.1,
+              **(dict(use_split_form=1, two_point_flux=O.RA, conv_flux=O.TWO_POINT_ROE) if split else
+                 dict(conv_flux=O.ROE)))
+    log(f"residual NS diss={diss} split={split} p={p}")
+    o, d = make_pair(**kw)
+    u = perturbed(o.ic_tgv(), o, seed=100 + p)
+    r_ref = o.residual(u)
+    d.set_solution(u)
+    r = d.assemble_residual()
+    for k in range(3):
+        assert rel(d.auxiliary_solution(k), o.auxiliary(k)) <= TOL_RESIDUAL
+    e = rel(r, r_ref)
+    bar, floor = residual_bar(o, u, r_ref)
+    log(f"  rel err {e:.3e} (bar {bar:.1e}, oracle 1-ulp floor {floor:.1e})")
+    assert e <= bar
+
+
+def test_residual_constant_viscosity_and_lexicographic_mesh():
+    kw = dict(dim=3, poly_degree=3, n_cells_1d=3, pde=O.NAVIER_STOKES, diss_flux=O.BR2,
+              use_constant_viscosity=1, constant_viscosity=1.0, reynolds_inf=1600.0, prandtl=0.71,
+              use_split_form=1, two_point_flux=O.RA, conv_flux=O.TWO_POINT_ROE)
+    o, d = make_pair(**kw)
+    u = perturbed(o.ic_tgv(), o, seed=7)
+    d.set_solution(u)
+    r_ref = o.residual(u)
+    bar, _ = residual_bar(o, u, r_ref)
+    assert rel(d.assemble_residual(), r_ref) <= bar
+
+
+@pytest.mark.parametrize("method", ["euler", "heun", "ssprk3", "rk4", "rk4_3_5_3sstar", "rk3_2_5f_3sstarplus",
+                                    "rk4_3_9f_3sstarplus", "rk5_4_10f_3sstarplus"])
+def test_time_stepping(method):
+    kw = dict(dim=3, poly_degree=3, n_cells_1d=4, use_split_form=1, two_point_flux=O.CH,
+              conv_flux=O.TWO_POINT_ROE)
+    log(f"stepping {method}")
+    o, d = make_pair(**kw)
+    u = o.ic_tgv()
+    d.set_solution(u)
+    dt = o.constant_time_step(0.1)
+    d.allocate_ode_system(method)
+    d.step_in_time(dt, 3)
+    up = ulp_perturb(u)
+    for _ in range(3):
+        o.step(method, dt, u)
+        o.step(method, dt, up)
+    floor = rel(up, u)
+    e = rel(d.solution(), u)
+    bar = max(1e-11, 4.0 * floor)
+    log(f"  rel err {e:.3e} (bar {bar:.1e}, oracle 1-ulp floor {floor:.1e})")
+    assert e <= bar
+    assert d.current_iteration == 3 and abs(d.current_time - 3 * dt) < 1e-15
+
+
+def test_lsrk_error_register_does_not_change_solution():
+    kw = dict(dim=3, poly_degree=2, n_cells_1d=4, use_split_form=1, two_point_flux=O.CH,
+              conv_flux=O.TWO_POINT_ROE)
+    d1, d2 = P.DG(**kw), P.DG(**kw)
+    o = O.Oracle(**kw)
+    u = o.ic_tgv()
+    for d, track in [(d1, False), (d2, True)]:
+        d.set_solution(u)
+        d.allocate_ode_system("rk3_2_5f_3sstarplus")
+        d.set_track_error_estimate(track)
+        d.step_in_time(0.01, 2)
+    assert np.array_equal(d1.solution(), d2.solution())
+
+
+def test_integrated_quantities_match_oracle():
+    o, d = make_pair(dim=3, poly_degree=3, n_cells_1d=4)
+    u = perturbed(o.ic_tgv(), o, seed=3)
+    d.set_solution(u)
+    qo, qd = o.integrated(u), d.integrated()
+    for k in qo:
+        assert abs(qd[k] - qo[k]) <= 1e-13 * abs(qo[k]), k
+
+
+def test_reference_golden_nsfr_gll_on_device():
+    """The reference's own golden value, reproduced by the device path end to end
+    (viscous_taylor_green_vortex_energy_check_nsfr_gll_flux_nodes_quick.prm:59-60,
+    taylor_green_vortex_energy_check.cpp:37-41, 1e-10 relative)."""
+    d = P.DG(dim=3, poly_degree=2, n_cells_1d=4, pde=P.NAVIER_STOKES, diss_flux=P.SIPG, mach_inf=0.1,
+             reynolds_inf=1600.0, prandtl=0.71, use_split_form=1, two_point_flux=P.IR, conv_flux=P.TWO_POINT)
+    o = O.Oracle(dim=3, poly_degree=2, n_cells_1d=4)
+    # initial condition from the oracle (test input); the device integrates and reduces
+    o2 = O.Oracle(dim=3, poly_degree=2, n_cells_1d=4, pde=O.NAVIER_STOKES, mach_inf=0.1)
+    d.set_solution(o2.ic_tgv())
+    dt = d.constant_time_step(0.003)
+    nsteps = d.run("ssprk3", dt, 1.2566370614400000e-02)
+    assert nsteps == 9
+    q = d.integrated()
+    log(f"  KE {q['kinetic_energy']!r} eps {q['vorticity_dissipation']!r}")
+    assert abs(q["kinetic_energy"] - 1.2071670877794241e-01) / 1.2071670877794241e-01 < 1e-10
+    assert abs(q["vorticity_dissipation"] - 4.5392245903930087e-04) / 4.5392245903930087e-04 < 1e-10
+    del o
+
+
+def test_invalid_configurations_raise():
+    with pytest.raises(P.DeviceError):
+        P.DG(dim=3, poly_degree=2, flux_nodes=P.GL)
+    with pytest.raises(P.DeviceError):
+        P.DG(dim=3, poly_degree=2, conv_flux=P.TWO_POINT, use_split_form=0)
+    with pytest.raises(P.DeviceError):
+        P.DG(dim=3, poly_degree=9)
