@@ -1,11 +1,8 @@
-// Fused strong-form DG right-hand-side kernels for collocated Gauss-Lobatto
-// nodes on periodic Cartesian hexahedra/quadrilaterals, with the explicit RK
-// stage update folded into the epilogue.
-//
-// One thread owns one solution node; a block holds EPB whole elements so the
-// element's state is read from HBM once, staged in LDS for the line
-// contractions / two-point pair loops, and the face coupling reads only the
-// neighbour's face layer.  Reference path restated:
+// Shared pieces of the strong-form DG kernels for collocated Gauss-Lobatto
+// nodes on periodic Cartesian hexahedra/quadrilaterals: operator tables, stage
+// arguments, node indexing, the entropy round trip, and the auxiliary
+// (lifted-gradient) kernel.  The fused stage kernel is in dg_stage2.hpp.
+// Reference path restated:
 //   volume   src/dg/strong_dg.cpp:1017-1579   (conservative D-divergence or
 //            Hadamard skew two-point form, Chan 2019 eq. 15)
 //   faces    src/dg/strong_dg.cpp:2368-3576   (numerical flux lifted with
@@ -135,190 +132,6 @@ aux_kernel(const double *__restrict__ u, double *__restrict__ q0, double *__rest
             }
         }
         for (int s = 0; s < NS; ++s) qout[d][((long)e * NS + s) * NP + i] = g[s] * inv_h;
-    }
-}
-
-// Residual / stage kernel. SPLIT: two-point flux-differencing volume term;
-// NSV: Navier-Stokes viscous terms; RT: entropy projection round trip (exact
-// reference behaviour for the split form).
-template <int D, int N, bool SPLIT, bool NSV, bool RT>
-__global__ void __launch_bounds__((Shape<D, N>::BLOCK))
-stage_kernel(StageArgs a, Ops1D ops, PhysConst pc)
-{
-    using S = Shape<D, N>;
-    constexpr int NP = S::NP, NS = S::NS, EPB = S::EPB;
-    __shared__ double sU[EPB][NS][NP];
-    __shared__ double sF[EPB][D][NS][NP];
-    const int tid = threadIdx.x;
-    const int le = tid / NP, i = tid % NP;
-    const int e = blockIdx.x * EPB + le;
-    const bool active = (le < EPB) && (e < a.ncells);
-    const double gamm1 = pc.gamm1;
-
-    double ui[NS], ut[NS], qi[D][NS];
-    if (active) {
-        for (int s = 0; s < NS; ++s) ui[s] = a.u[((long)e * NS + s) * NP + i];
-        if (NSV)
-            for (int d = 0; d < D; ++d)
-                for (int s = 0; s < NS; ++s) qi[d][s] = a.aux[d][((long)e * NS + s) * NP + i];
-        if (SPLIT && RT) roundtrip<double, D>(ui, pc, ut);
-        else
-            for (int s = 0; s < NS; ++s) ut[s] = ui[s];
-        for (int s = 0; s < NS; ++s) sU[le][s][i] = SPLIT ? ut[s] : ui[s];
-        for (int k = 0; k < D; ++k) {
-            double f[NS];
-            for (int s = 0; s < NS; ++s) f[s] = 0.0;
-            if (!SPLIT) convective_flux_dir<double, D>(ui, k, gamm1, f);
-            if (NSV) {
-                double g[NS * D], fv[NS];
-                for (int s = 0; s < NS; ++s)
-                    for (int d = 0; d < D; ++d) g[s * D + d] = qi[d][s];
-                viscous_flux_dir<double, D>(SPLIT ? ut : ui, g, k, pc, fv);
-                for (int s = 0; s < NS; ++s) f[s] += fv[s];
-            }
-            for (int s = 0; s < NS; ++s) sF[le][k][s][i] = f[s];
-        }
-    }
-    __syncthreads();
-    if (!active) return;
-
-    int c[3];
-    node_coords<D, N>(i, c);
-    double acc[NS];
-    for (int s = 0; s < NS; ++s) acc[s] = 0.0;
-
-    // ---- volume -------------------------------------------------------
-    if (SPLIT) {
-        const Prim<double, D> qa = to_prim<double, D>(ut, gamm1);
-        for (int k = 0; k < D; ++k) {
-            for (int l = 0; l < N; ++l) {
-                if (l == c[k]) continue;
-                const int j = node_with<D, N>(i, k, l);
-                double uj[NS];
-                for (int s = 0; s < NS; ++s) uj[s] = sU[le][s][j];
-                const Prim<double, D> qb = to_prim<double, D>(uj, gamm1);
-                double F[NS];
-                two_point_flux_dir<double, D>(ut, qa, uj, qb, k, pc, F);
-                const double sk = ops.Sk[c[k]][l];
-                for (int s = 0; s < NS; ++s) acc[s] -= sk * F[s];
-            }
-        }
-    }
-    if (!SPLIT || NSV) {
-        for (int k = 0; k < D; ++k)
-            for (int l = 0; l < N; ++l) {
-                const int j = node_with<D, N>(i, k, l);
-                const double dk = ops.D[c[k]][l];
-                for (int s = 0; s < NS; ++s) acc[s] -= dk * sF[le][k][s][j];
-            }
-    }
-
-    // ---- faces --------------------------------------------------------
-    for (int k = 0; k < D; ++k) {
-        for (int side = 0; side < 2; ++side) {
-            if (c[k] != (side ? N - 1 : 0)) continue;
-            const int nb = a.nbr[e * 2 * D + 2 * k + side];
-            const int j = node_with<D, N>(i, k, side ? 0 : N - 1);
-            const double sgn = side ? 1.0 : -1.0;
-            double un[NS], unt[NS];
-            for (int s = 0; s < NS; ++s) un[s] = a.u[((long)nb * NS + s) * NP + j];
-            if (SPLIT && RT) roundtrip<double, D>(un, pc, unt);
-            else
-                for (int s = 0; s < NS; ++s) unt[s] = un[s];
-            const double *sa = SPLIT ? ut : ui;
-            const double *sb = SPLIT ? unt : un;
-            const Prim<double, D> pa = to_prim<double, D>(sa, gamm1);
-            const Prim<double, D> pb = to_prim<double, D>(sb, gamm1);
-            double fs[NS];
-            if (e < nb) {
-                numerical_flux<double, D>(sa, pa, sb, pb, k, sgn, pc, fs);
-            } else {
-                numerical_flux<double, D>(sb, pb, sa, pa, k, -sgn, pc, fs);
-                for (int s = 0; s < NS; ++s) fs[s] = -fs[s];
-            }
-            double ft[NS];
-            for (int s = 0; s < NS; ++s) ft[s] = -fs[s];
-            if (!SPLIT)
-                for (int s = 0; s < NS; ++s) ft[s] += sgn * sF[le][k][s][i];
-            if (NSV) {
-                double gi[NS * D], gn[NS * D], jump[NS * D];
-                for (int s = 0; s < NS; ++s)
-                    for (int d = 0; d < D; ++d) {
-                        gi[s * D + d] = qi[d][s];
-                        gn[s * D + d] = a.aux[d][((long)nb * NS + s) * NP + j];
-                        jump[s * D + d] = (d == k) ? (ui[s] - un[s]) * sgn : (ui[s] - un[s]) * 0.0;
-                    }
-                double fa[NS], fb[NS];
-                viscous_flux_dir<double, D>(ui, gi, k, pc, fa);
-                viscous_flux_dir<double, D>(un, gn, k, pc, fb);
-                double fvs[NS];
-                for (int s = 0; s < NS; ++s) fvs[s] = 0.5 * (fa[s] * sgn + fb[s] * sgn);
-                if (pc.diss_flux == 0) {
-                    double ja[NS], jb[NS];
-                    viscous_flux_dir<double, D>(ui, jump, k, pc, ja);
-                    viscous_flux_dir<double, D>(un, jump, k, pc, jb);
-                    for (int s = 0; s < NS; ++s) fvs[s] = fvs[s] - a.penalty * (0.5 * (ja[s] * sgn + jb[s] * sgn));
-                }
-                for (int s = 0; s < NS; ++s) ft[s] -= fvs[s];
-                if (SPLIT)
-                    for (int s = 0; s < NS; ++s) ft[s] += sgn * fa[s];
-            }
-            for (int s = 0; s < NS; ++s) acc[s] += ops.inv_wb * ft[s];
-        }
-    }
-
-    double dudt[NS];
-    for (int s = 0; s < NS; ++s) dudt[s] = acc[s] * a.inv_h;
-
-    // ---- epilogue -----------------------------------------------------
-    const long base = (long)e * NS * NP + i;
-    if (a.mode == MODE_RHS) {
-        for (int s = 0; s < NS; ++s) a.out[base + s * NP] = dudt[s];
-    } else if (a.mode == MODE_RESIDUAL) {
-        double W = ops.w[c[0]] * ops.w[c[1]];
-        if (D == 3) W *= ops.w[c[2]];
-        for (int s = 0; s < NS; ++s) a.out[base + s * NP] = dudt[s] * (W * a.resid_scale);
-    } else if (a.mode == MODE_ACCUM) {
-        // register += bhat * (dt * R)   (low_storage_runge_kutta_ode_solver.cpp:66-68)
-        for (int s = 0; s < NS; ++s) {
-            const double r = dudt[s] * a.dt;
-            a.out[base + s * NP] = a.out[base + s * NP] + a.bhat * r;
-        }
-    } else if (a.mode == MODE_LSRK) {
-        for (int s = 0; s < NS; ++s) {
-            const long idx = base + s * NP;
-            const double s1 = ui[s];
-            const double s2 = a.lsrk_first ? 0.0 + a.delta * s1 : a.s2[idx] + a.delta * s1;
-            const double s3 = a.lsrk_first ? s1 : a.s3[idx];
-            double n1 = s1 * a.g0;
-            n1 += a.g1 * s2;
-            n1 += a.g2 * s3;
-            const double r = dudt[s] * a.dt;
-            n1 += a.beta * r;
-            a.s2[idx] = s2;
-            if (a.lsrk_first) a.s3[idx] = s1;
-            a.s1_out[idx] = n1;
-            if (a.s4) a.s4[idx] = (a.lsrk_first ? s1 : a.s4[idx]) + a.bhat * r;
-        }
-    } else { // MODE_BUTCHER
-        for (int s = 0; s < NS; ++s) {
-            const long idx = base + s * NP;
-            const double k = dudt[s];
-            double accv = a.first_stage ? a.u0[idx] : a.acc[idx];
-            accv += a.b_dt * k;
-            a.acc[idx] = accv;
-            if (a.k_store) a.k_store[idx] = k;
-            if (a.u_next) {
-                // rk_stage[i+1] = (sum_j a_{i+1,j} k_j) * dt + u_n  (runge_kutta_ode_solver.cpp:15-33)
-                double st = 0.0;
-                for (int j2 = 0; j2 < a.stage; ++j2)
-                    if (a.a_next[j2] != 0.0) st += a.a_next[j2] * a.k_prev[j2][idx];
-                if (a.a_next[a.stage] != 0.0) st += a.a_next[a.stage] * k;
-                st *= a.dt;
-                st += 1.0 * a.u0[idx];
-                a.u_next[idx] = st;
-            }
-        }
     }
 }
 

@@ -261,19 +261,19 @@ __device__ __forceinline__ void numerical_flux(const T *ui, const Prim<T, D> &L,
         convective_flux_dir<T, D>(ue, fd, gamm1, f2);
         for (int s = 0; s < D + 2; ++s) base[s] = (T(0.5) * (f1[s] + f2[s])) * sgn;
     }
+    // baseline + Riemann-solver dissipation; the central / entropy-conserving fluxes add a zero
+    // dissipation vector, as ZeroRiemannSolverDissipation does in the reference
+    T diss[D + 2];
+    for (int s = 0; s < D + 2; ++s) diss[s] = T(0);
     if (cf == 0 || cf == 4) {
         const T gam = T(pc.gam);
         const T ei = dabs<T>(L.vel[fd] * sgn) + dsqrt<T>(L.p * gam / L.rho);
         const T ee = dabs<T>(R.vel[fd] * sgn) + dsqrt<T>(R.p * gam / R.rho);
         const T lam = (ei > ee) ? ei : ee;
-        for (int s = 0; s < D + 2; ++s) out[s] = base[s] + T(-0.5) * lam * (ue[s] - ui[s]);
-    } else if (cf == 1 || cf == 5) {
-        T diss[D + 2];
-        roe_dissipation<T, D>(ui, L, ue, R, fd, sgn, pc, diss);
-        for (int s = 0; s < D + 2; ++s) out[s] = base[s] + diss[s];
-    } else {
-        for (int s = 0; s < D + 2; ++s) out[s] = base[s];
+        for (int s = 0; s < D + 2; ++s) diss[s] = T(-0.5) * lam * (ue[s] - ui[s]);
     }
+    if (cf == 1 || cf == 5) roe_dissipation<T, D>(ui, L, ue, R, fd, sgn, pc, diss);
+    for (int s = 0; s < D + 2; ++s) out[s] = base[s] + diss[s];
 }
 
 // dissipative_flux_templated (navier_stokes.cpp:1197-1220), direction fd component.

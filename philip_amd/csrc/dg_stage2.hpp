@@ -75,10 +75,6 @@ __device__ __forceinline__ int pair_index(int a, int b)
     return a * (2 * N - a - 1) / 2 + (b - a - 1);
 }
 
-template <typename T> struct PhysT {
-    T gam, gamm1;
-};
-
 template <int D, int N, bool SPLIT, bool NSV, bool RT, typename T>
 __global__ void __launch_bounds__((Geo<D, N>::BLOCK))
 stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
@@ -164,17 +160,22 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
         if (SPLIT && RT) roundtrip<T, D>(un, pc, unt);
         else
             for (int s = 0; s < NS; ++s) unt[s] = un[s];
-        const Prim<T, D> pa = to_prim<T, D>(us, gamm1);
-        const Prim<T, D> pb = to_prim<T, D>(unt, gamm1);
-        T fs[NS];
-        if (fe < nb) {
-            numerical_flux<T, D>(us, pa, unt, pb, k, sgn, pc, fs);
-        } else {
-            numerical_flux<T, D>(unt, pb, us, pa, k, -sgn, pc, fs);
-            for (int s = 0; s < NS; ++s) fs[s] = -fs[s];
+        // reference orientation: the lower cell index is the interior side; selected by value so
+        // the flux evaluation itself is not under a divergent branch
+        const bool own = fe < nb;
+        T ua[NS], ub[NS];
+        for (int s = 0; s < NS; ++s) {
+            ua[s] = own ? us[s] : unt[s];
+            ub[s] = own ? unt[s] : us[s];
         }
+        const T sg = own ? sgn : -sgn;
+        const Prim<T, D> pa = to_prim<T, D>(ua, gamm1);
+        const Prim<T, D> pb = to_prim<T, D>(ub, gamm1);
+        T fs[NS];
+        numerical_flux<T, D>(ua, pa, ub, pb, k, sg, pc, fs);
+        const T fsgn = own ? T(1) : T(-1);
         T ft[NS];
-        for (int s = 0; s < NS; ++s) ft[s] = -fs[s];
+        for (int s = 0; s < NS; ++s) ft[s] = -(fs[s] * fsgn);
         if (!SPLIT)
             for (int s = 0; s < NS; ++s) ft[s] += sgn * sF[fle][k][s][fi];
         if (NSV) {

@@ -86,6 +86,8 @@ def test_residual_3d(scheme, p):
     r_ref = o.residual(u)
     d.set_solution(u)
     r = d.assemble_residual()
+    # run-to-run determinism: no reductions or races, so two launches agree bitwise
+    assert np.array_equal(r, d.assemble_residual())
     e = rel(r, r_ref)
     bar, floor = residual_bar(o, u, r_ref)
     log(f"  rel err {e:.3e} (bar {bar:.1e}, oracle 1-ulp floor {floor:.1e})")
