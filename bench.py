@@ -7,7 +7,13 @@ vortex, p=3, 32^3 periodic hex cells (2,097,152 nodes x 5 states =
 (5 stages per step).  The state and all RK registers are resident in HBM before
 the timed region; a "step" is one RK step of the whole mesh.
 
-Metric: DOF-updates/s per RK stage = DOF x stages x steps / wall time.
+Time step: the reference's TGV rule dt = CFL*L/(n_dofs/nstate)^(1/3) with CFL 0.01
+(the rule ignores the sound speed 1/M = 10; CFL 0.01 is an acoustic CFL of ~0.14,
+CFL 0.1 diverges within ten steps in the FP64 oracle as well).
+
+Metric: DOF-updates/s per RK stage = DOF x stages x steps / wall time.  The
+headline line is the mixed path (FP32 fluxes and contractions, FP64 state and
+RK registers); the FP64 path's throughput is reported beside it.
 Roofline: HBM-bound; algorithmic bytes per stage launch (FP64 registers of the
 fused 3S* stage) = read S1,S2,S3 + write S1',S2 = 40 B/DOF (32 B/DOF for the
 first stage, which writes S3 = S1 and starts S2 from zero).
@@ -39,7 +45,7 @@ def cpu_baseline(threads):
     o = O.Oracle(dim=3, poly_degree=3, n_cells_1d=32, use_split_form=1, two_point_flux=O.CH,
                  conv_flux=O.TWO_POINT_ROE, mach_inf=0.1)
     u = o.ic_tgv()
-    dt = o.constant_time_step(0.1)
+    dt = o.constant_time_step(0.01)
     t0 = time.perf_counter()
     o.step("rk3_2_5f_3sstarplus", dt, u)
     t = time.perf_counter() - t0
@@ -47,13 +53,40 @@ def cpu_baseline(threads):
                 sample=f"1 RK3(2)5F[3S*+] step (5 stages + FSAL residual) of the 32^3 p=3 mesh, {t:.1f} s")
 
 
+def run_gpu(precision, steps, warmup):
+    import philip_amd as P
+
+    prec = P.FP64 if precision == "fp64" else P.MIXED
+    d = P.DG(**workload_params(prec))
+    d.set_initial_condition_tgv()
+    method = "rk3_2_5f_3sstarplus"
+    stages = 5
+    dt = d.constant_time_step(0.01)
+    d.allocate_ode_system(method)
+    d.step_in_time(dt, warmup)
+    d.solution()  # blocking copy: synchronizes the context stream
+    t0 = time.perf_counter()
+    d.step_in_time(dt, steps)
+    d.synchronize()
+    t = time.perf_counter() - t0
+    # per-launch kernel time with HIP events on the context stream
+    d.enable_timing(True)
+    d.step_in_time(dt, 2)
+    avg_ms, _ = d.kernel_timing()
+    d.enable_timing(False)
+    ke = d.integrated()["kinetic_energy"]
+    return dict(value=d.ndofs * stages * steps / t, ms_per_step=1e3 * t / steps, avg_ms=avg_ms,
+                ndofs=d.ndofs, stages=stages, ke=ke)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-line", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -61,59 +94,36 @@ def main():
               "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
         sys.exit(2)
 
-    import philip_amd as P
-
-    prec = P.FP64 if args.precision == "fp64" else P.MIXED
-    d = P.DG(**workload_params(prec))
-    d.set_initial_condition_tgv()
-    method = "rk3_2_5f_3sstarplus"
-    stages = 5
-    dt = d.constant_time_step(0.1)
-    d.allocate_ode_system(method)
-    d.step_in_time(dt, args.warmup)
-    d.solution()  # synchronizes
-    t0 = time.perf_counter()
-    d.step_in_time(dt, args.steps)
-    _ = d.solution()[:1]  # device sync through the blocking copy of the context stream
-    t_total = time.perf_counter() - t0
-    # the blocking copy of the full vector is inside the window: time it separately and remove it
-    t0c = time.perf_counter()
-    d.solution()
-    t_copy = time.perf_counter() - t0c
-    t = max(t_total - t_copy, 1e-9)
-
-    ndofs = d.ndofs
-    value = ndofs * stages * args.steps / t
-    ms_per_step = 1e3 * t / args.steps
-
-    # per-launch kernel time with HIP events on the context stream
-    d.enable_timing(True)
-    d.step_in_time(dt, 2)
-    avg_ms, nlaunch = d.kernel_timing()
-    d.enable_timing(False)
+    r = run_gpu(args.precision, args.steps, args.warmup)
+    ndofs = r["ndofs"]
+    # first stage of a 3S* step moves 32 B/DOF, the other four 40 B/DOF
     bytes_per_launch = ndofs * (32 + 4 * 40) / 5.0
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-
+    achieved = bytes_per_launch / (r["avg_ms"] * 1e-3) / 1e9
     line = {
         "metric": "DOF-updates/s per RK stage (p=3 Euler TGV)",
-        "value": value,
+        "value": r["value"],
         "unit": "DOF-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == P.FP64 else "f32-flux/f64-state",
+        "dtype": "f64" if args.precision == "fp64" else "f32-flux/f64-state",
         "data": "synthetic (TGV initial condition, reference's closed form)",
         "config": {"workload": "Euler TGV p=3 32^3 hex, CH split + EC/Roe surface flux, GLL, RK3(2)5F[3S*+]",
-                   "dof": ndofs, "stages_per_step": stages, "precision": args.precision},
+                   "dof": ndofs, "stages_per_step": r["stages"], "precision": args.precision,
+                   "kinetic_energy_after_run": r["ke"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage_kernel<3,4,split>", "avg_launch_ms": avg_ms,
+                     "kernel": "stage2_kernel<3,4,split>", "avg_launch_ms": r["avg_ms"],
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
+    if args.precision == "mixed" and not args.no_fp64_line:
+        r64 = run_gpu("fp64", max(5, args.steps // 4), 2)
+        line["fp64_path"] = {"value": r64["value"], "avg_launch_ms": r64["avg_ms"],
+                             "roofline_frac": bytes_per_launch / (r64["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_threads)
     print(json.dumps(line), flush=True)

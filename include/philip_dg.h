@@ -142,6 +142,8 @@ double dgx_constant_time_step(const dgx_ctx *ctx, double cfl);
    [3]=vorticity-based dissipation rate; reductions in FP64 on the device. */
 int dgx_integrated_quantities(dgx_ctx *ctx, int overint, double *out4);
 
+/* Blocks until the context's stream has finished all launched work. */
+int dgx_synchronize(dgx_ctx *ctx);
 /* HIP stream the context launches on (as an opaque pointer); 0 = default stream. */
 int dgx_set_stream(dgx_ctx *ctx, void *hip_stream);
 /* Average duration (ms) of the last `dgx_step_in_time` stage kernels, measured with

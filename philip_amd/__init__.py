@@ -27,7 +27,7 @@ EXPORTED = [
     "dgx_assemble_residual_device", "dgx_rhs_device", "dgx_allocate_ode_system", "dgx_step_in_time",
     "dgx_current_time", "dgx_current_iteration", "dgx_set_track_error_estimate",
     "dgx_constant_time_step", "dgx_integrated_quantities", "dgx_set_stream", "dgx_enable_timing",
-    "dgx_kernel_timing", "dgx_set_initial_condition_tgv", "dgx_set_initial_condition_vortex",
+    "dgx_kernel_timing", "dgx_set_initial_condition_tgv", "dgx_set_initial_condition_vortex", "dgx_synchronize",
 ]
 
 
@@ -93,6 +93,7 @@ def load():
     lib.dgx_set_stream.argtypes = [vp, vp]
     lib.dgx_enable_timing.argtypes = [vp, i]
     lib.dgx_kernel_timing.argtypes = [vp, dp, P(i64)]
+    lib.dgx_synchronize.argtypes = [vp]
     _lib = lib
     return lib
 
@@ -156,6 +157,10 @@ class DG:
         v_inf = m * np.sin(np.pi / 4) if v_inf is None else v_inf
         self._chk(load().dgx_set_initial_condition_vortex(self.h, strength, u_inf, v_inf),
                   "set_initial_condition_vortex")
+
+    def synchronize(self):
+        """Wait for the context stream (a 1-element blocking copy)."""
+        self._chk(load().dgx_synchronize(self.h), "synchronize")
 
     def solution(self):
         u = np.empty(self.ndofs)

@@ -743,6 +743,13 @@ int dgx_integrated_quantities(dgx_ctx *c, int overint, double *out4)
     return launch_integrated(c, c->u_dev, overint, out4);
 }
 
+int dgx_synchronize(dgx_ctx *c)
+{
+    if (!c) return DGX_ERR_INVALID;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DGX_OK;
+}
+
 int dgx_set_stream(dgx_ctx *c, void *s)
 {
     if (!c) return DGX_ERR_INVALID;

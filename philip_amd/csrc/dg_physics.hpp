@@ -26,10 +26,16 @@ template <typename T> __device__ __forceinline__ T dexp(T x) { return exp(x); }
 template <typename T> __device__ __forceinline__ T dpow(T x, T y) { return pow(x, y); }
 template <typename T> __device__ __forceinline__ T dabs(T x) { return fabs(x); }
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
-template <> __device__ __forceinline__ float dlog<float>(float x) { return logf(x); }
-template <> __device__ __forceinline__ float dexp<float>(float x) { return expf(x); }
 template <> __device__ __forceinline__ float dpow<float>(float x, float y) { return powf(x, y); }
 template <> __device__ __forceinline__ float dabs<float>(float x) { return fabsf(x); }
+// Division: IEEE in double (the FP64 path reproduces the reference's operations);
+// reciprocal-multiply in the mixed path's float arithmetic.
+template <typename T> __device__ __forceinline__ T fdiv(T a, T b) { return a / b; }
+#ifndef DGX_HOST_TEST
+template <> __device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
+template <> __device__ __forceinline__ float dexp<float>(float x) { return __expf(x); }
+template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { return __fdividef(a, b); }
+#endif
 
 // compute_pressure_templated (euler.cpp:486-500)
 template <typename T, int D>
@@ -37,7 +43,7 @@ __device__ __forceinline__ T pressure(const T *u, T gamm1)
 {
     const T rho = u[0];
     T vel2 = T(0);
-    for (int k = 0; k < D; ++k) { const T v = u[1 + k] / rho; vel2 = vel2 + v * v; }
+    for (int k = 0; k < D; ++k) { const T v = fdiv<T>(u[1 + k], rho); vel2 = vel2 + v * v; }
     return gamm1 * (u[D + 1] - T(0.5) * rho * vel2);
 }
 
@@ -48,9 +54,9 @@ __device__ __forceinline__ void convective_flux_dir(const T *u, int fd, T gamm1,
     const T rho = u[0];
     const T p = pressure<T, D>(u, gamm1);
     T vel[D];
-    for (int k = 0; k < D; ++k) vel[k] = u[1 + k] / rho;
-    const T e = u[D + 1] / u[0];
-    const T H = e + p / rho;
+    for (int k = 0; k < D; ++k) vel[k] = fdiv<T>(u[1 + k], rho);
+    const T e = fdiv<T>(u[D + 1], u[0]);
+    const T H = e + fdiv<T>(p, rho);
     f[0] = u[1 + fd];
     for (int vd = 0; vd < D; ++vd) f[1 + vd] = rho * vel[fd] * vel[vd];
     f[1 + fd] += p;
@@ -89,13 +95,13 @@ __device__ __forceinline__ void conservative_from_entropy(const T *v, const Phys
 template <typename T>
 __device__ __forceinline__ T log_mean(T a, T b)
 {
-    const T zeta = a / b;
-    const T f = (zeta - T(1)) / (zeta + T(1));
+    const T zeta = fdiv<T>(a, b);
+    const T f = fdiv<T>(zeta - T(1), zeta + T(1));
     const T uu = f * f;
     T F;
-    if (uu < T(1.0e-2)) F = T(1) + uu / T(3) + uu * uu / T(5) + uu * uu * uu / T(7);
-    else F = dlog<T>(zeta) / T(2) / f;
-    return (a + b) / (T(2) * F);
+    if (uu < T(1.0e-2)) F = T(1) + fdiv<T>(uu, T(3)) + fdiv<T>(uu * uu, T(5)) + fdiv<T>(uu * uu * uu, T(7));
+    else F = fdiv<T>(fdiv<T>(dlog<T>(zeta), T(2)), f);
+    return fdiv<T>(a + b, T(2) * F);
 }
 
 // Primitive cache used by the pair loops: rho, velocity, pressure.
@@ -106,7 +112,7 @@ __device__ __forceinline__ Prim<T, D> to_prim(const T *u, T gamm1)
 {
     Prim<T, D> q;
     q.rho = u[0];
-    for (int k = 0; k < D; ++k) q.vel[k] = u[1 + k] / u[0];
+    for (int k = 0; k < D; ++k) q.vel[k] = fdiv<T>(u[1 + k], u[0]);
     q.p = pressure<T, D>(u, gamm1);
     return q;
 }
@@ -121,19 +127,19 @@ __device__ __forceinline__ void two_point_flux_dir(const T *u1, const Prim<T, D>
     if (pc.two_point == 2 || pc.two_point == 3) {
         const bool ranocha = (pc.two_point == 3);
         const T rho_log = log_mean<T>(q1.rho, q2.rho);
-        const T b1 = ranocha ? q1.rho / q1.p : q1.rho / (T(2) * q1.p);
-        const T b2 = ranocha ? q2.rho / q2.p : q2.rho / (T(2) * q2.p);
+        const T b1 = ranocha ? fdiv<T>(q1.rho, q1.p) : fdiv<T>(q1.rho, T(2) * q1.p);
+        const T b2 = ranocha ? fdiv<T>(q2.rho, q2.p) : fdiv<T>(q2.rho, T(2) * q2.p);
         const T beta_log = log_mean<T>(b1, b2);
         const T p_hat = ranocha ? T(0.5) * (q1.p + q2.p)
-                                : T(0.5) * (q1.rho + q2.rho) / (T(2) * T(0.5) * (b1 + b2));
+                                : fdiv<T>(T(0.5) * (q1.rho + q2.rho), T(2) * T(0.5) * (b1 + b2));
         T va[D];
         T vsq = T(0);
         for (int k = 0; k < D; ++k) {
             va[k] = T(0.5) * (q1.vel[k] + q2.vel[k]);
             vsq += (T(0.5) * (q1.vel[k] + q2.vel[k])) * (T(0.5) * (q1.vel[k] + q2.vel[k]));
         }
-        T H = ranocha ? T(1) / (beta_log * gamm1) + vsq + T(2) * p_hat / rho_log
-                      : T(1) / (T(2) * beta_log * gamm1) + vsq + p_hat / rho_log;
+        T H = ranocha ? fdiv<T>(T(1), beta_log * gamm1) + vsq + fdiv<T>(T(2) * p_hat, rho_log)
+                      : fdiv<T>(T(1), T(2) * beta_log * gamm1) + vsq + fdiv<T>(p_hat, rho_log);
         for (int k = 0; k < D; ++k) H -= T(0.5) * (T(0.5) * (q1.vel[k] * q1.vel[k] + q2.vel[k] * q2.vel[k]));
         f[0] = rho_log * va[fd];
         for (int vd = 0; vd < D; ++vd) f[1 + vd] = rho_log * va[fd] * va[vd];
@@ -143,8 +149,8 @@ __device__ __forceinline__ void two_point_flux_dir(const T *u1, const Prim<T, D>
     } else if (pc.two_point == 1) {
         // Ismail-Roe (euler.cpp:614-707)
         T z1[D + 2], z2[D + 2];
-        z1[0] = dsqrt<T>(q1.rho / q1.p);
-        z2[0] = dsqrt<T>(q2.rho / q2.p);
+        z1[0] = dsqrt<T>(fdiv<T>(q1.rho, q1.p));
+        z2[0] = dsqrt<T>(fdiv<T>(q2.rho, q2.p));
         for (int k = 0; k < D; ++k) { z1[1 + k] = z1[0] * q1.vel[k]; z2[1 + k] = z2[0] * q2.vel[k]; }
         z1[D + 1] = dsqrt<T>(q1.rho * q1.p);
         z2[D + 1] = dsqrt<T>(q2.rho * q2.p);
@@ -154,11 +160,11 @@ __device__ __forceinline__ void two_point_flux_dir(const T *u1, const Prim<T, D>
         const T lmE = log_mean<T>(z1[D + 1], z2[D + 1]);
         const T mrho = avg[0] * lmE;
         T mv[D];
-        for (int k = 0; k < D; ++k) mv[k] = avg[1 + k] / avg[0];
-        const T mp = avg[D + 1] / avg[0];
-        T mh = (gam + T(1)) * (lmE / lm0) + gamm1 * mp;
-        mh /= T(2) * gam;
-        mh *= gam / (mrho * gamm1);
+        for (int k = 0; k < D; ++k) mv[k] = fdiv<T>(avg[1 + k], avg[0]);
+        const T mp = fdiv<T>(avg[D + 1], avg[0]);
+        T mh = (gam + T(1)) * fdiv<T>(lmE, lm0) + gamm1 * mp;
+        mh = fdiv<T>(mh, T(2) * gam);
+        mh *= fdiv<T>(gam, mrho * gamm1);
         T vsq = T(0);
         for (int k = 0; k < D; ++k) vsq += mv[k] * mv[k];
         mh += T(0.5) * vsq;
@@ -168,11 +174,11 @@ __device__ __forceinline__ void two_point_flux_dir(const T *u1, const Prim<T, D>
         f[D + 1] = mrho * mv[fd] * mh;
     } else {
         // Kennedy-Gruber (euler.cpp:587-611)
-        const T mrho = (q1.rho + q2.rho) / T(2);
-        const T mp = (q1.p + q2.p) / T(2);
+        const T mrho = fdiv<T>(q1.rho + q2.rho, T(2));
+        const T mp = fdiv<T>(q1.p + q2.p, T(2));
         T mv[D];
         for (int k = 0; k < D; ++k) mv[k] = T(0.5) * (q1.vel[k] + q2.vel[k]);
-        const T me = ((u1[D + 1] / u1[0]) + (u2[D + 1] / u2[0])) / T(2);
+        const T me = fdiv<T>(fdiv<T>(u1[D + 1], u1[0]) + fdiv<T>(u2[D + 1], u2[0]), T(2));
         f[0] = mrho * mv[fd];
         for (int vd = 0; vd < D; ++vd) f[1 + vd] = mrho * mv[fd] * mv[vd];
         f[1 + fd] += mp;
@@ -193,14 +199,14 @@ __device__ __forceinline__ void roe_dissipation(const T *ui, const Prim<T, D> &L
     T vnL = T(0), vnR = T(0);
     for (int k = 0; k < D; ++k) vnL += L.vel[k] * n[k];
     for (int k = 0; k < D; ++k) vnR += R.vel[k] * n[k];
-    const T hL = (ui[D + 1] + L.p) / L.rho;
-    const T hR = (ue[D + 1] + R.p) / R.rho;
-    const T r = dsqrt<T>(R.rho / L.rho);
+    const T hL = fdiv<T>(ui[D + 1] + L.p, L.rho);
+    const T hR = fdiv<T>(ue[D + 1] + R.p, R.rho);
+    const T r = dsqrt<T>(fdiv<T>(R.rho, L.rho));
     const T rp1 = r + T(1);
     const T rho_ravg = r * L.rho;
     T v_ravg[D];
-    for (int k = 0; k < D; ++k) v_ravg[k] = (r * R.vel[k] + L.vel[k]) / rp1;
-    const T h_ravg = (r * hR + hL) / rp1;
+    for (int k = 0; k < D; ++k) v_ravg[k] = fdiv<T>(r * R.vel[k] + L.vel[k], rp1);
+    const T h_ravg = fdiv<T>(r * hR + hL, rp1);
     T vel2_ravg = T(0);
     for (int k = 0; k < D; ++k) vel2_ravg = vel2_ravg + v_ravg[k] * v_ravg[k];
     T vn_ravg = T(0);
@@ -209,7 +215,7 @@ __device__ __forceinline__ void roe_dissipation(const T *ui, const Prim<T, D> &L
     T c_ravg = T(1e10);
     if (c2 > T(0)) c_ravg = dsqrt<T>(c2);
     T eig[3] = {dabs<T>(vn_ravg - c_ravg), dabs<T>(vn_ravg), dabs<T>(vn_ravg + c_ravg)};
-    const T cL = dsqrt<T>(L.p * gam / L.rho), cR = dsqrt<T>(R.p * gam / R.rho);
+    const T cL = dsqrt<T>(fdiv<T>(L.p * gam, L.rho)), cR = dsqrt<T>(fdiv<T>(R.p * gam, R.rho));
     const T eL[3] = {dabs<T>(vnL - cL), dabs<T>(vnL), dabs<T>(vnL + cL)};
     const T eR[3] = {dabs<T>(vnR - cR), dabs<T>(vnR), dabs<T>(vnR + cR)};
     const T dp = R.p - L.p, drho = R.rho - L.rho;
@@ -219,12 +225,12 @@ __device__ __forceinline__ void roe_dissipation(const T *ui, const Prim<T, D> &L
     for (int e = 0; e < 3; ++e) {
         const T a1 = dabs<T>(eig[e] - eL[e]), a2 = dabs<T>(eR[e] - eig[e]);
         const T eps = a1 > a2 ? a1 : a2;
-        if (eig[e] < eps) eig[e] = T(0.5) * (eig[e] * eig[e] / eps + eps);
+        if (eig[e] < eps) eig[e] = T(0.5) * (fdiv<T>(eig[e] * eig[e], eps) + eps);
     }
-    T co0 = eig[0] * (dp - rho_ravg * c_ravg * dVn) / (T(2) * c2);
-    T co1 = eig[1] * (drho - dp / c2);
+    T co0 = fdiv<T>(eig[0] * (dp - rho_ravg * c_ravg * dVn), T(2) * c2);
+    T co1 = eig[1] * (drho - fdiv<T>(dp, c2));
     T co2 = eig[1] * rho_ravg;
-    T co3 = eig[2] * (dp + rho_ravg * c_ravg * dVn) / (T(2) * c2);
+    T co3 = fdiv<T>(eig[2] * (dp + rho_ravg * c_ravg * dVn), T(2) * c2);
     T A[D + 2];
     A[0] = co0;
     for (int k = 0; k < D; ++k) A[1 + k] = co0 * (v_ravg[k] - c_ravg * n[k]);
@@ -267,8 +273,8 @@ __device__ __forceinline__ void numerical_flux(const T *ui, const Prim<T, D> &L,
     for (int s = 0; s < D + 2; ++s) diss[s] = T(0);
     if (cf == 0 || cf == 4) {
         const T gam = T(pc.gam);
-        const T ei = dabs<T>(L.vel[fd] * sgn) + dsqrt<T>(L.p * gam / L.rho);
-        const T ee = dabs<T>(R.vel[fd] * sgn) + dsqrt<T>(R.p * gam / R.rho);
+        const T ei = dabs<T>(L.vel[fd] * sgn) + dsqrt<T>(fdiv<T>(L.p * gam, L.rho));
+        const T ee = dabs<T>(R.vel[fd] * sgn) + dsqrt<T>(fdiv<T>(R.p * gam, R.rho));
         const T lam = (ei > ee) ? ei : ee;
         for (int s = 0; s < D + 2; ++s) diss[s] = T(-0.5) * lam * (ue[s] - ui[s]);
     }
