@@ -79,6 +79,19 @@ def run_gpu(precision, steps, warmup):
                 ndofs=d.ndofs, stages=stages, ke=ke)
 
 
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of known size (read nbytes + write nbytes per launch): the
+    reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to HBM bytes."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": nbytes, "copy_calibration_bytes_written": nbytes,
+                      "launches": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,7 +101,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-line", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs a single MI355X "
               "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
@@ -117,7 +135,7 @@ def main():
                    "kinetic_energy_after_run": r["ke"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage2_kernel<3,4,split>", "avg_launch_ms": r["avg_ms"],
+                     "kernel": "line_kernel<3,4,mixed>", "avg_launch_ms": r["avg_ms"],
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
     if args.precision == "mixed" and not args.no_fp64_line:

@@ -26,6 +26,8 @@ struct Ops1D {
     double Sk[MAXN1D][MAXN1D];  // (Q - Q^T)[a][b] / w_a  with Q = W D
     double w[MAXN1D];           // GLL weights on [0,1]
     double inv_wb;              // 1 / w_0 (boundary weight)
+    float Skf[MAXN1D][MAXN1D];  // Sk rounded to float (mixed path)
+    float inv_wbf;              // inv_wb rounded to float
 };
 
 enum StageMode { MODE_RESIDUAL = 0, MODE_RHS = 1, MODE_LSRK = 2, MODE_BUTCHER = 3, MODE_ACCUM = 4 };
@@ -85,6 +87,69 @@ __device__ __forceinline__ void roundtrip(const T *u, const PhysConst &pc, T *ut
     T v[D + 2];
     entropy_variables<T, D>(u, pc, v);
     conservative_from_entropy<T, D>(v, pc, ut);
+}
+
+// Phase E of the stage kernels, in FP64: du/dt = acc / h and the output of the
+// selected mode (residual, lifted right-hand side, low-storage RK 3S* register
+// update, Butcher-tableau stage update, or the FSAL error-register accumulation).
+template <int D, int N, typename T>
+__device__ __forceinline__ void stage_epilogue(const StageArgs &a, const Ops1D &ops, int e, int i, const int *c,
+                                               const double *ud, const T *acc)
+{
+    constexpr int NP = Shape<D, N>::NP, NS = D + 2;
+    double dudt[NS];
+    for (int s = 0; s < NS; ++s) dudt[s] = double(acc[s]) * a.inv_h;
+    const long base = (long)e * NS * NP + i;
+    if (a.mode == MODE_RHS) {
+        for (int s = 0; s < NS; ++s) a.out[base + s * NP] = dudt[s];
+    } else if (a.mode == MODE_RESIDUAL) {
+        double W = ops.w[c[0]] * ops.w[c[1]];
+        if (D == 3) W *= ops.w[c[2]];
+        for (int s = 0; s < NS; ++s) a.out[base + s * NP] = dudt[s] * (W * a.resid_scale);
+    } else if (a.mode == MODE_ACCUM) {
+        for (int s = 0; s < NS; ++s) {
+            const double r = dudt[s] * a.dt;
+            a.out[base + s * NP] = a.out[base + s * NP] + a.bhat * r;
+        }
+    } else if (a.mode == MODE_LSRK) {
+        for (int s = 0; s < NS; ++s) {
+            const long idx = base + s * NP;
+            const double s1 = ud[s];
+            const double s2 = a.lsrk_first ? 0.0 + a.delta * s1 : a.s2[idx] + a.delta * s1;
+            const double s3 = a.lsrk_first ? s1 : a.s3[idx];
+            double n1 = s1 * a.g0;
+            n1 += a.g1 * s2;
+            n1 += a.g2 * s3;
+            const double r = dudt[s] * a.dt;
+            n1 += a.beta * r;
+            a.s2[idx] = s2;
+            if (a.lsrk_first) a.s3[idx] = s1;
+            a.s1_out[idx] = n1;
+            if (a.s4) a.s4[idx] = (a.lsrk_first ? s1 : a.s4[idx]) + a.bhat * r;
+        }
+    } else {
+        for (int s = 0; s < NS; ++s) {
+            const long idx = base + s * NP;
+            const double kk = dudt[s];
+            double accv = a.first_stage ? a.u0[idx] : a.acc[idx];
+            accv += a.b_dt * kk;
+            a.acc[idx] = accv;
+            if (a.k_store) a.k_store[idx] = kk;
+            if (a.u_next) {
+                // stage indices unrolled so the register file is never indexed per lane
+                double st = 0.0, a_cur = 0.0;
+#pragma unroll
+                for (int j2 = 0; j2 < 4; ++j2) {
+                    if (j2 < a.stage && a.a_next[j2] != 0.0) st += a.a_next[j2] * a.k_prev[j2][idx];
+                    if (j2 == a.stage) a_cur = a.a_next[j2];
+                }
+                if (a_cur != 0.0) st += a_cur * kk;
+                st *= a.dt;
+                st += 1.0 * a.u0[idx];
+                a.u_next[idx] = st;
+            }
+        }
+    }
 }
 
 // Lifted-gradient (auxiliary) equation, collocated GLL:

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "dg_host.hpp"
+#include "dg_line.hpp"
 #include "dg_stage2.hpp"
 
 namespace dgx {
@@ -34,12 +35,12 @@ static void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
         else if (!split && nsv)
             hipLaunchKernelGGL((stage2_kernel<D, N, false, true, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
         else if (split)
-            hipLaunchKernelGGL((stage2_kernel<D, N, true, false, RT, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
+            hipLaunchKernelGGL((line_kernel<D, N, RT, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
         else
             hipLaunchKernelGGL((stage2_kernel<D, N, false, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
     } else {
         if (split)
-            hipLaunchKernelGGL((stage2_kernel<D, N, true, false, RT, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
+            hipLaunchKernelGGL((line_kernel<D, N, RT, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
         else
             hipLaunchKernelGGL((stage2_kernel<D, N, false, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
     }

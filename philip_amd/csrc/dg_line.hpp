@@ -1,0 +1,289 @@
+// Line-structured fused stage kernel for the split (two-point) form of the
+// Euler equations on collocated GLL nodes.
+//
+// The skew-symmetric Hadamard volume term (strong_dg.cpp:1017-1579, Chan 2019
+// eq. 15) couples node a only with the nodes b on its own grid line in each
+// direction, so one work item per (element, direction K, line) evaluates every
+// two-point flux F#(u_a, u_b), a < b, of its line once, keeps the N nodal
+// sums of the line in registers (node a gets -Sk[a][b] F, node b gets
+// -Sk[b][a] F, the flux being symmetric), adds the two surface numerical
+// fluxes at the line's ends (strong_dg.cpp:2368-3576, lift 1/w_0), and writes
+// one partial result per node and direction to LDS.  The direction is a
+// template parameter of the item's code, so no register array is indexed per
+// lane.
+//
+// Per block: EPB whole elements.
+//   A  node-parallel  load the stage state (FP64, coalesced), entropy round
+//                     trip (FP64 path), per-node cache into LDS
+//   L  line-parallel  pairs + faces of one line, partial sums into LDS
+//   E  node-parallel  sum of the direction partials, FP64 epilogue
+// T = float: mixed path (per-node cache of derived quantities, float fluxes,
+// FP64 state and RK registers); T = double: FP64 path (the point physics of
+// dg_physics.hpp evaluated as in the reference).
+#pragma once
+#include <type_traits>
+
+#include "dg_fast.hpp"
+#include "dg_stage2.hpp"
+
+namespace dgx {
+
+template <int D, int N, typename T> struct LineGeo {
+    using G = Geo<D, N>;
+    static constexpr int NP = G::NP, NS = D + 2, NL = G::NL, EPB = G::EPB, BLOCK = G::BLOCK;
+    static constexpr bool FAST = std::is_same<T, float>::value;
+    // node cache fields: FAST u[NS], vel[D], p, b, |v|^2, sqrt(rho), c, h; FP64 u[NS], vel[D], p
+    static constexpr int NQ = FAST ? node_f_count<D>() : NS + D + 1;
+    // element strides padded to 1 mod 32 words so the stride-N lines of
+    // neighbouring elements fall in different LDS banks
+    static constexpr int pad1(int x) { return x + (33 - x % 32) % 32; }
+    static constexpr int ESQ = pad1(NQ * NP);
+    static constexpr int ESV = pad1(D * NS * NP);
+    static constexpr int ITEMS = EPB * D * NL;
+};
+
+// ---- mixed path: node cache in float ------------------------------------
+
+template <int D>
+__device__ __forceinline__ void nodef_store(float *q, int st, const NodeF<D> &n)
+{
+    constexpr int NS = D + 2;
+    for (int s = 0; s < NS; ++s) q[s * st] = n.u[s];
+    for (int k = 0; k < D; ++k) q[(NS + k) * st] = n.vel[k];
+    q[(NS + D) * st] = n.p;
+    q[(NS + D + 1) * st] = n.b;
+    q[(NS + D + 2) * st] = n.v2;
+    q[(NS + D + 3) * st] = n.sr;
+    q[(NS + D + 4) * st] = n.c;
+    q[(NS + D + 5) * st] = n.h;
+}
+
+template <int D>
+__device__ __forceinline__ void nodef_load(const float *q, int st, NodeF<D> &n)
+{
+    constexpr int NS = D + 2;
+    for (int s = 0; s < NS; ++s) n.u[s] = q[s * st];
+    for (int k = 0; k < D; ++k) n.vel[k] = q[(NS + k) * st];
+    n.p = q[(NS + D) * st];
+    n.b = q[(NS + D + 1) * st];
+    n.v2 = q[(NS + D + 2) * st];
+    n.sr = q[(NS + D + 3) * st];
+    n.c = q[(NS + D + 4) * st];
+    n.h = q[(NS + D + 5) * st];
+}
+
+template <int D>
+__device__ __forceinline__ void nodef_select(bool first, const NodeF<D> &x, const NodeF<D> &y, NodeF<D> &o)
+{
+    for (int s = 0; s < D + 2; ++s) o.u[s] = first ? x.u[s] : y.u[s];
+    for (int k = 0; k < D; ++k) o.vel[k] = first ? x.vel[k] : y.vel[k];
+    o.p = first ? x.p : y.p;
+    o.b = first ? x.b : y.b;
+    o.v2 = first ? x.v2 : y.v2;
+    o.sr = first ? x.sr : y.sr;
+    o.c = first ? x.c : y.c;
+    o.h = first ? x.h : y.h;
+}
+
+template <int D, int N, int K>
+__device__ __forceinline__ void line_work_f(const StageArgs &a, const Ops1D &ops, const PhysConst &pc,
+                                            const FastConst &fc, const float *sq, float *sv, int e, int line)
+{
+    constexpr int NP = Geo<D, N>::NP, NS = D + 2;
+    float acc[N][NS];
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+        for (int s = 0; s < NS; ++s) acc[p][s] = 0.f;
+#pragma unroll
+    for (int p0 = 0; p0 < N; ++p0) {
+#pragma unroll
+        for (int p1 = p0 + 1; p1 < N; ++p1) {
+            NodeF<D> A, B;
+            nodef_load<D>(sq + line_node<D, N>(K, line, p0), NP, A);
+            nodef_load<D>(sq + line_node<D, N>(K, line, p1), NP, B);
+            float F[NS];
+            pair_flux_fast<D, K>(A, B, fc, pc, F);
+            const float s01 = ops.Skf[p0][p1], s10 = ops.Skf[p1][p0];
+            for (int s = 0; s < NS; ++s) {
+                acc[p0][s] -= s01 * F[s];
+                acc[p1][s] -= s10 * F[s];
+            }
+        }
+    }
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int pos = side ? N - 1 : 0;
+        const int fi = line_node<D, N>(K, line, pos);
+        const int nb = a.nbr[e * 2 * D + 2 * K + side];
+        const int fj = line_node<D, N>(K, line, side ? 0 : N - 1);
+        const float sgn = side ? 1.f : -1.f;
+        NodeF<D> qi, qn, L, R;
+        nodef_load<D>(sq + fi, NP, qi);
+        for (int s = 0; s < NS; ++s) qn.u[s] = float(a.u[((long)nb * NS + s) * NP + fj]);
+        node_fill<D>(qn, fc);
+        // reference orientation: the lower cell index is the interior side
+        const bool own = e < nb;
+        nodef_select<D>(own, qi, qn, L);
+        nodef_select<D>(own, qn, qi, R);
+        float fs[NS];
+        num_flux_fast<D, K>(L, R, own ? sgn : -sgn, fc, pc, fs);
+        const float w = own ? -ops.inv_wbf : ops.inv_wbf;
+        for (int s = 0; s < NS; ++s) acc[pos][s] += w * fs[s];
+    }
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+        const int n = line_node<D, N>(K, line, p);
+        for (int s = 0; s < NS; ++s) sv[s * NP + n] = acc[p][s];
+    }
+}
+
+// ---- FP64 path ------------------------------------------------------------
+
+template <int D, int N, int K, bool RT>
+__device__ __forceinline__ void line_work_d(const StageArgs &a, const Ops1D &ops, const PhysConst &pc,
+                                            const double *sq, double *sv, int e, int line)
+{
+    constexpr int NP = Geo<D, N>::NP, NS = D + 2;
+    const double gamm1 = pc.gamm1;
+    double acc[N][NS];
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+        for (int s = 0; s < NS; ++s) acc[p][s] = 0.0;
+#pragma unroll
+    for (int p0 = 0; p0 < N; ++p0) {
+#pragma unroll
+        for (int p1 = p0 + 1; p1 < N; ++p1) {
+            const int na = line_node<D, N>(K, line, p0), nbn = line_node<D, N>(K, line, p1);
+            double u1[NS], u2[NS];
+            Prim<double, D> q1, q2;
+            for (int s = 0; s < NS; ++s) {
+                u1[s] = sq[s * NP + na];
+                u2[s] = sq[s * NP + nbn];
+            }
+            q1.rho = u1[0];
+            q2.rho = u2[0];
+            for (int d = 0; d < D; ++d) {
+                q1.vel[d] = sq[(NS + d) * NP + na];
+                q2.vel[d] = sq[(NS + d) * NP + nbn];
+            }
+            q1.p = sq[(NS + D) * NP + na];
+            q2.p = sq[(NS + D) * NP + nbn];
+            double F[NS];
+            two_point_flux_dir<double, D>(u1, q1, u2, q2, K, pc, F);
+            const double s01 = ops.Sk[p0][p1], s10 = ops.Sk[p1][p0];
+            for (int s = 0; s < NS; ++s) {
+                acc[p0][s] -= s01 * F[s];
+                acc[p1][s] -= s10 * F[s];
+            }
+        }
+    }
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int pos = side ? N - 1 : 0;
+        const int fi = line_node<D, N>(K, line, pos);
+        const int nb = a.nbr[e * 2 * D + 2 * K + side];
+        const int fj = line_node<D, N>(K, line, side ? 0 : N - 1);
+        const double sgn = side ? 1.0 : -1.0;
+        double us[NS], un[NS], unt[NS];
+        for (int s = 0; s < NS; ++s) {
+            us[s] = sq[s * NP + fi];
+            un[s] = a.u[((long)nb * NS + s) * NP + fj];
+        }
+        if (RT) roundtrip<double, D>(un, pc, unt);
+        else
+            for (int s = 0; s < NS; ++s) unt[s] = un[s];
+        const bool own = e < nb;
+        double ua[NS], ub[NS];
+        for (int s = 0; s < NS; ++s) {
+            ua[s] = own ? us[s] : unt[s];
+            ub[s] = own ? unt[s] : us[s];
+        }
+        const Prim<double, D> pa = to_prim<double, D>(ua, gamm1);
+        const Prim<double, D> pb = to_prim<double, D>(ub, gamm1);
+        double fs[NS];
+        numerical_flux<double, D>(ua, pa, ub, pb, K, own ? sgn : -sgn, pc, fs);
+        const double fsgn = own ? 1.0 : -1.0;
+        for (int s = 0; s < NS; ++s) acc[pos][s] += ops.inv_wb * (-(fs[s] * fsgn));
+    }
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+        const int n = line_node<D, N>(K, line, p);
+        for (int s = 0; s < NS; ++s) sv[s * NP + n] = acc[p][s];
+    }
+}
+
+template <int D, int N, bool RT, typename T>
+__global__ void __launch_bounds__((Geo<D, N>::BLOCK))
+line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
+{
+    using LG = LineGeo<D, N, T>;
+    constexpr int NP = LG::NP, NS = LG::NS, NL = LG::NL, EPB = LG::EPB;
+    constexpr bool FAST = LG::FAST;
+    __shared__ T sQ[EPB * LG::ESQ];   // per-node cache [element][field][node]
+    __shared__ T sV[EPB * LG::ESV];   // line partials [element][direction][state][node]
+
+    const int tid = threadIdx.x;
+    const int blk0 = blockIdx.x * EPB;
+    const FastConst fc = fast_const(pc);
+
+    // ---------------- phase A: node-parallel load -----------------------
+    const int le = tid / NP, i = tid % NP;
+    const int e = blk0 + le;
+    const bool active = (le < EPB) && (e < a.ncells);
+    double ud[NS];
+    if (active) {
+        for (int s = 0; s < NS; ++s) ud[s] = a.u[((long)e * NS + s) * NP + i];
+        T *q = sQ + le * LG::ESQ + i;
+        if constexpr (FAST) {
+            NodeF<D> n;
+            for (int s = 0; s < NS; ++s) n.u[s] = float(ud[s]);
+            node_fill<D>(n, fc);
+            nodef_store<D>(q, NP, n);
+        } else {
+            double ut[NS];
+            if (RT) roundtrip<double, D>(ud, pc, ut);
+            else
+                for (int s = 0; s < NS; ++s) ut[s] = ud[s];
+            for (int s = 0; s < NS; ++s) q[s * NP] = ut[s];
+            const double rho = ut[0];
+            for (int k = 0; k < D; ++k) q[(NS + k) * NP] = ut[1 + k] / rho;
+            q[(NS + D) * NP] = pressure<double, D>(ut, pc.gamm1);
+        }
+    }
+    __syncthreads();
+
+    // ---------------- phase L: line-parallel pairs and faces -------------
+    for (int t = tid; t < LG::ITEMS; t += LG::BLOCK) {
+        const int k = t / (EPB * NL), r = t % (EPB * NL);
+        const int lle = r / NL, line = r % NL;
+        const int le2 = blk0 + lle;
+        if (le2 >= a.ncells) continue;
+        const T *q = sQ + lle * LG::ESQ;
+        T *v = sV + lle * LG::ESV;
+        if constexpr (FAST) {
+            if (k == 0) line_work_f<D, N, 0>(a, ops, pc, fc, q, v, le2, line);
+            else if (k == 1) line_work_f<D, N, 1>(a, ops, pc, fc, q, v + NS * NP, le2, line);
+            else if (D == 3) line_work_f<D, N, (D == 3 ? 2 : 1)>(a, ops, pc, fc, q, v + 2 * NS * NP, le2, line);
+        } else {
+            if (k == 0) line_work_d<D, N, 0, RT>(a, ops, pc, q, v, le2, line);
+            else if (k == 1) line_work_d<D, N, 1, RT>(a, ops, pc, q, v + NS * NP, le2, line);
+            else if (D == 3) line_work_d<D, N, (D == 3 ? 2 : 1), RT>(a, ops, pc, q, v + 2 * NS * NP, le2, line);
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+
+    // ---------------- phase E: sum of direction partials, FP64 epilogue ---
+    T acc[NS];
+    const T *v = sV + le * LG::ESV + i;
+    for (int s = 0; s < NS; ++s) {
+        T x = v[s * NP];
+        for (int k = 1; k < D; ++k) x += v[(k * NS + s) * NP];
+        acc[s] = x;
+    }
+    int c[3] = {0, 0, 0};
+    node_coords<D, N>(i, c);
+    stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc);
+}
+
+} // namespace dgx

@@ -270,55 +270,7 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
         for (int s = 0; s < NS; ++s) acc[s] += inv_wb * sFace[le][k][s][i];
 
     // ---------------- phase E: FP64 epilogue -----------------------------
-    double dudt[NS];
-    for (int s = 0; s < NS; ++s) dudt[s] = double(acc[s]) * a.inv_h;
-    const long base = (long)e * NS * NP + i;
-    if (a.mode == MODE_RHS) {
-        for (int s = 0; s < NS; ++s) a.out[base + s * NP] = dudt[s];
-    } else if (a.mode == MODE_RESIDUAL) {
-        double W = ops.w[c[0]] * ops.w[c[1]];
-        if (D == 3) W *= ops.w[c[2]];
-        for (int s = 0; s < NS; ++s) a.out[base + s * NP] = dudt[s] * (W * a.resid_scale);
-    } else if (a.mode == MODE_ACCUM) {
-        for (int s = 0; s < NS; ++s) {
-            const double r = dudt[s] * a.dt;
-            a.out[base + s * NP] = a.out[base + s * NP] + a.bhat * r;
-        }
-    } else if (a.mode == MODE_LSRK) {
-        for (int s = 0; s < NS; ++s) {
-            const long idx = base + s * NP;
-            const double s1 = ud[s];
-            const double s2 = a.lsrk_first ? 0.0 + a.delta * s1 : a.s2[idx] + a.delta * s1;
-            const double s3 = a.lsrk_first ? s1 : a.s3[idx];
-            double n1 = s1 * a.g0;
-            n1 += a.g1 * s2;
-            n1 += a.g2 * s3;
-            const double r = dudt[s] * a.dt;
-            n1 += a.beta * r;
-            a.s2[idx] = s2;
-            if (a.lsrk_first) a.s3[idx] = s1;
-            a.s1_out[idx] = n1;
-            if (a.s4) a.s4[idx] = (a.lsrk_first ? s1 : a.s4[idx]) + a.bhat * r;
-        }
-    } else {
-        for (int s = 0; s < NS; ++s) {
-            const long idx = base + s * NP;
-            const double kk = dudt[s];
-            double accv = a.first_stage ? a.u0[idx] : a.acc[idx];
-            accv += a.b_dt * kk;
-            a.acc[idx] = accv;
-            if (a.k_store) a.k_store[idx] = kk;
-            if (a.u_next) {
-                double st = 0.0;
-                for (int j2 = 0; j2 < a.stage; ++j2)
-                    if (a.a_next[j2] != 0.0) st += a.a_next[j2] * a.k_prev[j2][idx];
-                if (a.a_next[a.stage] != 0.0) st += a.a_next[a.stage] * kk;
-                st *= a.dt;
-                st += 1.0 * a.u0[idx];
-                a.u_next[idx] = st;
-            }
-        }
-    }
+    stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc);
 }
 
 } // namespace dgx
