@@ -86,7 +86,9 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
     for _ in range(reps):
-        y.copy_(x)
+        y.copy_(x)          # runtime copy kernel
+    for _ in range(reps):
+        torch.mul(x, 2.0, out=y)  # elementwise kernel, same bytes
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes_read": nbytes, "copy_calibration_bytes_written": nbytes,
                       "launches": reps}), flush=True)

@@ -241,7 +241,9 @@ int dgx_create(const dgx_params *p, dgx_ctx **out)
         return bad(DGX_ERR_INVALID, "two point flux and not using split form are not compatible");
     if (p->pde == DGX_NAVIER_STOKES && p->dim != 3) return bad(DGX_ERR_UNSUPPORTED, "Navier-Stokes device path is 3D");
     if (p->precision != DGX_FP64 && p->precision != DGX_MIXED) return bad(DGX_ERR_INVALID, "unknown precision");
-    hipGetDevice(&c->device);
+    if (hipGetDevice(&c->device) != hipSuccess ||
+        hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
+        return bad(DGX_ERR_HIP, "no HIP device");
     c->dim = p->dim;
     c->p = p->poly_degree;
     c->n = c->p + 1;

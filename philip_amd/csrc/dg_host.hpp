@@ -62,6 +62,7 @@ struct dgx_ctx {
     std::string err;
     dgx::Timing timing;
     int device = 0;
+    int n_cu = 0;                  // compute units of the device (persistent-kernel grid size)
     // integrated-quantity operators (QGauss(p+1+overint))
     int iq_overint = -1;
     double *iq_ops = nullptr;      // B [nx][n], G [nx][n], w [nx]

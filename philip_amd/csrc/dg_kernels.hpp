@@ -64,6 +64,21 @@ template <int D, int N> struct Shape {
     static constexpr int BLOCK = ((EPB * NP + 63) / 64) * 64;
 };
 
+// XCD-aware block order.  Workgroups are dispatched to the 8 XCDs of the chip
+// round-robin (hardware block b runs on XCD b % 8), and each XCD has its own
+// L2.  Remapping so that XCD x processes a contiguous range of logical blocks
+// gives every XCD a compact region of the (Morton- or lexicographically
+// ordered) mesh, so the face neighbours an element reads were mostly loaded
+// into the same L2 by nearby blocks.  A bijection for any grid size.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block()
+{
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q = nb / NUM_XCD, r = nb % NUM_XCD;
+    const int xcd = b % NUM_XCD, slot = b / NUM_XCD;
+    return (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
 template <int D, int N>
 __device__ __forceinline__ void node_coords(int i, int *c)
 {
@@ -94,7 +109,8 @@ __device__ __forceinline__ void roundtrip(const T *u, const PhysConst &pc, T *ut
 // update, Butcher-tableau stage update, or the FSAL error-register accumulation).
 template <int D, int N, typename T>
 __device__ __forceinline__ void stage_epilogue(const StageArgs &a, const Ops1D &ops, int e, int i, const int *c,
-                                               const double *ud, const T *acc)
+                                               const double *ud, const T *acc, bool use_pf = false,
+                                               const double *pf2 = nullptr, const double *pf3 = nullptr)
 {
     constexpr int NP = Shape<D, N>::NP, NS = D + 2;
     double dudt[NS];
@@ -115,8 +131,9 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs &a, const Ops1D &
         for (int s = 0; s < NS; ++s) {
             const long idx = base + s * NP;
             const double s1 = ud[s];
-            const double s2 = a.lsrk_first ? 0.0 + a.delta * s1 : a.s2[idx] + a.delta * s1;
-            const double s3 = a.lsrk_first ? s1 : a.s3[idx];
+            // use_pf: S2, S3 already loaded by the caller into pf2 / pf3 (non-first stages)
+            const double s2 = a.lsrk_first ? 0.0 + a.delta * s1 : (use_pf ? pf2[s] : a.s2[idx]) + a.delta * s1;
+            const double s3 = a.lsrk_first ? s1 : (use_pf ? pf3[s] : a.s3[idx]);
             double n1 = s1 * a.g0;
             n1 += a.g1 * s2;
             n1 += a.g2 * s3;

@@ -24,23 +24,26 @@ static int check_launch(dgx_ctx *c, const char *what)
     return DGX_OK;
 }
 
+
 template <int D, int N, typename T, bool RT>
 static void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
 {
     const bool split = c->P.use_split_form != 0;
     const bool nsv = c->P.pde == DGX_NAVIER_STOKES;
+    using LG = LineGeo<D, N, T>;
+    const dim3 lgrid((c->ncells + LG::EPB - 1) / LG::EPB), lblock(LG::BLOCK);
     if constexpr (D == 3) {
         if (split && nsv)
             hipLaunchKernelGGL((stage2_kernel<D, N, true, true, RT, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
         else if (!split && nsv)
             hipLaunchKernelGGL((stage2_kernel<D, N, false, true, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
         else if (split)
-            hipLaunchKernelGGL((line_kernel<D, N, RT, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
+            hipLaunchKernelGGL((line_kernel<D, N, RT, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
         else
             hipLaunchKernelGGL((stage2_kernel<D, N, false, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
     } else {
         if (split)
-            hipLaunchKernelGGL((line_kernel<D, N, RT, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
+            hipLaunchKernelGGL((line_kernel<D, N, RT, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
         else
             hipLaunchKernelGGL((stage2_kernel<D, N, false, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
     }

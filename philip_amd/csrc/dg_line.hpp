@@ -14,7 +14,10 @@
 //
 // Per block: EPB whole elements.
 //   A  node-parallel  load the stage state (FP64, coalesced), entropy round
-//                     trip (FP64 path), per-node cache into LDS
+//                     trip (FP64 path), per-node cache into LDS; in the same
+//                     phase the neighbours' face traces (one item per face
+//                     node) are loaded into LDS, so the line phase makes no
+//                     global loads
 //   L  line-parallel  pairs + faces of one line, partial sums into LDS
 //   E  node-parallel  sum of the direction partials, FP64 epilogue
 // T = float: mixed path (per-node cache of derived quantities, float fluxes,
@@ -28,9 +31,18 @@
 
 namespace dgx {
 
+// Issue the loads of the low-storage RK registers S2, S3 at the start of the
+// kernel (held in registers through the line phase) instead of in the epilogue.
+constexpr bool LINE_PREFETCH_RK = false;
+// Elements per block relative to the 256-node default (k > 0: k times, k < 0: 1/|k|).
+constexpr int LINE_EPB_SCALE = 1;
+
 template <int D, int N, typename T> struct LineGeo {
     using G = Geo<D, N>;
-    static constexpr int NP = G::NP, NS = D + 2, NL = G::NL, EPB = G::EPB, BLOCK = G::BLOCK;
+    static constexpr int NP = G::NP, NS = D + 2, NL = G::NL;
+    static constexpr int EPB = LINE_EPB_SCALE > 0 ? G::EPB * LINE_EPB_SCALE
+                                                  : (G::EPB / -LINE_EPB_SCALE > 0 ? G::EPB / -LINE_EPB_SCALE : 1);
+    static constexpr int BLOCK = ((EPB * NP + 63) / 64) * 64;
     static constexpr bool FAST = std::is_same<T, float>::value;
     // node cache fields: FAST u[NS], vel[D], p, b, |v|^2, sqrt(rho), c, h; FP64 u[NS], vel[D], p
     static constexpr int NQ = FAST ? node_f_count<D>() : NS + D + 1;
@@ -39,7 +51,10 @@ template <int D, int N, typename T> struct LineGeo {
     static constexpr int pad1(int x) { return x + (33 - x % 32) % 32; }
     static constexpr int ESQ = pad1(NQ * NP);
     static constexpr int ESV = pad1(D * NS * NP);
+    static constexpr int NFN = 2 * D * NL;           // face nodes per element
+    static constexpr int ESN = pad1(NS * NFN);       // neighbour traces [state][face][face node]
     static constexpr int ITEMS = EPB * D * NL;
+    static constexpr int FITEMS = EPB * NFN;
 };
 
 // ---- mixed path: node cache in float ------------------------------------
@@ -86,10 +101,14 @@ __device__ __forceinline__ void nodef_select(bool first, const NodeF<D> &x, cons
 }
 
 template <int D, int N, int K>
-__device__ __forceinline__ void line_work_f(const StageArgs &a, const Ops1D &ops, const PhysConst &pc,
-                                            const FastConst &fc, const float *sq, float *sv, int e, int line)
+__device__ __forceinline__ void line_work_f(const Ops1D &ops, const PhysConst &pc, const FastConst &fc,
+                                            const float *sqb, int lle, const float *sn, const int *snb, float *sv,
+                                            int blk0, int line)
 {
-    constexpr int NP = Geo<D, N>::NP, NS = D + 2;
+    using LG = LineGeo<D, N, float>;
+    constexpr int NP = LG::NP, NS = D + 2, NFN = LG::NFN;
+    const float *sq = sqb + lle * LG::ESQ;
+    const int e = blk0 + lle;
     float acc[N][NS];
 #pragma unroll
     for (int p = 0; p < N; ++p)
@@ -114,13 +133,18 @@ __device__ __forceinline__ void line_work_f(const StageArgs &a, const Ops1D &ops
     for (int side = 0; side < 2; ++side) {
         const int pos = side ? N - 1 : 0;
         const int fi = line_node<D, N>(K, line, pos);
-        const int nb = a.nbr[e * 2 * D + 2 * K + side];
-        const int fj = line_node<D, N>(K, line, side ? 0 : N - 1);
+        const int nb = snb[2 * K + side];
         const float sgn = side ? 1.f : -1.f;
         NodeF<D> qi, qn, L, R;
         nodef_load<D>(sq + fi, NP, qi);
-        for (int s = 0; s < NS; ++s) qn.u[s] = float(a.u[((long)nb * NS + s) * NP + fj]);
-        node_fill<D>(qn, fc);
+        const int nl = nb - blk0;
+        if (nl >= 0 && nl < LG::EPB) {
+            // neighbour in this block: its node cache is already in LDS
+            nodef_load<D>(sqb + nl * LG::ESQ + line_node<D, N>(K, line, side ? 0 : N - 1), NP, qn);
+        } else {
+            for (int s = 0; s < NS; ++s) qn.u[s] = sn[s * NFN + (2 * K + side) * LG::NL + line];
+            node_fill<D>(qn, fc);
+        }
         // reference orientation: the lower cell index is the interior side
         const bool own = e < nb;
         nodef_select<D>(own, qi, qn, L);
@@ -139,11 +163,14 @@ __device__ __forceinline__ void line_work_f(const StageArgs &a, const Ops1D &ops
 
 // ---- FP64 path ------------------------------------------------------------
 
-template <int D, int N, int K, bool RT>
-__device__ __forceinline__ void line_work_d(const StageArgs &a, const Ops1D &ops, const PhysConst &pc,
-                                            const double *sq, double *sv, int e, int line)
+template <int D, int N, int K>
+__device__ __forceinline__ void line_work_d(const Ops1D &ops, const PhysConst &pc, const double *sqb, int lle,
+                                            const double *sn, const int *snb, double *sv, int blk0, int line)
 {
-    constexpr int NP = Geo<D, N>::NP, NS = D + 2;
+    using LG = LineGeo<D, N, double>;
+    constexpr int NP = LG::NP, NS = D + 2, NFN = LG::NFN;
+    const double *sq = sqb + lle * LG::ESQ;
+    const int e = blk0 + lle;
     const double gamm1 = pc.gamm1;
     double acc[N][NS];
 #pragma unroll
@@ -181,17 +208,17 @@ __device__ __forceinline__ void line_work_d(const StageArgs &a, const Ops1D &ops
     for (int side = 0; side < 2; ++side) {
         const int pos = side ? N - 1 : 0;
         const int fi = line_node<D, N>(K, line, pos);
-        const int nb = a.nbr[e * 2 * D + 2 * K + side];
-        const int fj = line_node<D, N>(K, line, side ? 0 : N - 1);
+        const int nb = snb[2 * K + side];
         const double sgn = side ? 1.0 : -1.0;
-        double us[NS], un[NS], unt[NS];
+        double us[NS], unt[NS];
+        const int nl = nb - blk0;
+        const bool inblock = nl >= 0 && nl < LG::EPB;
+        const int fj = line_node<D, N>(K, line, side ? 0 : N - 1);
         for (int s = 0; s < NS; ++s) {
             us[s] = sq[s * NP + fi];
-            un[s] = a.u[((long)nb * NS + s) * NP + fj];
+            // neighbour in this block: its (round-tripped) state is already in LDS
+            unt[s] = inblock ? sqb[nl * LG::ESQ + s * NP + fj] : sn[s * NFN + (2 * K + side) * LG::NL + line];
         }
-        if (RT) roundtrip<double, D>(un, pc, unt);
-        else
-            for (int s = 0; s < NS; ++s) unt[s] = un[s];
         const bool own = e < nb;
         double ua[NS], ub[NS];
         for (int s = 0; s < NS; ++s) {
@@ -213,26 +240,61 @@ __device__ __forceinline__ void line_work_d(const StageArgs &a, const Ops1D &ops
 }
 
 template <int D, int N, bool RT, typename T>
-__global__ void __launch_bounds__((Geo<D, N>::BLOCK))
+__global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK))
 line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
     using LG = LineGeo<D, N, T>;
     constexpr int NP = LG::NP, NS = LG::NS, NL = LG::NL, EPB = LG::EPB;
     constexpr bool FAST = LG::FAST;
+    constexpr int NFI = (LG::FITEMS + LG::BLOCK - 1) / LG::BLOCK;  // face-trace items per thread
     __shared__ T sQ[EPB * LG::ESQ];   // per-node cache [element][field][node]
     __shared__ T sV[EPB * LG::ESV];   // line partials [element][direction][state][node]
+    __shared__ T sN[EPB * LG::ESN];   // neighbour face traces (FP64 path: after the round trip)
+    __shared__ int sNb[EPB][2 * D];   // face neighbours
 
     const int tid = threadIdx.x;
-    const int blk0 = blockIdx.x * EPB;
+    const int blk0 = xcd_block() * EPB;
     const FastConst fc = fast_const(pc);
 
-    // ---------------- phase A: node-parallel load -----------------------
+    // ---------------- phase A: loads first, then the node cache ----------
     const int le = tid / NP, i = tid % NP;
     const int e = blk0 + le;
     const bool active = (le < EPB) && (e < a.ncells);
     double ud[NS];
-    if (active) {
+    double un[NFI][NS];
+    int fdst[NFI];
+    if (active)
         for (int s = 0; s < NS; ++s) ud[s] = a.u[((long)e * NS + s) * NP + i];
+    // RK registers read by the epilogue, issued now so their latency hides behind the line phase
+    double pf2[NS], pf3[NS];
+    const bool pf = LINE_PREFETCH_RK && active && a.mode == MODE_LSRK && !a.lsrk_first;
+    if (pf)
+        for (int s = 0; s < NS; ++s) {
+            const long idx = ((long)e * NS + s) * NP + i;
+            pf2[s] = a.s2[idx];
+            pf3[s] = a.s3[idx];
+        }
+    // neighbour face traces (one item per face node), loaded together with the own state
+#pragma unroll
+    for (int j = 0; j < NFI; ++j) {
+        const int t = tid + j * LG::BLOCK;
+        fdst[j] = -1;
+        if (t < LG::FITEMS) {
+            const int fle = t / LG::NFN, r = t % LG::NFN;
+            const int fe = blk0 + fle;
+            if (fe < a.ncells) {
+                const int f = r / NL, fn = r % NL;
+                const int nb = a.nbr[fe * 2 * D + f];
+                if (fn == 0) sNb[fle][f] = nb;
+                if (nb < blk0 || nb >= blk0 + EPB) {  // in-block neighbours are read from the node cache
+                    const int fj = line_node<D, N>(f >> 1, fn, (f & 1) ? 0 : N - 1);
+                    for (int s = 0; s < NS; ++s) un[j][s] = a.u[((long)nb * NS + s) * NP + fj];
+                    fdst[j] = fle * LG::ESN + r;
+                }
+            }
+        }
+    }
+    if (active) {
         T *q = sQ + le * LG::ESQ + i;
         if constexpr (FAST) {
             NodeF<D> n;
@@ -250,24 +312,40 @@ line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
             q[(NS + D) * NP] = pressure<double, D>(ut, pc.gamm1);
         }
     }
+#pragma unroll
+    for (int j = 0; j < NFI; ++j) {
+        if (fdst[j] < 0) continue;
+        T *dst = sN + fdst[j];
+        if constexpr (FAST) {
+            for (int s = 0; s < NS; ++s) dst[s * LG::NFN] = float(un[j][s]);
+        } else {
+            double unt[NS];
+            if (RT) roundtrip<double, D>(un[j], pc, unt);
+            else
+                for (int s = 0; s < NS; ++s) unt[s] = un[j][s];
+            for (int s = 0; s < NS; ++s) dst[s * LG::NFN] = unt[s];
+        }
+    }
     __syncthreads();
 
     // ---------------- phase L: line-parallel pairs and faces -------------
     for (int t = tid; t < LG::ITEMS; t += LG::BLOCK) {
         const int k = t / (EPB * NL), r = t % (EPB * NL);
         const int lle = r / NL, line = r % NL;
-        const int le2 = blk0 + lle;
-        if (le2 >= a.ncells) continue;
-        const T *q = sQ + lle * LG::ESQ;
+        if (blk0 + lle >= a.ncells) continue;
+        const T *nt = sN + lle * LG::ESN;
+        const int *nb = sNb[lle];
         T *v = sV + lle * LG::ESV;
         if constexpr (FAST) {
-            if (k == 0) line_work_f<D, N, 0>(a, ops, pc, fc, q, v, le2, line);
-            else if (k == 1) line_work_f<D, N, 1>(a, ops, pc, fc, q, v + NS * NP, le2, line);
-            else if (D == 3) line_work_f<D, N, (D == 3 ? 2 : 1)>(a, ops, pc, fc, q, v + 2 * NS * NP, le2, line);
+            if (k == 0) line_work_f<D, N, 0>(ops, pc, fc, sQ, lle, nt, nb, v, blk0, line);
+            else if (k == 1) line_work_f<D, N, 1>(ops, pc, fc, sQ, lle, nt, nb, v + NS * NP, blk0, line);
+            else if (D == 3)
+                line_work_f<D, N, (D == 3 ? 2 : 1)>(ops, pc, fc, sQ, lle, nt, nb, v + 2 * NS * NP, blk0, line);
         } else {
-            if (k == 0) line_work_d<D, N, 0, RT>(a, ops, pc, q, v, le2, line);
-            else if (k == 1) line_work_d<D, N, 1, RT>(a, ops, pc, q, v + NS * NP, le2, line);
-            else if (D == 3) line_work_d<D, N, (D == 3 ? 2 : 1), RT>(a, ops, pc, q, v + 2 * NS * NP, le2, line);
+            if (k == 0) line_work_d<D, N, 0>(ops, pc, sQ, lle, nt, nb, v, blk0, line);
+            else if (k == 1) line_work_d<D, N, 1>(ops, pc, sQ, lle, nt, nb, v + NS * NP, blk0, line);
+            else if (D == 3)
+                line_work_d<D, N, (D == 3 ? 2 : 1)>(ops, pc, sQ, lle, nt, nb, v + 2 * NS * NP, blk0, line);
         }
     }
     __syncthreads();
@@ -277,13 +355,13 @@ line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
     T acc[NS];
     const T *v = sV + le * LG::ESV + i;
     for (int s = 0; s < NS; ++s) {
-        T x = v[s * NP];
-        for (int k = 1; k < D; ++k) x += v[(k * NS + s) * NP];
-        acc[s] = x;
+        T xs = v[s * NP];
+        for (int k = 1; k < D; ++k) xs += v[(k * NS + s) * NP];
+        acc[s] = xs;
     }
     int c[3] = {0, 0, 0};
     node_coords<D, N>(i, c);
-    stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc);
+    stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc, pf, pf2, pf3);
 }
 
 } // namespace dgx

@@ -25,16 +25,19 @@ template <typename T> __device__ __forceinline__ T dlog(T x) { return log(x); }
 template <typename T> __device__ __forceinline__ T dexp(T x) { return exp(x); }
 template <typename T> __device__ __forceinline__ T dpow(T x, T y) { return pow(x, y); }
 template <typename T> __device__ __forceinline__ T dabs(T x) { return fabs(x); }
-template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
 template <> __device__ __forceinline__ float dpow<float>(float x, float y) { return powf(x, y); }
 template <> __device__ __forceinline__ float dabs<float>(float x) { return fabsf(x); }
-// Division: IEEE in double (the FP64 path reproduces the reference's operations);
-// reciprocal-multiply in the mixed path's float arithmetic.
+// Division and square root: IEEE in double (the FP64 path reproduces the
+// reference's operations); hardware reciprocal / square root (about 1 ulp) in
+// the mixed path's float arithmetic.
 template <typename T> __device__ __forceinline__ T fdiv(T a, T b) { return a / b; }
-#ifndef DGX_HOST_TEST
+#ifdef DGX_HOST_TEST
+template <> inline float dsqrt<float>(float x) { return sqrtf(x); }
+#else
 template <> __device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
 template <> __device__ __forceinline__ float dexp<float>(float x) { return __expf(x); }
-template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { return __fdividef(a, b); }
+template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 #endif
 
 // compute_pressure_templated (euler.cpp:486-500)
