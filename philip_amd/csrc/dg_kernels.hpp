@@ -104,6 +104,22 @@ __device__ __forceinline__ void roundtrip(const T *u, const PhysConst &pc, T *ut
     conservative_from_entropy<T, D>(v, pc, ut);
 }
 
+// Streaming access to the RK registers: each value is read or written once per
+// stage and the registers together exceed the caches, so the stage kernels
+// mark these accesses non-temporal (STREAM_NT) to keep L2 for the neighbour
+// face traces.
+constexpr bool STREAM_NT = true;
+__device__ __forceinline__ double ld_stream(const double *p)
+{
+    if constexpr (STREAM_NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+__device__ __forceinline__ void st_stream(double *p, double v)
+{
+    if constexpr (STREAM_NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 // Phase E of the stage kernels, in FP64: du/dt = acc / h and the output of the
 // selected mode (residual, lifted right-hand side, low-storage RK 3S* register
 // update, Butcher-tableau stage update, or the FSAL error-register accumulation).
@@ -132,16 +148,16 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs &a, const Ops1D &
             const long idx = base + s * NP;
             const double s1 = ud[s];
             // use_pf: S2, S3 already loaded by the caller into pf2 / pf3 (non-first stages)
-            const double s2 = a.lsrk_first ? 0.0 + a.delta * s1 : (use_pf ? pf2[s] : a.s2[idx]) + a.delta * s1;
-            const double s3 = a.lsrk_first ? s1 : (use_pf ? pf3[s] : a.s3[idx]);
+            const double s2 = a.lsrk_first ? 0.0 + a.delta * s1 : (use_pf ? pf2[s] : ld_stream(a.s2 + idx)) + a.delta * s1;
+            const double s3 = a.lsrk_first ? s1 : (use_pf ? pf3[s] : ld_stream(a.s3 + idx));
             double n1 = s1 * a.g0;
             n1 += a.g1 * s2;
             n1 += a.g2 * s3;
             const double r = dudt[s] * a.dt;
             n1 += a.beta * r;
-            a.s2[idx] = s2;
-            if (a.lsrk_first) a.s3[idx] = s1;
-            a.s1_out[idx] = n1;
+            st_stream(a.s2 + idx, s2);
+            if (a.lsrk_first) st_stream(a.s3 + idx, s1);
+            st_stream(a.s1_out + idx, n1);
             if (a.s4) a.s4[idx] = (a.lsrk_first ? s1 : a.s4[idx]) + a.bhat * r;
         }
     } else {

@@ -24,7 +24,6 @@ static int check_launch(dgx_ctx *c, const char *what)
     return DGX_OK;
 }
 
-
 template <int D, int N, typename T, bool RT>
 static void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
 {

@@ -18,7 +18,9 @@
 //                     phase the neighbours' face traces (one item per face
 //                     node) are loaded into LDS, so the line phase makes no
 //                     global loads
-//   L  line-parallel  pairs + faces of one line, partial sums into LDS
+//   L  line-parallel  pairs + faces of one line, sums kept in registers;
+//                     after a barrier they go to LDS over the (now dead)
+//                     node cache and traces
 //   E  node-parallel  sum of the direction partials, FP64 epilogue
 // T = float: mixed path (per-node cache of derived quantities, float fluxes,
 // FP64 state and RK registers); T = double: FP64 path (the point physics of
@@ -55,6 +57,8 @@ template <int D, int N, typename T> struct LineGeo {
     static constexpr int ESN = pad1(NS * NFN);       // neighbour traces [state][face][face node]
     static constexpr int ITEMS = EPB * D * NL;
     static constexpr int FITEMS = EPB * NFN;
+    static constexpr int ESF = pad1(2 * D * NS * NL);   // face terms [face][state][face node]
+    static constexpr int SBUF = (ESQ + ESN > ESV + ESF) ? EPB * (ESQ + ESN) : EPB * (ESV + ESF);
 };
 
 // ---- mixed path: node cache in float ------------------------------------
@@ -101,15 +105,10 @@ __device__ __forceinline__ void nodef_select(bool first, const NodeF<D> &x, cons
 }
 
 template <int D, int N, int K>
-__device__ __forceinline__ void line_work_f(const Ops1D &ops, const PhysConst &pc, const FastConst &fc,
-                                            const float *sqb, int lle, const float *sn, const int *snb, float *sv,
-                                            int blk0, int line)
+__device__ __forceinline__ void pairs_f(const Ops1D &ops, const PhysConst &pc, const FastConst &fc, const float *sq,
+                                        int line, float (&acc)[N][D + 2])
 {
-    using LG = LineGeo<D, N, float>;
-    constexpr int NP = LG::NP, NS = D + 2, NFN = LG::NFN;
-    const float *sq = sqb + lle * LG::ESQ;
-    const int e = blk0 + lle;
-    float acc[N][NS];
+    constexpr int NP = Geo<D, N>::NP, NS = D + 2;
 #pragma unroll
     for (int p = 0; p < N; ++p)
         for (int s = 0; s < NS; ++s) acc[p][s] = 0.f;
@@ -129,50 +128,47 @@ __device__ __forceinline__ void line_work_f(const Ops1D &ops, const PhysConst &p
             }
         }
     }
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-        const int pos = side ? N - 1 : 0;
-        const int fi = line_node<D, N>(K, line, pos);
-        const int nb = snb[2 * K + side];
-        const float sgn = side ? 1.f : -1.f;
-        NodeF<D> qi, qn, L, R;
-        nodef_load<D>(sq + fi, NP, qi);
-        const int nl = nb - blk0;
-        if (nl >= 0 && nl < LG::EPB) {
-            // neighbour in this block: its node cache is already in LDS
-            nodef_load<D>(sqb + nl * LG::ESQ + line_node<D, N>(K, line, side ? 0 : N - 1), NP, qn);
-        } else {
-            for (int s = 0; s < NS; ++s) qn.u[s] = sn[s * NFN + (2 * K + side) * LG::NL + line];
-            node_fill<D>(qn, fc);
-        }
-        // reference orientation: the lower cell index is the interior side
-        const bool own = e < nb;
-        nodef_select<D>(own, qi, qn, L);
-        nodef_select<D>(own, qn, qi, R);
-        float fs[NS];
-        num_flux_fast<D, K>(L, R, own ? sgn : -sgn, fc, pc, fs);
-        const float w = own ? -ops.inv_wbf : ops.inv_wbf;
-        for (int s = 0; s < NS; ++s) acc[pos][s] += w * fs[s];
+}
+
+// Surface term of element e at face node fn of its face (K, side): the
+// numerical flux with the reference orientation (the lower cell index is the
+// interior side), times -1/w_0 for the interior and +1/w_0 for the exterior
+// side, so the neighbour's term is the negative.  nbl >= 0: the neighbour is
+// element nbl of this block (node cache in LDS); otherwise its trace is tr
+// (state-major, stride tst).
+template <int D, int N, int K>
+__device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, const FastConst &fc, const float *sqb,
+                                       int le, int side, int fn, int e, int nb, int nbl, const float *tr, int tst,
+                                       float (&out)[D + 2])
+{
+    using LG = LineGeo<D, N, float>;
+    constexpr int NP = LG::NP, NS = D + 2;
+    const int fi = line_node<D, N>(K, fn, side ? N - 1 : 0);
+    NodeF<D> qi, qn, L, R;
+    nodef_load<D>(sqb + le * LG::ESQ + fi, NP, qi);
+    if (nbl >= 0) {
+        nodef_load<D>(sqb + nbl * LG::ESQ + line_node<D, N>(K, fn, side ? 0 : N - 1), NP, qn);
+    } else {
+        for (int s = 0; s < NS; ++s) qn.u[s] = tr[s * tst];
+        node_fill<D>(qn, fc);
     }
-#pragma unroll
-    for (int p = 0; p < N; ++p) {
-        const int n = line_node<D, N>(K, line, p);
-        for (int s = 0; s < NS; ++s) sv[s * NP + n] = acc[p][s];
-    }
+    const bool own = e < nb;
+    nodef_select<D>(own, qi, qn, L);
+    nodef_select<D>(own, qn, qi, R);
+    const float sgn = side ? 1.f : -1.f;
+    float fs[NS];
+    num_flux_fast<D, K>(L, R, own ? sgn : -sgn, fc, pc, fs);
+    const float w = own ? -ops.inv_wbf : ops.inv_wbf;
+    for (int s = 0; s < NS; ++s) out[s] = w * fs[s];
 }
 
 // ---- FP64 path ------------------------------------------------------------
 
 template <int D, int N, int K>
-__device__ __forceinline__ void line_work_d(const Ops1D &ops, const PhysConst &pc, const double *sqb, int lle,
-                                            const double *sn, const int *snb, double *sv, int blk0, int line)
+__device__ __forceinline__ void pairs_d(const Ops1D &ops, const PhysConst &pc, const double *sq, int line,
+                                        double (&acc)[N][D + 2])
 {
-    using LG = LineGeo<D, N, double>;
-    constexpr int NP = LG::NP, NS = D + 2, NFN = LG::NFN;
-    const double *sq = sqb + lle * LG::ESQ;
-    const int e = blk0 + lle;
-    const double gamm1 = pc.gamm1;
-    double acc[N][NS];
+    constexpr int NP = Geo<D, N>::NP, NS = D + 2;
 #pragma unroll
     for (int p = 0; p < N; ++p)
         for (int s = 0; s < NS; ++s) acc[p][s] = 0.0;
@@ -204,39 +200,36 @@ __device__ __forceinline__ void line_work_d(const Ops1D &ops, const PhysConst &p
             }
         }
     }
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-        const int pos = side ? N - 1 : 0;
-        const int fi = line_node<D, N>(K, line, pos);
-        const int nb = snb[2 * K + side];
-        const double sgn = side ? 1.0 : -1.0;
-        double us[NS], unt[NS];
-        const int nl = nb - blk0;
-        const bool inblock = nl >= 0 && nl < LG::EPB;
-        const int fj = line_node<D, N>(K, line, side ? 0 : N - 1);
-        for (int s = 0; s < NS; ++s) {
-            us[s] = sq[s * NP + fi];
-            // neighbour in this block: its (round-tripped) state is already in LDS
-            unt[s] = inblock ? sqb[nl * LG::ESQ + s * NP + fj] : sn[s * NFN + (2 * K + side) * LG::NL + line];
-        }
-        const bool own = e < nb;
-        double ua[NS], ub[NS];
-        for (int s = 0; s < NS; ++s) {
-            ua[s] = own ? us[s] : unt[s];
-            ub[s] = own ? unt[s] : us[s];
-        }
-        const Prim<double, D> pa = to_prim<double, D>(ua, gamm1);
-        const Prim<double, D> pb = to_prim<double, D>(ub, gamm1);
-        double fs[NS];
-        numerical_flux<double, D>(ua, pa, ub, pb, K, own ? sgn : -sgn, pc, fs);
-        const double fsgn = own ? 1.0 : -1.0;
-        for (int s = 0; s < NS; ++s) acc[pos][s] += ops.inv_wb * (-(fs[s] * fsgn));
+}
+
+template <int D, int N, int K>
+__device__ __forceinline__ void face_d(const Ops1D &ops, const PhysConst &pc, const double *sqb, int le, int side,
+                                       int fn, int e, int nb, int nbl, const double *tr, int tst,
+                                       double (&out)[D + 2])
+{
+    using LG = LineGeo<D, N, double>;
+    constexpr int NP = LG::NP, NS = D + 2;
+    const int fi = line_node<D, N>(K, fn, side ? N - 1 : 0);
+    const int fj = line_node<D, N>(K, fn, side ? 0 : N - 1);
+    double us[NS], unt[NS];
+    for (int s = 0; s < NS; ++s) {
+        us[s] = sqb[le * LG::ESQ + s * NP + fi];
+        // in-block neighbour: its (round-tripped) state is in the node cache
+        unt[s] = (nbl >= 0) ? sqb[nbl * LG::ESQ + s * NP + fj] : tr[s * tst];
     }
-#pragma unroll
-    for (int p = 0; p < N; ++p) {
-        const int n = line_node<D, N>(K, line, p);
-        for (int s = 0; s < NS; ++s) sv[s * NP + n] = acc[p][s];
+    const bool own = e < nb;
+    double ua[NS], ub[NS];
+    for (int s = 0; s < NS; ++s) {
+        ua[s] = own ? us[s] : unt[s];
+        ub[s] = own ? unt[s] : us[s];
     }
+    const Prim<double, D> pa = to_prim<double, D>(ua, pc.gamm1);
+    const Prim<double, D> pb = to_prim<double, D>(ub, pc.gamm1);
+    const double sgn = side ? 1.0 : -1.0;
+    double fs[NS];
+    numerical_flux<double, D>(ua, pa, ub, pb, K, own ? sgn : -sgn, pc, fs);
+    const double fsgn = own ? 1.0 : -1.0;
+    for (int s = 0; s < NS; ++s) out[s] = ops.inv_wb * (-(fs[s] * fsgn));
 }
 
 template <int D, int N, bool RT, typename T>
@@ -244,13 +237,20 @@ __global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK))
 line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
     using LG = LineGeo<D, N, T>;
-    constexpr int NP = LG::NP, NS = LG::NS, NL = LG::NL, EPB = LG::EPB;
+    constexpr int NP = LG::NP, NS = LG::NS, NL = LG::NL, EPB = LG::EPB, BLOCK = LG::BLOCK;
     constexpr bool FAST = LG::FAST;
-    constexpr int NFI = (LG::FITEMS + LG::BLOCK - 1) / LG::BLOCK;  // face-trace items per thread
-    __shared__ T sQ[EPB * LG::ESQ];   // per-node cache [element][field][node]
-    __shared__ T sV[EPB * LG::ESV];   // line partials [element][direction][state][node]
-    __shared__ T sN[EPB * LG::ESN];   // neighbour face traces (FP64 path: after the round trip)
-    __shared__ int sNb[EPB][2 * D];   // face neighbours
+    constexpr int NFI = (LG::FITEMS + BLOCK - 1) / BLOCK;  // face-trace loads per thread
+    constexpr int NFS = EPB * 2 * D;                       // face sides of the block
+    constexpr int NWAVE = BLOCK / 64;
+    // The node cache and the neighbour traces are dead once every item has its
+    // result in registers, so the line and face partials reuse their storage.
+    __shared__ T sBuf[LG::SBUF];
+    T *const sQ = sBuf;                       // per-node cache [element][field][node]
+    T *const sN = sBuf + EPB * LG::ESQ;       // neighbour face traces [element][state][face][face node]
+    T *const sV = sBuf;                       // line partials [element][direction][state][node]
+    T *const sF = sBuf + EPB * LG::ESV;       // face terms [element][face][state][face node]
+    __shared__ int sNb[EPB][2 * D];           // face neighbours
+    __shared__ short sList[NWAVE][NFS];       // per-wave list of the faces evaluated in this block
 
     const int tid = threadIdx.x;
     const int blk0 = xcd_block() * EPB;
@@ -265,7 +265,6 @@ line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
     int fdst[NFI];
     if (active)
         for (int s = 0; s < NS; ++s) ud[s] = a.u[((long)e * NS + s) * NP + i];
-    // RK registers read by the epilogue, issued now so their latency hides behind the line phase
     double pf2[NS], pf3[NS];
     const bool pf = LINE_PREFETCH_RK && active && a.mode == MODE_LSRK && !a.lsrk_first;
     if (pf)
@@ -277,7 +276,7 @@ line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
     // neighbour face traces (one item per face node), loaded together with the own state
 #pragma unroll
     for (int j = 0; j < NFI; ++j) {
-        const int t = tid + j * LG::BLOCK;
+        const int t = tid + j * BLOCK;
         fdst[j] = -1;
         if (t < LG::FITEMS) {
             const int fle = t / LG::NFN, r = t % LG::NFN;
@@ -328,30 +327,110 @@ line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
     }
     __syncthreads();
 
-    // ---------------- phase L: line-parallel pairs and faces -------------
-    for (int t = tid; t < LG::ITEMS; t += LG::BLOCK) {
-        const int k = t / (EPB * NL), r = t % (EPB * NL);
-        const int lle = r / NL, line = r % NL;
-        if (blk0 + lle >= a.ncells) continue;
-        const T *nt = sN + lle * LG::ESN;
-        const int *nb = sNb[lle];
-        T *v = sV + lle * LG::ESV;
-        if constexpr (FAST) {
-            if (k == 0) line_work_f<D, N, 0>(ops, pc, fc, sQ, lle, nt, nb, v, blk0, line);
-            else if (k == 1) line_work_f<D, N, 1>(ops, pc, fc, sQ, lle, nt, nb, v + NS * NP, blk0, line);
-            else if (D == 3)
-                line_work_f<D, N, (D == 3 ? 2 : 1)>(ops, pc, fc, sQ, lle, nt, nb, v + 2 * NS * NP, blk0, line);
-        } else {
-            if (k == 0) line_work_d<D, N, 0>(ops, pc, sQ, lle, nt, nb, v, blk0, line);
-            else if (k == 1) line_work_d<D, N, 1>(ops, pc, sQ, lle, nt, nb, v + NS * NP, blk0, line);
-            else if (D == 3)
-                line_work_d<D, N, (D == 3 ? 2 : 1)>(ops, pc, sQ, lle, nt, nb, v + 2 * NS * NP, blk0, line);
+    // ---------------- faces evaluated in this block ----------------------
+    // Face side (le, f) is evaluated here unless its neighbour is an element
+    // of this block with a lower index (that element evaluates the shared face
+    // once for both).  Each wave builds the same list in its own LDS row,
+    // ordered by direction so that a wave's face items mostly share one.
+    const int lane = tid & 63, wv = tid >> 6;
+    int nface = 0;
+    for (int c0 = 0; c0 < NFS; c0 += 64) {
+        const int id = c0 + lane;  // id = (k * EPB + le) * 2 + side
+        bool want = false;
+        if (id < NFS) {
+            const int kk = id / (2 * EPB), r = id % (2 * EPB);
+            const int fle = r >> 1, f = 2 * kk + (r & 1);
+            if (blk0 + fle < a.ncells) {
+                const int nb = sNb[fle][f];
+                want = !(nb >= blk0 && nb < blk0 + EPB && nb < blk0 + fle);
+            }
+        }
+        const unsigned long long m = __ballot(want);
+        if (want) sList[wv][nface + __popcll(m & ((1ull << lane) - 1ull))] = (short)id;
+        nface += __popcll(m);
+    }
+
+    // ---------------- phase L: pair-line items, then face items ----------
+    constexpr int PITEMS = LG::ITEMS;                          // (direction, element, line)
+    constexpr int JP = (PITEMS + BLOCK - 1) / BLOCK;           // slots that can hold a line item
+    constexpr int JT = (PITEMS + NFS * NL + BLOCK - 1) / BLOCK;
+    const int items = PITEMS + nface * NL;
+    T rp[JP][N][NS];
+    T rf[JT][NS];
+#pragma unroll
+    for (int j = 0; j < JT; ++j) {
+        const int t = tid + j * BLOCK;
+        if (j < JP && t < PITEMS) {
+            const int k = t / (EPB * NL), r = t % (EPB * NL);
+            const int lle = r / NL, line = r % NL;
+            if (blk0 + lle >= a.ncells) continue;
+            const T *q = sQ + lle * LG::ESQ;
+            if constexpr (FAST) {
+                if (k == 0) pairs_f<D, N, 0>(ops, pc, fc, q, line, rp[j < JP ? j : 0]);
+                else if (k == 1) pairs_f<D, N, 1>(ops, pc, fc, q, line, rp[j < JP ? j : 0]);
+                else if (D == 3) pairs_f<D, N, (D == 3 ? 2 : 1)>(ops, pc, fc, q, line, rp[j < JP ? j : 0]);
+            } else {
+                if (k == 0) pairs_d<D, N, 0>(ops, pc, q, line, rp[j < JP ? j : 0]);
+                else if (k == 1) pairs_d<D, N, 1>(ops, pc, q, line, rp[j < JP ? j : 0]);
+                else if (D == 3) pairs_d<D, N, (D == 3 ? 2 : 1)>(ops, pc, q, line, rp[j < JP ? j : 0]);
+            }
+        } else if (t >= PITEMS && t < items) {
+            const int u = (t - PITEMS) / NL, fn = (t - PITEMS) % NL;
+            const int id = sList[wv][u];
+            const int k = id / (2 * EPB), r = id % (2 * EPB);
+            const int fle = r >> 1, side = r & 1;
+            const int nb = sNb[fle][2 * k + side];
+            const int nbl = (nb >= blk0 && nb < blk0 + EPB) ? nb - blk0 : -1;
+            const T *tr = sN + fle * LG::ESN + (2 * k + side) * NL + fn;
+            if constexpr (FAST) {
+                if (k == 0) face_f<D, N, 0>(ops, pc, fc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN, rf[j]);
+                else if (k == 1) face_f<D, N, 1>(ops, pc, fc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN, rf[j]);
+                else if (D == 3)
+                    face_f<D, N, (D == 3 ? 2 : 1)>(ops, pc, fc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN,
+                                                   rf[j]);
+            } else {
+                if (k == 0) face_d<D, N, 0>(ops, pc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN, rf[j]);
+                else if (k == 1) face_d<D, N, 1>(ops, pc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN, rf[j]);
+                else if (D == 3)
+                    face_d<D, N, (D == 3 ? 2 : 1)>(ops, pc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN, rf[j]);
+            }
+        }
+    }
+    __syncthreads();  // node cache and traces consumed; their storage now takes the partials
+#pragma unroll
+    for (int j = 0; j < JT; ++j) {
+        const int t = tid + j * BLOCK;
+        if (j < JP && t < PITEMS) {
+            const int k = t / (EPB * NL), r = t % (EPB * NL);
+            const int lle = r / NL, line = r % NL;
+            if (blk0 + lle >= a.ncells) continue;
+            T *v = sV + lle * LG::ESV + k * NS * NP;
+#pragma unroll
+            for (int p = 0; p < N; ++p) {
+                const int n = (k == 0) ? line_node<D, N>(0, line, p)
+                                       : (k == 1) ? line_node<D, N>(1, line, p) : line_node<D, N>(D == 3 ? 2 : 1, line, p);
+                for (int s = 0; s < NS; ++s) v[s * NP + n] = rp[j < JP ? j : 0][p][s];
+            }
+        } else if (t >= PITEMS && t < items) {
+            const int u = (t - PITEMS) / NL, fn = (t - PITEMS) % NL;
+            const int id = sList[wv][u];
+            const int k = id / (2 * EPB), r = id % (2 * EPB);
+            const int fle = r >> 1, f = 2 * k + (r & 1);
+            const int nb = sNb[fle][f];
+            T *w = sF + fle * LG::ESF + f * NS * NL + fn;
+            for (int s = 0; s < NS; ++s) w[s * NL] = rf[j][s];
+            if (nb >= blk0 && nb < blk0 + EPB) {  // the shared face's term for the neighbour
+                T *w2 = sF + (nb - blk0) * LG::ESF + (f ^ 1) * NS * NL + fn;
+                for (int s = 0; s < NS; ++s) w2[s * NL] = -rf[j][s];
+            }
         }
     }
     __syncthreads();
     if (!active) return;
 
-    // ---------------- phase E: sum of direction partials, FP64 epilogue ---
+    // ---------------- phase E: volume + surface terms, FP64 epilogue ------
+    int c[3] = {0, 0, 0};
+    node_coords<D, N>(i, c);
     T acc[NS];
     const T *v = sV + le * LG::ESV + i;
     for (int s = 0; s < NS; ++s) {
@@ -359,8 +438,13 @@ line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
         for (int k = 1; k < D; ++k) xs += v[(k * NS + s) * NP];
         acc[s] = xs;
     }
-    int c[3] = {0, 0, 0};
-    node_coords<D, N>(i, c);
+    const T *fb = sF + le * LG::ESF;
+    for (int k = 0; k < D; ++k) {
+        if (c[k] != 0 && c[k] != N - 1) continue;
+        const int f = 2 * k + (c[k] == N - 1 ? 1 : 0);
+        const T *w = fb + f * NS * NL + node_line<D, N>(k, i);
+        for (int s = 0; s < NS; ++s) acc[s] += w[s * NL];
+    }
     stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc, pf, pf2, pf3);
 }
 
