@@ -120,6 +120,9 @@ int dgx_apply_inverse_mass_to_rhs(dgx_ctx *ctx, double *host_out);
    u_dev, r_dev: n_dofs doubles on the context's device. */
 int dgx_assemble_residual_device(dgx_ctx *ctx, const double *u_dev, double *r_dev);
 int dgx_rhs_device(dgx_ctx *ctx, const double *u_dev, double *dudt_dev);
+/* DGBase::apply_inverse_global_mass_matrix(in, out) (dg_base.hpp:267, dg_base.cpp:4089-4260)
+   on device buffers of n_dofs doubles; in_dev == out_dev is allowed. */
+int dgx_apply_inverse_mass_device(dgx_ctx *ctx, const double *in_dev, double *out_dev);
 
 /* ODESolverFactory::create_ODESolver + allocate_ode_system (ode_solver_factory.cpp:20-35,
    ode_solver_base.h:79): allocates the RK registers for `method`. */
@@ -150,6 +153,38 @@ int dgx_set_stream(dgx_ctx *ctx, void *hip_stream);
    HIP events on the context stream when timing is enabled. */
 int dgx_enable_timing(dgx_ctx *ctx, int enable);
 int dgx_kernel_timing(const dgx_ctx *ctx, double *avg_stage_ms, int64_t *n_stage_launches);
+
+/* ---------------------------------------------------------------------------
+   Batched independent solves.  No counterpart in the reference, which advances
+   independent solves one FlowSolver at a time: these entry points replace a loop
+   over instances of ODESolverBase::step_in_time (ode_solver_base.h:70) with one
+   HIP graph launch per RK step that advances every instance.  Instances of the
+   split-form Euler scheme that share (dim, p, precision) are advanced by one
+   batched kernel launch per stage; other schemes by one launch per instance
+   inside the same graph.  Each instance's arithmetic is the one of its single
+   solve, so its result equals the single solve's.
+   --------------------------------------------------------------------------- */
+typedef struct dgx_batch dgx_batch;
+/* n instances, params[i] describing instance i (dim, p, cells, scheme, precision may differ). */
+int dgx_batch_create(const dgx_params *params, int n, dgx_batch **out);
+int dgx_batch_destroy(dgx_batch *b);
+const char *dgx_batch_last_error(const dgx_batch *b);
+int dgx_batch_size(const dgx_batch *b);
+/* Context of instance i (set/get solution, initial conditions, residual, integrated quantities);
+   owned by the batch. */
+dgx_ctx *dgx_batch_instance(dgx_batch *b, int i);
+/* Allocates every instance's RK registers for an explicit Butcher method with at least two
+   stages (Heun, SSPRK3, RK4) and the time step dt[i] of each instance, picks the
+   elements-per-block variant of each (dim, p, precision) group by timing the candidates
+   (cached per process), and captures one RK step of all instances in a HIP graph. */
+int dgx_batch_allocate(dgx_batch *b, int method, const double *dt);
+/* Advances every instance by n_steps steps (one graph launch per step); per-instance
+   current_time / current_iteration advance as in dgx_step_in_time. */
+int dgx_batch_step_in_time(dgx_batch *b, int n_steps);
+int dgx_batch_synchronize(dgx_batch *b);
+/* Elements-per-block variant chosen for instance i's group (1 = 256-node blocks, 2 = twice),
+   0 if the instance is launched on its own. */
+int dgx_batch_variant(const dgx_batch *b, int i);
 
 #ifdef __cplusplus
 }

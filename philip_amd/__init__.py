@@ -24,10 +24,13 @@ EXPORTED = [
     "dgx_default_params", "dgx_create", "dgx_destroy", "dgx_last_error", "dgx_n_dofs", "dgx_n_cells",
     "dgx_set_solution", "dgx_get_solution", "dgx_get_right_hand_side", "dgx_get_auxiliary_solution",
     "dgx_get_neighbors", "dgx_assemble_residual", "dgx_apply_inverse_mass_to_rhs",
-    "dgx_assemble_residual_device", "dgx_rhs_device", "dgx_allocate_ode_system", "dgx_step_in_time",
-    "dgx_current_time", "dgx_current_iteration", "dgx_set_track_error_estimate",
-    "dgx_constant_time_step", "dgx_integrated_quantities", "dgx_set_stream", "dgx_enable_timing",
-    "dgx_kernel_timing", "dgx_set_initial_condition_tgv", "dgx_set_initial_condition_vortex", "dgx_synchronize",
+    "dgx_assemble_residual_device", "dgx_rhs_device", "dgx_apply_inverse_mass_device",
+    "dgx_allocate_ode_system", "dgx_step_in_time", "dgx_current_time", "dgx_current_iteration",
+    "dgx_set_track_error_estimate", "dgx_constant_time_step", "dgx_integrated_quantities", "dgx_set_stream",
+    "dgx_enable_timing", "dgx_kernel_timing", "dgx_set_initial_condition_tgv",
+    "dgx_set_initial_condition_vortex", "dgx_synchronize",
+    "dgx_batch_create", "dgx_batch_destroy", "dgx_batch_last_error", "dgx_batch_size", "dgx_batch_instance",
+    "dgx_batch_allocate", "dgx_batch_step_in_time", "dgx_batch_synchronize", "dgx_batch_variant",
 ]
 
 
@@ -94,6 +97,18 @@ def load():
     lib.dgx_enable_timing.argtypes = [vp, i]
     lib.dgx_kernel_timing.argtypes = [vp, dp, P(i64)]
     lib.dgx_synchronize.argtypes = [vp]
+    lib.dgx_apply_inverse_mass_device.argtypes = [vp, vp, vp]
+    lib.dgx_batch_create.argtypes = [P(Params), i, P(vp)]
+    lib.dgx_batch_destroy.argtypes = [vp]
+    lib.dgx_batch_last_error.restype = ctypes.c_char_p
+    lib.dgx_batch_last_error.argtypes = [vp]
+    lib.dgx_batch_size.argtypes = [vp]
+    lib.dgx_batch_instance.restype = vp
+    lib.dgx_batch_instance.argtypes = [vp, i]
+    lib.dgx_batch_allocate.argtypes = [vp, i, dp]
+    lib.dgx_batch_step_in_time.argtypes = [vp, i]
+    lib.dgx_batch_synchronize.argtypes = [vp]
+    lib.dgx_batch_variant.argtypes = [vp, i]
     _lib = lib
     return lib
 
@@ -113,14 +128,17 @@ def _dp(a):
 class DG:
     """DGBase-shaped handle over the device context (one per HIP device/stream)."""
 
-    def __init__(self, **kw):
+    def __init__(self, _handle=None, _params=None, **kw):
         lib = load()
-        self.p = params(**kw)
-        h = ctypes.c_void_p()
-        rc = lib.dgx_create(ctypes.byref(self.p), ctypes.byref(h))
-        if rc != 0:
-            raise DeviceError(f"dgx_create failed with code {rc}")
-        self.h = h
+        if _handle is not None:  # a context owned by a Batch
+            self.p, self.h, self._owned = _params, ctypes.c_void_p(_handle), False
+        else:
+            self.p = params(**kw)
+            h = ctypes.c_void_p()
+            rc = lib.dgx_create(ctypes.byref(self.p), ctypes.byref(h))
+            if rc != 0:
+                raise DeviceError(f"dgx_create failed with code {rc}")
+            self.h, self._owned = h, True
         self.dim = self.p.dim
         self.ns = self.dim + 2
         self.n = self.p.poly_degree + 1
@@ -134,7 +152,8 @@ class DG:
 
     def close(self):
         if getattr(self, "h", None):
-            load().dgx_destroy(self.h)
+            if self._owned:
+                load().dgx_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -237,3 +256,53 @@ class DG:
             t = self.current_time
             steps += 1
         return steps
+
+
+class Batch:
+    """Independent solves advanced together: one HIP graph launch per RK step
+    (dgx_batch_* in the C ABI).  ``instances[i]`` is a DG handle on instance i."""
+
+    def __init__(self, param_dicts):
+        lib = load()
+        self.params = (Params * len(param_dicts))()
+        for k, kw in enumerate(param_dicts):
+            self.params[k] = params(**kw)
+        h = ctypes.c_void_p()
+        rc = lib.dgx_batch_create(self.params, len(param_dicts), ctypes.byref(h))
+        if rc != 0:
+            raise DeviceError(f"dgx_batch_create failed with code {rc}")
+        self.h = h
+        self.instances = [DG(_handle=lib.dgx_batch_instance(h, k), _params=self.params[k])
+                          for k in range(len(param_dicts))]
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise DeviceError(f"{what}: code {rc}: {load().dgx_batch_last_error(self.h).decode()}")
+
+    def allocate(self, method, dts):
+        m = METHODS[method] if isinstance(method, str) else method
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        assert dts.size == len(self.instances)
+        self._chk(load().dgx_batch_allocate(self.h, m, _dp(dts)), "batch_allocate")
+
+    def step_in_time(self, n_steps=1):
+        self._chk(load().dgx_batch_step_in_time(self.h, n_steps), "batch_step_in_time")
+
+    def synchronize(self):
+        self._chk(load().dgx_batch_synchronize(self.h), "batch_synchronize")
+
+    def variant(self, i):
+        return load().dgx_batch_variant(self.h, i)
+
+    def close(self):
+        if getattr(self, "h", None):
+            for d in self.instances:
+                d.h = None
+            load().dgx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -292,6 +292,7 @@ int dgx_destroy(dgx_ctx *c)
     hipFree(c->nbr_dev);
     hipFree(c->u_dev);
     hipFree(c->r_dev);
+    hipFree(c->minv_dev);
     for (int k = 0; k < 3; ++k) hipFree(c->aux_dev[k]);
     for (double *r : c->regs) hipFree(r);
     hipFree(c->iq_ops);
@@ -400,20 +401,22 @@ int dgx_rhs_device(dgx_ctx *c, const double *u_dev, double *dudt_dev)
     return residual_into(c, u_dev, dudt_dev, MODE_RHS);
 }
 
+int dgx_apply_inverse_mass_device(dgx_ctx *c, const double *in_dev, double *out_dev)
+{
+    if (!c || !in_dev || !out_dev) return DGX_ERR_INVALID;
+    return launch_inverse_mass(c, in_dev, out_dev);
+}
+
 int dgx_apply_inverse_mass_to_rhs(dgx_ctx *c, double *host_out)
 {
-    // M^-1 of the collocated GLL cell is diagonal: out_i = r_i / (W_i det J)
+    // M^-1 of the collocated GLL cell is diagonal, applied on the device
     if (!c || !host_out) return DGX_ERR_INVALID;
-    std::vector<double> r((size_t)c->ndofs);
-    HIPCHK(hipMemcpy(r.data(), c->r_dev, sizeof(double) * c->ndofs, hipMemcpyDeviceToHost));
-    const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
-    for (int64_t q = 0; q < c->ndofs; ++q) {
-        const int node = (int)(q % c->Np);
-        const int a = node % c->n, b = (node / c->n) % c->n, d = node / (c->n * c->n);
-        double W = c->ops.w[a] * c->ops.w[b];
-        if (c->dim == 3) W *= c->ops.w[d];
-        host_out[q] = r[q] / (W * detJ);
-    }
+    if (!c->minv_dev && hipMalloc(&c->minv_dev, sizeof(double) * c->ndofs) != hipSuccess)
+        return fail(c, DGX_ERR_HIP, "hipMalloc failed");
+    int rc = launch_inverse_mass(c, c->r_dev, c->minv_dev);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(host_out, c->minv_dev, sizeof(double) * c->ndofs, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return DGX_OK;
 }
 
@@ -530,6 +533,7 @@ static bool get_lsrk(int m, Lsrk &L)
 int dgx_allocate_ode_system(dgx_ctx *c, int method)
 {
     if (!c) return DGX_ERR_INVALID;
+    if (c->batch_locked) return fail(c, DGX_ERR_STATE, "instance registers belong to a batch graph");
     Butcher B;
     Lsrk L;
     for (double *r : c->regs) hipFree(r);
@@ -557,45 +561,65 @@ int dgx_set_track_error_estimate(dgx_ctx *c, int enable)
     return DGX_OK;
 }
 
-static int step_butcher(dgx_ctx *c, const Butcher &B, double dt)
+// Registers of explicit RK: 0 accumulator (solution_update), 1/2 stage
+// ping-pong, 3.. stored k_j (only those a later stage reads).
+static void butcher_needs_k(const Butcher &B, bool need_k[4])
 {
-    // registers: 0 acc, 1/2 stage ping-pong, 3.. stored k_j
-    double *acc = c->regs[0];
-    double *stage_buf[2] = {c->regs[1], c->regs[2]};
-    bool need_k[4] = {false, false, false, false};
+    for (int j = 0; j < 4; ++j) need_k[j] = false;
     for (int j = 0; j < B.s; ++j)
         for (int m = j + 2; m < B.s; ++m)
             if (B.a[m][j] != 0.0) need_k[j] = true;
-    const double *cur = c->u_dev;
+}
+
+// Stage i of a Butcher step (runge_kutta_ode_solver.cpp:15-99): stage state in,
+// k_i = M^-1 R(stage state), accumulator += dt b_i k_i, next stage state
+// u_n + dt sum_j a_{i+1,j} k_j.  The last stage writes u_{n+1} over u_n, which it
+// no longer reads (its stage state is a stage buffer); a one-stage method reads
+// u_n as its stage state, so it keeps the accumulator and the caller swaps.
+static int butcher_stage_args(dgx_ctx *c, const Butcher &B, double dt, int i, StageArgs &a)
+{
+    bool need_k[4];
+    butcher_needs_k(B, need_k);
+    double *acc = c->regs[0];
+    double *stage_buf[2] = {c->regs[1], c->regs[2]};
+    const double *cur = (i == 0) ? c->u_dev : stage_buf[(i - 1) & 1];
+    a = base_args(c, cur);
+    a.mode = MODE_BUTCHER;
+    a.dt = dt;
+    a.u0 = c->u_dev;
+    a.acc = acc;
+    a.acc_out = (i == B.s - 1 && B.s > 1) ? c->u_dev : acc;
+    a.first_stage = (i == 0);
+    a.last_stage = (i == B.s - 1);
+    a.stage = i;
+    a.b_dt = dt * B.b[i];
+    a.k_store = need_k[i] ? c->regs[3 + i] : nullptr;
+    if (i < B.s - 1) {
+        a.u_next = stage_buf[i & 1];
+        for (int j = 0; j <= i; ++j) {
+            a.a_next[j] = B.a[i + 1][j];
+            a.k_prev[j] = (j < i && need_k[j]) ? c->regs[3 + j] : nullptr;
+            if (j < i && B.a[i + 1][j] != 0.0 && !need_k[j])
+                return fail(c, DGX_ERR_INVALID, "tableau register bookkeeping");
+        }
+    }
+    return DGX_OK;
+}
+
+static int step_butcher(dgx_ctx *c, const Butcher &B, double dt)
+{
     for (int i = 0; i < B.s; ++i) {
+        StageArgs a;
+        int rc = butcher_stage_args(c, B, dt, i, a);
+        if (rc) return rc;
         if (c->P.pde == DGX_NAVIER_STOKES) {
-            int rc = launch_aux(c, cur);
+            rc = launch_aux(c, a.u);
             if (rc) return rc;
         }
-        StageArgs a = base_args(c, cur);
-        a.mode = MODE_BUTCHER;
-        a.dt = dt;
-        a.u0 = c->u_dev;
-        a.acc = acc;
-        a.first_stage = (i == 0);
-        a.last_stage = (i == B.s - 1);
-        a.stage = i;
-        a.b_dt = dt * B.b[i];
-        a.k_store = need_k[i] ? c->regs[3 + i] : nullptr;
-        if (i < B.s - 1) {
-            a.u_next = stage_buf[i & 1];
-            for (int j = 0; j <= i; ++j) {
-                a.a_next[j] = B.a[i + 1][j];
-                a.k_prev[j] = (j < i && need_k[j]) ? c->regs[3 + j] : nullptr;
-                if (j < i && B.a[i + 1][j] != 0.0 && !need_k[j])
-                    return fail(c, DGX_ERR_INVALID, "tableau register bookkeeping");
-            }
-        }
-        int rc = launch_stage(c, a);
+        rc = launch_stage(c, a);
         if (rc) return rc;
-        cur = stage_buf[i & 1];
     }
-    std::swap(c->u_dev, c->regs[0]); // solution = solution_update
+    if (B.s == 1) std::swap(c->u_dev, c->regs[0]); // solution = solution_update
     return DGX_OK;
 }
 
@@ -780,6 +804,318 @@ int dgx_kernel_timing(const dgx_ctx *c, double *avg_ms, int64_t *n)
     if (!c) return DGX_ERR_INVALID;
     if (avg_ms) *avg_ms = c->timing.n ? c->timing.sum_ms / c->timing.n : 0.0;
     if (n) *n = c->timing.n;
+    return DGX_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched independent solves
+// ---------------------------------------------------------------------------
+namespace {
+
+struct BatchGroup {
+    int dim = 0, n = 0, precision = 0;
+    int es = 0;                          // elements-per-block variant; 0: per-instance launches
+    std::vector<int> inst;               // instance indices
+    BatchBlock *blocks = nullptr;        // device block table
+    int nblocks = 0;
+    PhysConst *pcs = nullptr;            // device, per slot
+    std::vector<StageArgs *> args;       // device, per stage: per slot
+};
+
+// elements-per-block variant chosen per (dim, n, precision) in this process
+std::vector<std::pair<long, int>> g_variant_cache;
+
+} // namespace
+
+struct dgx_batch {
+    std::vector<dgx_ctx *> inst;
+    std::vector<double> dt;
+    std::vector<BatchGroup> groups;
+    std::vector<int> variant;            // per instance
+    int method = -1;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string err;
+};
+
+#define BHIPCHK(call)                                                                    \
+    do {                                                                                 \
+        hipError_t _e = (call);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            b->err = std::string(#call) + ": " + hipGetErrorString(_e);                   \
+            return DGX_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+static void batch_free_groups(dgx_batch *b)
+{
+    for (BatchGroup &g : b->groups) {
+        hipFree(g.blocks);
+        hipFree(g.pcs);
+        for (StageArgs *a : g.args) hipFree(a);
+    }
+    b->groups.clear();
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+static bool batch_line_eligible(const dgx_ctx *c)
+{
+    return c->P.use_split_form && c->P.pde == DGX_EULER;
+}
+
+// block table of a group for variant es: (slot, first element) per element group
+static int batch_blocks(dgx_batch *b, BatchGroup &g, int es)
+{
+    int epb = 0;
+    int rc = launch_line_batch(g.dim, g.n, g.precision, es, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr, &epb);
+    if (rc) return rc;
+    std::vector<BatchBlock> tab;
+    for (int slot = 0; slot < (int)g.inst.size(); ++slot) {
+        const dgx_ctx *c = b->inst[g.inst[slot]];
+        for (int e = 0; e < c->ncells; e += epb) tab.push_back(BatchBlock{slot, e});
+    }
+    hipFree(g.blocks);
+    g.blocks = nullptr;
+    BHIPCHK(hipMalloc(&g.blocks, sizeof(BatchBlock) * tab.size()));
+    BHIPCHK(hipMemcpy(g.blocks, tab.data(), sizeof(BatchBlock) * tab.size(), hipMemcpyHostToDevice));
+    g.nblocks = (int)tab.size();
+    g.es = es;
+    return DGX_OK;
+}
+
+// Times the candidate variants on the group's stage-0 arguments in MODE_RHS (writing into
+// each instance's second stage buffer, which the step overwrites anyway) and keeps the fastest.
+static int batch_autotune(dgx_batch *b, BatchGroup &g, const Ops1D &ops)
+{
+    const long key = ((long)g.dim * 16 + g.n) * 4 + g.precision;
+    for (auto &kv : g_variant_cache)
+        if (kv.first == key) return batch_blocks(b, g, kv.second);
+    std::vector<StageArgs> probe(g.inst.size());
+    for (size_t slot = 0; slot < g.inst.size(); ++slot) {
+        dgx_ctx *c = b->inst[g.inst[slot]];
+        StageArgs a = base_args(c, c->u_dev);
+        a.mode = MODE_RHS;
+        a.out = c->regs[2];
+        probe[slot] = a;
+    }
+    StageArgs *pa = nullptr;
+    BHIPCHK(hipMalloc(&pa, sizeof(StageArgs) * probe.size()));
+    BHIPCHK(hipMemcpy(pa, probe.data(), sizeof(StageArgs) * probe.size(), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    BHIPCHK(hipEventCreate(&e0));
+    BHIPCHK(hipEventCreate(&e1));
+    int best = 1;
+    float best_ms = 1e30f;
+    for (int es : {1, 2}) {
+        if (batch_blocks(b, g, es) != DGX_OK) continue;  // variant absent for this shape
+        float ms = 0.f;
+        for (int rep = 0; rep < 4; ++rep) {  // first launch is a warm-up
+            BHIPCHK(hipEventRecord(e0, b->stream));
+            if (launch_line_batch(g.dim, g.n, g.precision, es, g.blocks, g.nblocks, pa, g.pcs, ops, b->stream,
+                                  nullptr))
+                return DGX_ERR_HIP;
+            BHIPCHK(hipEventRecord(e1, b->stream));
+            BHIPCHK(hipEventSynchronize(e1));
+            float t = 0.f;
+            BHIPCHK(hipEventElapsedTime(&t, e0, e1));
+            if (rep) ms += t;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = es;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(pa);
+    g_variant_cache.push_back({key, best});
+    return batch_blocks(b, g, best);
+}
+
+extern "C" {
+
+int dgx_batch_create(const dgx_params *params, int n, dgx_batch **out)
+{
+    if (!params || n < 1 || !out) return DGX_ERR_INVALID;
+    *out = nullptr;
+    dgx_batch *b = new dgx_batch();
+    if (hipStreamCreate(&b->stream) != hipSuccess) {
+        delete b;
+        return DGX_ERR_HIP;
+    }
+    for (int i = 0; i < n; ++i) {
+        dgx_ctx *c = nullptr;
+        int rc = dgx_create(&params[i], &c);
+        if (rc) {
+            dgx_batch_destroy(b);
+            return rc;
+        }
+        c->stream = b->stream;
+        b->inst.push_back(c);
+    }
+    b->variant.assign(n, 0);
+    *out = b;
+    return DGX_OK;
+}
+
+int dgx_batch_destroy(dgx_batch *b)
+{
+    if (!b) return DGX_OK;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    batch_free_groups(b);
+    for (dgx_ctx *c : b->inst) {
+        c->stream = nullptr;
+        c->batch_locked = false;
+        dgx_destroy(c);
+    }
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return DGX_OK;
+}
+
+const char *dgx_batch_last_error(const dgx_batch *b) { return b ? b->err.c_str() : "null batch"; }
+int dgx_batch_size(const dgx_batch *b) { return b ? (int)b->inst.size() : 0; }
+
+dgx_ctx *dgx_batch_instance(dgx_batch *b, int i)
+{
+    if (!b || i < 0 || i >= (int)b->inst.size()) return nullptr;
+    return b->inst[i];
+}
+
+int dgx_batch_variant(const dgx_batch *b, int i)
+{
+    if (!b || i < 0 || i >= (int)b->inst.size()) return -1;
+    return b->variant[i];
+}
+
+int dgx_batch_allocate(dgx_batch *b, int method, const double *dt)
+{
+    if (!b || !dt) return DGX_ERR_INVALID;
+    Butcher B;
+    if (!get_butcher(method, B) || B.s < 2) {
+        b->err = "batched stepping supports explicit Butcher methods with at least two stages";
+        return DGX_ERR_UNSUPPORTED;
+    }
+    BHIPCHK(hipStreamSynchronize(b->stream));
+    batch_free_groups(b);
+    const int n = (int)b->inst.size();
+    b->dt.assign(dt, dt + n);
+    for (int i = 0; i < n; ++i) {
+        dgx_ctx *c = b->inst[i];
+        c->batch_locked = false;
+        int rc = dgx_allocate_ode_system(c, method);
+        if (rc) {
+            b->err = c->err;
+            return rc;
+        }
+        c->batch_locked = true;
+        c->timing.enabled = false;  // no host synchronisation inside the captured graph
+    }
+    b->method = method;
+    // groups of split-form Euler instances sharing (dim, n, precision)
+    for (int i = 0; i < n; ++i) {
+        dgx_ctx *c = b->inst[i];
+        b->variant[i] = 0;
+        if (!batch_line_eligible(c)) continue;
+        BatchGroup *g = nullptr;
+        for (BatchGroup &h : b->groups)
+            if (h.dim == c->dim && h.n == c->n && h.precision == c->P.precision) g = &h;
+        if (!g) {
+            b->groups.emplace_back();
+            g = &b->groups.back();
+            g->dim = c->dim;
+            g->n = c->n;
+            g->precision = c->P.precision;
+        }
+        g->inst.push_back(i);
+    }
+    for (BatchGroup &g : b->groups) {
+        std::vector<PhysConst> pcs;
+        for (int i : g.inst) pcs.push_back(b->inst[i]->pc);
+        BHIPCHK(hipMalloc(&g.pcs, sizeof(PhysConst) * pcs.size()));
+        BHIPCHK(hipMemcpy(g.pcs, pcs.data(), sizeof(PhysConst) * pcs.size(), hipMemcpyHostToDevice));
+        for (int st = 0; st < B.s; ++st) {
+            std::vector<StageArgs> as(g.inst.size());
+            for (size_t slot = 0; slot < g.inst.size(); ++slot) {
+                dgx_ctx *c = b->inst[g.inst[slot]];
+                int rc = butcher_stage_args(c, B, b->dt[g.inst[slot]], st, as[slot]);
+                if (rc) {
+                    b->err = c->err;
+                    return rc;
+                }
+            }
+            StageArgs *d = nullptr;
+            BHIPCHK(hipMalloc(&d, sizeof(StageArgs) * as.size()));
+            BHIPCHK(hipMemcpy(d, as.data(), sizeof(StageArgs) * as.size(), hipMemcpyHostToDevice));
+            g.args.push_back(d);
+        }
+        const Ops1D ops = b->inst[g.inst[0]]->ops;
+        int rc = batch_autotune(b, g, ops);
+        if (rc) return rc;
+        for (int i : g.inst) b->variant[i] = g.es;
+    }
+    // capture one RK step of every instance
+    BHIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = DGX_OK;
+    for (int st = 0; st < B.s && rc == DGX_OK; ++st) {
+        for (BatchGroup &g : b->groups) {
+            const Ops1D &ops = b->inst[g.inst[0]]->ops;
+            rc = launch_line_batch(g.dim, g.n, g.precision, g.es, g.blocks, g.nblocks, g.args[st], g.pcs, ops,
+                                   b->stream, nullptr);
+            if (rc) break;
+        }
+        for (int i = 0; i < n && rc == DGX_OK; ++i) {
+            dgx_ctx *c = b->inst[i];
+            if (b->variant[i]) continue;
+            StageArgs a;
+            rc = butcher_stage_args(c, B, b->dt[i], st, a);
+            if (!rc && c->P.pde == DGX_NAVIER_STOKES) rc = launch_aux(c, a.u);
+            if (!rc) rc = launch_stage(c, a);
+            if (rc) b->err = c->err;
+        }
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ce != hipSuccess) {
+        b->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(ce);
+        return DGX_ERR_HIP;
+    }
+    b->graph = graph;
+    BHIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return DGX_OK;
+}
+
+int dgx_batch_step_in_time(dgx_batch *b, int n_steps)
+{
+    if (!b) return DGX_ERR_INVALID;
+    if (!b->exec) {
+        b->err = "dgx_batch_allocate first";
+        return DGX_ERR_STATE;
+    }
+    for (int s = 0; s < n_steps; ++s) {
+        BHIPCHK(hipGraphLaunch(b->exec, b->stream));
+        for (size_t i = 0; i < b->inst.size(); ++i) {
+            ++b->inst[i]->iteration;
+            b->inst[i]->current_time += b->dt[i];
+        }
+    }
+    return DGX_OK;
+}
+
+int dgx_batch_synchronize(dgx_batch *b)
+{
+    if (!b) return DGX_ERR_INVALID;
+    BHIPCHK(hipStreamSynchronize(b->stream));
     return DGX_OK;
 }
 

@@ -11,6 +11,7 @@
 
 #include "../../include/philip_dg.h"
 #include "dg_kernels.hpp"
+#include "dg_line.hpp"
 
 namespace dgx {
 
@@ -51,6 +52,7 @@ struct dgx_ctx {
     double *u_dev = nullptr;       // DGBase::solution
     double *r_dev = nullptr;       // DGBase::right_hand_side
     double *aux_dev[3] = {nullptr, nullptr, nullptr};
+    double *minv_dev = nullptr;    // lifted right-hand side (dgx_apply_inverse_mass_to_rhs)
     // ODE system
     int method = -1;
     bool track_error = false;
@@ -62,7 +64,8 @@ struct dgx_ctx {
     std::string err;
     dgx::Timing timing;
     int device = 0;
-    int n_cu = 0;                  // compute units of the device (persistent-kernel grid size)
+    int n_cu = 0;                  // compute units of the device
+    bool batch_locked = false;     // RK registers captured in a batch graph: not re-allocatable
     // integrated-quantity operators (QGauss(p+1+overint))
     int iq_overint = -1;
     double *iq_ops = nullptr;      // B [nx][n], G [nx][n], w [nx]
@@ -73,6 +76,13 @@ struct dgx_ctx {
 // kernel launch helpers (dg_launch.hip)
 namespace dgx {
 int launch_stage(dgx_ctx *c, const StageArgs &a);
+int launch_inverse_mass(dgx_ctx *c, const double *in, double *out);
+// Batched split-form line kernel over the element groups `blocks` of several solves of one
+// (dim, n, precision); es = elements-per-block variant (1 or 2).  With blocks == nullptr only
+// reports the variant's elements per block in *epb.  DGX_ERR_UNSUPPORTED if the variant does
+// not exist for this shape.
+int launch_line_batch(int dim, int n, int precision, int es, const BatchBlock *blocks, int nblocks,
+                      const StageArgs *args, const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb);
 int launch_aux(dgx_ctx *c, const double *u);
 int launch_integrated(dgx_ctx *c, const double *u, int overint, double *host_out4);
 }

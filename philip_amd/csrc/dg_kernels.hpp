@@ -48,7 +48,8 @@ struct StageArgs {
     double delta, g0, g1, g2, beta, bhat, dt;
     // Butcher explicit RK
     const double *u0;         // u_n
-    double *acc;              // solution_update accumulator
+    double *acc;              // solution_update accumulator (read)
+    double *acc_out;          // where the updated accumulator is written (acc, or u_n at the last stage)
     double *u_next;           // next stage solution (null at the last stage)
     double *k_store;          // where to store k_i (null if not needed later)
     const double *k_prev[4];  // stored k_j, j < i
@@ -166,7 +167,7 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs &a, const Ops1D &
             const double kk = dudt[s];
             double accv = a.first_stage ? a.u0[idx] : a.acc[idx];
             accv += a.b_dt * kk;
-            a.acc[idx] = accv;
+            a.acc_out[idx] = accv;
             if (a.k_store) a.k_store[idx] = kk;
             if (a.u_next) {
                 // stage indices unrolled so the register file is never indexed per lane

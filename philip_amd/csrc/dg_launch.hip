@@ -3,6 +3,7 @@
 // integrated-quantity reduction (kinetic energy, enstrophy).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -99,6 +100,82 @@ static int launch_aux_dn(dgx_ctx *c, const double *u)
     }
 
 int launch_stage(dgx_ctx *c, const StageArgs &a) { DISPATCH_N(launch_stage_dn, c, a) }
+
+// ---------------------------------------------------------------------------
+// Batched line kernel: one launch over the element groups of many solves of one
+// (dim, p, precision); es selects the elements-per-block variant.
+// ---------------------------------------------------------------------------
+template <int D, int N, typename T, bool RT, int ES>
+static int line_batch_es(const BatchBlock *blocks, int nblocks, const StageArgs *args, const PhysConst *pcs,
+                         const Ops1D &ops, hipStream_t st, int *epb)
+{
+    using LG = LineGeo<D, N, T, ES>;
+    if constexpr (LG::SBUF * sizeof(T) > 65536) {
+        return DGX_ERR_UNSUPPORTED;  // variant would not fit the per-block LDS budget
+    } else {
+        if (epb) *epb = LG::EPB;
+        if (!blocks) return DGX_OK;  // query only
+        hipLaunchKernelGGL((line_kernel_batch<D, N, RT, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks, args,
+                           pcs, ops);
+        return hipGetLastError() == hipSuccess ? DGX_OK : DGX_ERR_HIP;
+    }
+}
+
+template <int D, int N>
+static int line_batch_dn(int precision, int es, const BatchBlock *blocks, int nblocks, const StageArgs *args,
+                         const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb)
+{
+    if (precision == DGX_MIXED) {
+        if (es == 2) return line_batch_es<D, N, float, false, 2>(blocks, nblocks, args, pcs, ops, st, epb);
+        return line_batch_es<D, N, float, false, 1>(blocks, nblocks, args, pcs, ops, st, epb);
+    }
+    if (es == 2) return line_batch_es<D, N, double, true, 2>(blocks, nblocks, args, pcs, ops, st, epb);
+    return line_batch_es<D, N, double, true, 1>(blocks, nblocks, args, pcs, ops, st, epb);
+}
+
+int launch_line_batch(int dim, int n, int precision, int es, const BatchBlock *blocks, int nblocks,
+                      const StageArgs *args, const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb)
+{
+#define LB_CASE(NN)                                                                                 \
+    case NN:                                                                                        \
+        return (dim == 3) ? line_batch_dn<3, NN>(precision, es, blocks, nblocks, args, pcs, ops, st, epb) \
+                          : line_batch_dn<2, NN>(precision, es, blocks, nblocks, args, pcs, ops, st, epb);
+    switch (n) {
+        LB_CASE(2)
+        LB_CASE(3)
+        LB_CASE(4)
+        LB_CASE(5)
+        LB_CASE(6)
+        LB_CASE(7)
+    default: return DGX_ERR_UNSUPPORTED;
+    }
+#undef LB_CASE
+}
+
+// Inverse of the collocated GLL mass matrix (diagonal): out_q = in_q / (W_q det J),
+// W_q the tensor-product GLL weight of the node (dg_base.cpp:4089-4260).
+__global__ void __launch_bounds__(256)
+inverse_mass_kernel(const double *__restrict__ in, double *__restrict__ out, long ndofs, int n, int dim,
+                    double detJ, Ops1D ops)
+{
+    const int Np = (dim == 3) ? n * n * n : n * n;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < ndofs; q += (long)gridDim.x * blockDim.x) {
+        const int node = (int)(q % Np);
+        const int a = node % n, b = (node / n) % n, c = node / (n * n);
+        double W = ops.w[a] * ops.w[b];
+        if (dim == 3) W *= ops.w[c];
+        out[q] = in[q] / (W * detJ);
+    }
+}
+
+int launch_inverse_mass(dgx_ctx *c, const double *in, double *out)
+{
+    const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
+    const long blocks = std::min<long>((c->ndofs + 255) / 256, 4096);
+    hipLaunchKernelGGL(inverse_mass_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, in, out, (long)c->ndofs,
+                       c->n, c->dim, detJ, c->ops);
+    return check_launch(c, "inverse_mass_kernel");
+}
 
 int launch_aux(dgx_ctx *c, const double *u)
 {

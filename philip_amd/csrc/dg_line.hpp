@@ -36,14 +36,12 @@ namespace dgx {
 // Issue the loads of the low-storage RK registers S2, S3 at the start of the
 // kernel (held in registers through the line phase) instead of in the epilogue.
 constexpr bool LINE_PREFETCH_RK = false;
-// Elements per block relative to the 256-node default (k > 0: k times, k < 0: 1/|k|).
-constexpr int LINE_EPB_SCALE = 1;
-
-template <int D, int N, typename T> struct LineGeo {
+// ES: elements per block relative to the 256-node default (ES > 0: ES times,
+// ES < 0: 1/|ES|); per-element strides do not depend on it.
+template <int D, int N, typename T, int ES = 1> struct LineGeo {
     using G = Geo<D, N>;
     static constexpr int NP = G::NP, NS = D + 2, NL = G::NL;
-    static constexpr int EPB = LINE_EPB_SCALE > 0 ? G::EPB * LINE_EPB_SCALE
-                                                  : (G::EPB / -LINE_EPB_SCALE > 0 ? G::EPB / -LINE_EPB_SCALE : 1);
+    static constexpr int EPB = ES > 0 ? G::EPB * ES : (G::EPB / -ES > 0 ? G::EPB / -ES : 1);
     static constexpr int BLOCK = ((EPB * NP + 63) / 64) * 64;
     static constexpr bool FAST = std::is_same<T, float>::value;
     // node cache fields: FAST u[NS], vel[D], p, b, |v|^2, sqrt(rho), c, h; FP64 u[NS], vel[D], p
@@ -232,11 +230,11 @@ __device__ __forceinline__ void face_d(const Ops1D &ops, const PhysConst &pc, co
     for (int s = 0; s < NS; ++s) out[s] = ops.inv_wb * (-(fs[s] * fsgn));
 }
 
-template <int D, int N, bool RT, typename T>
-__global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK))
-line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
+// One block's work: elements blk0 .. blk0 + EPB - 1 of the solve described by a.
+template <int D, int N, bool RT, typename T, int ES>
+__device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops, const PhysConst &pc, const int blk0)
 {
-    using LG = LineGeo<D, N, T>;
+    using LG = LineGeo<D, N, T, ES>;
     constexpr int NP = LG::NP, NS = LG::NS, NL = LG::NL, EPB = LG::EPB, BLOCK = LG::BLOCK;
     constexpr bool FAST = LG::FAST;
     constexpr int NFI = (LG::FITEMS + BLOCK - 1) / BLOCK;  // face-trace loads per thread
@@ -253,7 +251,6 @@ line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
     __shared__ short sList[NWAVE][NFS];       // per-wave list of the faces evaluated in this block
 
     const int tid = threadIdx.x;
-    const int blk0 = xcd_block() * EPB;
     const FastConst fc = fast_const(pc);
 
     // ---------------- phase A: loads first, then the node cache ----------
@@ -446,6 +443,29 @@ line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
         for (int s = 0; s < NS; ++s) acc[s] += w[s * NL];
     }
     stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc, pf, pf2, pf3);
+}
+
+// One solve: blocks in XCD-aware order over the element groups.
+template <int D, int N, bool RT, typename T>
+__global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK))
+line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
+{
+    line_block<D, N, RT, T, 1>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
+}
+
+// Many independent solves of one (dim, p, precision) in one launch: block b
+// (in XCD-aware order) takes element group blocks[b] of solve blocks[b].inst,
+// whose stage arguments and physical constants are read from device tables.
+struct BatchBlock {
+    int inst, blk0;
+};
+template <int D, int N, bool RT, typename T, int ES>
+__global__ void __launch_bounds__((LineGeo<D, N, T, ES>::BLOCK))
+line_kernel_batch(const BatchBlock *__restrict__ blocks, const StageArgs *__restrict__ args,
+                  const PhysConst *__restrict__ pcs, Ops1D ops)
+{
+    const BatchBlock bb = blocks[xcd_block()];
+    line_block<D, N, RT, T, ES>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
 }
 
 } // namespace dgx
