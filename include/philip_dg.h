@@ -186,6 +186,55 @@ int dgx_batch_synchronize(dgx_batch *b);
    0 if the instance is launched on its own. */
 int dgx_batch_variant(const dgx_batch *b, int i);
 
+/* ---------------------------------------------------------------------------
+   Serving: a request loop over independent parametric solves.  No counterpart
+   in the reference (each PHiLiP run is one FlowSolver::run, flow_solver.cpp:
+   552-700).  Requests queue on submit; each scheduler step admits queued
+   requests while their HBM footprint fits the budget and a slot is free
+   (first come, first served), advances every active request by one RK step
+   with the stages of all active requests batched together (one launch per
+   (dim, p, precision) group per stage for the split-form Euler scheme), and
+   retires requests that reached their final time (solution copied to the
+   host, latency = completion - submission).  The time loop of each request is
+   FlowSolver::run's: constant dt, last step clipped to the final time
+   (flow_solver.cpp:554-561).
+   --------------------------------------------------------------------------- */
+enum { DGX_IC_TGV = 0, DGX_IC_VORTEX = 1 };
+typedef struct {
+    dgx_params params;
+    int initial_condition;                   /* DGX_IC_TGV or DGX_IC_VORTEX */
+    int tgv_isothermal;
+    double vortex_strength, vortex_u, vortex_v;
+    double final_time;
+    /* dt = cfl * dx / wave_speed with dx = L / (n_dofs / nstate)^(1/dim)
+       (periodic_turbulence.cpp:115-125 with wave_speed = 1, periodic_entropy_tests.cpp:22-36) */
+    double cfl, wave_speed;
+} dgx_request;
+
+typedef struct dgx_server dgx_server;
+/* hbm_budget_bytes: device memory the active requests may hold; max_active: slots;
+   method: explicit Butcher method with at least two stages. */
+int dgx_server_create(int64_t hbm_budget_bytes, int max_active, int method, dgx_server **out);
+int dgx_server_destroy(dgx_server *s);
+const char *dgx_server_last_error(const dgx_server *s);
+/* HBM bytes a request holds while active (solution, residual, RK registers, connectivity). */
+int64_t dgx_request_bytes(const dgx_request *r, int method);
+int64_t dgx_request_n_dofs(const dgx_request *r);
+/* Queues a request (returns at once); *id identifies it.  DGX_ERR_INVALID if it can never
+   fit the budget or its parameters are rejected by dgx_create. */
+int dgx_server_submit(dgx_server *s, const dgx_request *r, int64_t *id);
+/* One scheduler step (admit, advance all active requests by one RK step, retire); blocks
+   until the step's device work is done.  Counts after the step are returned. */
+int dgx_server_step(dgx_server *s, int *n_active, int *n_queued, int *n_done);
+/* Result of a finished request: solution (n_dofs doubles), seconds from submit to completion,
+   RK steps taken, DOF-updates performed.  DGX_ERR_STATE while not finished. */
+int dgx_server_result(dgx_server *s, int64_t id, double *host_u, double *latency_s, int64_t *steps,
+                      double *dof_updates);
+/* Drops a finished request's stored result. */
+int dgx_server_release(dgx_server *s, int64_t id);
+/* HBM bytes held by the active requests and the budget. */
+int dgx_server_memory(const dgx_server *s, int64_t *in_use, int64_t *budget);
+
 #ifdef __cplusplus
 }
 #endif

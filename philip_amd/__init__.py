@@ -31,6 +31,8 @@ EXPORTED = [
     "dgx_set_initial_condition_vortex", "dgx_synchronize",
     "dgx_batch_create", "dgx_batch_destroy", "dgx_batch_last_error", "dgx_batch_size", "dgx_batch_instance",
     "dgx_batch_allocate", "dgx_batch_step_in_time", "dgx_batch_synchronize", "dgx_batch_variant",
+    "dgx_request_bytes", "dgx_request_n_dofs", "dgx_server_create", "dgx_server_destroy", "dgx_server_last_error",
+    "dgx_server_submit", "dgx_server_step", "dgx_server_result", "dgx_server_release", "dgx_server_memory",
 ]
 
 
@@ -52,6 +54,26 @@ class DeviceError(RuntimeError):
 
 
 _lib = None
+
+
+class Request(ctypes.Structure):
+    """dgx_request: one parametric solve for the server."""
+    _fields_ = [("params", Params), ("initial_condition", ctypes.c_int), ("tgv_isothermal", ctypes.c_int),
+                ("vortex_strength", ctypes.c_double), ("vortex_u", ctypes.c_double), ("vortex_v", ctypes.c_double),
+                ("final_time", ctypes.c_double), ("cfl", ctypes.c_double), ("wave_speed", ctypes.c_double)]
+
+
+IC_TGV, IC_VORTEX = 0, 1
+
+
+def request(final_time, cfl, wave_speed=1.0, ic=IC_TGV, vortex=(5.0, 0.0, 0.0), isothermal=False, **kw):
+    r = Request()
+    r.params = params(**kw)
+    r.initial_condition = ic
+    r.tgv_isothermal = int(isothermal)
+    r.vortex_strength, r.vortex_u, r.vortex_v = vortex
+    r.final_time, r.cfl, r.wave_speed = final_time, cfl, wave_speed
+    return r
 
 
 def load():
@@ -109,6 +131,19 @@ def load():
     lib.dgx_batch_step_in_time.argtypes = [vp, i]
     lib.dgx_batch_synchronize.argtypes = [vp]
     lib.dgx_batch_variant.argtypes = [vp, i]
+    lib.dgx_request_bytes.restype = i64
+    lib.dgx_request_bytes.argtypes = [P(Request), i]
+    lib.dgx_request_n_dofs.restype = i64
+    lib.dgx_request_n_dofs.argtypes = [P(Request)]
+    lib.dgx_server_create.argtypes = [i64, i, i, P(vp)]
+    lib.dgx_server_destroy.argtypes = [vp]
+    lib.dgx_server_last_error.restype = ctypes.c_char_p
+    lib.dgx_server_last_error.argtypes = [vp]
+    lib.dgx_server_submit.argtypes = [vp, P(Request), P(i64)]
+    lib.dgx_server_step.argtypes = [vp, P(i), P(i), P(i)]
+    lib.dgx_server_result.argtypes = [vp, i64, dp, dp, P(i64), dp]
+    lib.dgx_server_release.argtypes = [vp, i64]
+    lib.dgx_server_memory.argtypes = [vp, P(i64), P(i64)]
     _lib = lib
     return lib
 
@@ -306,3 +341,67 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Request loop with HBM-budget admission and continuous batching (dgx_server_* in the C ABI)."""
+
+    def __init__(self, hbm_budget_bytes, max_active=64, method="rk4"):
+        m = METHODS[method] if isinstance(method, str) else method
+        h = ctypes.c_void_p()
+        rc = load().dgx_server_create(int(hbm_budget_bytes), max_active, m, ctypes.byref(h))
+        if rc != 0:
+            raise DeviceError(f"dgx_server_create failed with code {rc}")
+        self.h = h
+        self.method = m
+        self.ndofs = {}
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise DeviceError(f"{what}: code {rc}: {load().dgx_server_last_error(self.h).decode()}")
+
+    def submit(self, req):
+        i = ctypes.c_int64()
+        self._chk(load().dgx_server_submit(self.h, ctypes.byref(req), ctypes.byref(i)), "server_submit")
+        self.ndofs[i.value] = load().dgx_request_n_dofs(ctypes.byref(req))
+        return i.value
+
+    def step(self):
+        a, q, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._chk(load().dgx_server_step(self.h, ctypes.byref(a), ctypes.byref(q), ctypes.byref(d)), "server_step")
+        return a.value, q.value, d.value
+
+    def result(self, rid):
+        """(solution, latency seconds, steps, DOF-updates) of a finished request, or None."""
+        u = np.empty(self.ndofs[rid])
+        lat, dofu = ctypes.c_double(), ctypes.c_double()
+        st = ctypes.c_int64()
+        rc = load().dgx_server_result(self.h, rid, _dp(u), ctypes.byref(lat), ctypes.byref(st), ctypes.byref(dofu))
+        if rc != 0:
+            return None
+        return u, lat.value, st.value, dofu.value
+
+    def release(self, rid):
+        load().dgx_server_release(self.h, rid)
+
+    def memory(self):
+        """(bytes held by active requests, budget)."""
+        u, b = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(load().dgx_server_memory(self.h, ctypes.byref(u), ctypes.byref(b)), "server_memory")
+        return u.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().dgx_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def request_bytes(req, method="rk4"):
+    m = METHODS[method] if isinstance(method, str) else method
+    return load().dgx_request_bytes(ctypes.byref(req), m)

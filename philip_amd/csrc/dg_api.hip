@@ -5,7 +5,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <chrono>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -1116,6 +1118,357 @@ int dgx_batch_synchronize(dgx_batch *b)
 {
     if (!b) return DGX_ERR_INVALID;
     BHIPCHK(hipStreamSynchronize(b->stream));
+    return DGX_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving: continuous batching of independent requests
+// ---------------------------------------------------------------------------
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    int64_t id = 0;
+    dgx_request r{};
+    Clock::time_point submitted;
+    int64_t bytes = 0;
+    dgx_ctx *c = nullptr;   // while active
+    double dt = 0.0;        // constant step (last one clipped)
+    int64_t steps = 0;
+};
+
+struct Result {
+    std::vector<double> u;
+    double latency = 0.0, dof_updates = 0.0;
+    int64_t steps = 0;
+};
+
+int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+} // namespace
+
+struct dgx_server {
+    int64_t budget = 0, used = 0;
+    int max_active = 0, method = -1;
+    Butcher B{};
+    hipStream_t stream = nullptr;
+    int64_t next_id = 1;
+    std::vector<Request> queue;     // FIFO
+    std::vector<Request> active;
+    std::map<int64_t, Result> done;
+    char *stage_host = nullptr;     // pinned staging of the per-step tables
+    char *stage_dev = nullptr;
+    int64_t stage_cap = 0;
+    std::string err;
+};
+
+static int64_t request_ndofs(const dgx_params &p)
+{
+    const int64_t n = p.poly_degree + 1, N = p.n_cells_1d;
+    const int64_t np = (p.dim == 3) ? n * n * n : n * n, ncells = (p.dim == 3) ? N * N * N : N * N;
+    return ncells * np * (p.dim + 2);
+}
+
+#define SHIPCHK(call)                                                                    \
+    do {                                                                                 \
+        hipError_t _e = (call);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            s->err = std::string(#call) + ": " + hipGetErrorString(_e);                   \
+            return DGX_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+static int server_admit(dgx_server *s)
+{
+    while (!s->queue.empty() && (int)s->active.size() < s->max_active && s->used + s->queue.front().bytes <= s->budget) {
+        Request q = s->queue.front();
+        s->queue.erase(s->queue.begin());
+        dgx_ctx *c = nullptr;
+        int rc = dgx_create(&q.r.params, &c);
+        if (rc) {
+            s->err = "dgx_create failed for an admitted request";
+            return rc;
+        }
+        c->stream = s->stream;
+        rc = (q.r.initial_condition == DGX_IC_VORTEX)
+                 ? dgx_set_initial_condition_vortex(c, q.r.vortex_strength, q.r.vortex_u, q.r.vortex_v)
+                 : dgx_set_initial_condition_tgv(c, q.r.tgv_isothermal);
+        if (!rc) rc = dgx_allocate_ode_system(c, s->method);
+        if (rc) {
+            s->err = c->err;
+            c->stream = nullptr;
+            dgx_destroy(c);
+            return rc;
+        }
+        const double L = q.r.params.domain_right - q.r.params.domain_left;
+        const double dx = L / std::pow((double)(unsigned)(c->ndofs / c->ns), 1.0 / c->dim);
+        q.dt = q.r.cfl * dx / q.r.wave_speed;
+        q.c = c;
+        s->used += q.bytes;
+        s->active.push_back(q);
+    }
+    return DGX_OK;
+}
+
+// One RK step of every active request; stage arguments of the split-form Euler
+// requests go to the device in one packed copy, then one batched launch per
+// (dim, p, precision) group per stage; other schemes launch per request.
+static int server_advance(dgx_server *s)
+{
+    const int na = (int)s->active.size();
+    const int S = s->B.s;
+    std::vector<double> step(na);
+    std::vector<std::vector<StageArgs>> args(na, std::vector<StageArgs>(S));
+    for (int k = 0; k < na; ++k) {
+        Request &q = s->active[k];
+        double h = q.dt;
+        if (q.c->current_time + h > q.r.final_time) h = q.r.final_time - q.c->current_time;
+        step[k] = h;
+        for (int st = 0; st < S; ++st) {
+            int rc = butcher_stage_args(q.c, s->B, h, st, args[k][st]);
+            if (rc) {
+                s->err = q.c->err;
+                return rc;
+            }
+        }
+    }
+    struct G {
+        int dim, n, precision, es, epb;
+        std::vector<int> members;
+        int64_t off_blocks, off_pcs, off_args;
+        int nblocks;
+    };
+    std::vector<G> groups;
+    std::vector<bool> grouped(na, false);
+    for (int k = 0; k < na; ++k) {
+        dgx_ctx *c = s->active[k].c;
+        if (!batch_line_eligible(c)) continue;
+        G *g = nullptr;
+        for (G &x : groups)
+            if (x.dim == c->dim && x.n == c->n && x.precision == c->P.precision) g = &x;
+        if (!g) {
+            G x{};
+            x.dim = c->dim;
+            x.n = c->n;
+            x.precision = c->P.precision;
+            const long key = ((long)x.dim * 16 + x.n) * 4 + x.precision;
+            x.es = 1;
+            for (auto &kv : g_variant_cache)
+                if (kv.first == key) x.es = kv.second;
+            if (launch_line_batch(x.dim, x.n, x.precision, x.es, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr,
+                                  &x.epb) != DGX_OK) {
+                x.es = 1;
+                launch_line_batch(x.dim, x.n, x.precision, 1, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr, &x.epb);
+            }
+            groups.push_back(x);
+            g = &groups.back();
+        }
+        g->members.push_back(k);
+        grouped[k] = true;
+    }
+    // pack the tables
+    int64_t bytes = 0;
+    for (G &g : groups) {
+        g.nblocks = 0;
+        for (int k : g.members) g.nblocks += (s->active[k].c->ncells + g.epb - 1) / g.epb;
+        g.off_blocks = bytes;
+        bytes = align256(bytes + sizeof(BatchBlock) * g.nblocks);
+        g.off_pcs = bytes;
+        bytes = align256(bytes + sizeof(PhysConst) * g.members.size());
+        g.off_args = bytes;
+        bytes = align256(bytes + sizeof(StageArgs) * g.members.size() * S);
+    }
+    if (bytes > s->stage_cap) {
+        hipHostFree(s->stage_host);
+        hipFree(s->stage_dev);
+        s->stage_host = nullptr;
+        s->stage_dev = nullptr;
+        const int64_t cap = std::max<int64_t>(bytes * 2, 1 << 20);
+        SHIPCHK(hipHostMalloc((void **)&s->stage_host, cap, hipHostMallocDefault));
+        SHIPCHK(hipMalloc((void **)&s->stage_dev, cap));
+        s->stage_cap = cap;
+    }
+    for (G &g : groups) {
+        BatchBlock *bb = (BatchBlock *)(s->stage_host + g.off_blocks);
+        PhysConst *pp = (PhysConst *)(s->stage_host + g.off_pcs);
+        StageArgs *aa = (StageArgs *)(s->stage_host + g.off_args);
+        int nb = 0;
+        const int m = (int)g.members.size();
+        for (int slot = 0; slot < m; ++slot) {
+            dgx_ctx *c = s->active[g.members[slot]].c;
+            for (int e = 0; e < c->ncells; e += g.epb) bb[nb++] = BatchBlock{slot, e};
+            pp[slot] = c->pc;
+            for (int st = 0; st < S; ++st) aa[st * m + slot] = args[g.members[slot]][st];
+        }
+    }
+    if (bytes) SHIPCHK(hipMemcpyAsync(s->stage_dev, s->stage_host, bytes, hipMemcpyHostToDevice, s->stream));
+    for (int st = 0; st < S; ++st) {
+        for (G &g : groups) {
+            const int m = (int)g.members.size();
+            const Ops1D &ops = s->active[g.members[0]].c->ops;
+            int rc = launch_line_batch(g.dim, g.n, g.precision, g.es, (const BatchBlock *)(s->stage_dev + g.off_blocks),
+                                       g.nblocks, (const StageArgs *)(s->stage_dev + g.off_args) + st * m,
+                                       (const PhysConst *)(s->stage_dev + g.off_pcs), ops, s->stream, nullptr);
+            if (rc) {
+                s->err = "batched stage launch failed";
+                return rc;
+            }
+        }
+        for (int k = 0; k < na; ++k) {
+            if (grouped[k]) continue;
+            dgx_ctx *c = s->active[k].c;
+            int rc = DGX_OK;
+            if (c->P.pde == DGX_NAVIER_STOKES) rc = launch_aux(c, args[k][st].u);
+            if (!rc) rc = launch_stage(c, args[k][st]);
+            if (rc) {
+                s->err = c->err;
+                return rc;
+            }
+        }
+    }
+    SHIPCHK(hipStreamSynchronize(s->stream));  // also releases the staging buffer
+    for (int k = 0; k < na; ++k) {
+        Request &q = s->active[k];
+        ++q.c->iteration;
+        q.c->current_time += step[k];
+        ++q.steps;
+    }
+    return DGX_OK;
+}
+
+static int server_retire(dgx_server *s)
+{
+    for (size_t k = 0; k < s->active.size();) {
+        Request &q = s->active[k];
+        if (q.c->current_time < q.r.final_time) {
+            ++k;
+            continue;
+        }
+        Result res;
+        res.u.resize((size_t)q.c->ndofs);
+        SHIPCHK(hipMemcpyAsync(res.u.data(), q.c->u_dev, sizeof(double) * q.c->ndofs, hipMemcpyDeviceToHost,
+                               s->stream));
+        SHIPCHK(hipStreamSynchronize(s->stream));
+        res.latency = std::chrono::duration<double>(Clock::now() - q.submitted).count();
+        res.steps = q.steps;
+        res.dof_updates = (double)q.c->ndofs * s->B.s * q.steps;
+        s->done[q.id] = std::move(res);
+        q.c->stream = nullptr;
+        dgx_destroy(q.c);
+        s->used -= q.bytes;
+        s->active.erase(s->active.begin() + k);
+    }
+    return DGX_OK;
+}
+
+extern "C" {
+
+int64_t dgx_request_n_dofs(const dgx_request *r) { return r ? request_ndofs(r->params) : 0; }
+
+int64_t dgx_request_bytes(const dgx_request *r, int method)
+{
+    if (!r) return 0;
+    Butcher B;
+    const int nregs = get_butcher(method, B) ? 3 + B.s : 4;
+    const int64_t nd = request_ndofs(r->params);
+    const int64_t N = r->params.n_cells_1d, ncells = (r->params.dim == 3) ? N * N * N : N * N;
+    int64_t b = (int64_t)sizeof(double) * nd * (2 + nregs) + (int64_t)sizeof(int) * ncells * 2 * r->params.dim;
+    if (r->params.pde == DGX_NAVIER_STOKES) b += (int64_t)sizeof(double) * nd * r->params.dim;
+    return b;
+}
+
+int dgx_server_create(int64_t hbm_budget_bytes, int max_active, int method, dgx_server **out)
+{
+    if (!out || hbm_budget_bytes <= 0 || max_active < 1) return DGX_ERR_INVALID;
+    *out = nullptr;
+    Butcher B;
+    if (!get_butcher(method, B) || B.s < 2) return DGX_ERR_UNSUPPORTED;
+    dgx_server *s = new dgx_server();
+    s->budget = hbm_budget_bytes;
+    s->max_active = max_active;
+    s->method = method;
+    s->B = B;
+    if (hipStreamCreate(&s->stream) != hipSuccess) {
+        delete s;
+        return DGX_ERR_HIP;
+    }
+    *out = s;
+    return DGX_OK;
+}
+
+int dgx_server_destroy(dgx_server *s)
+{
+    if (!s) return DGX_OK;
+    hipStreamSynchronize(s->stream);
+    for (Request &q : s->active) {
+        q.c->stream = nullptr;
+        dgx_destroy(q.c);
+    }
+    hipHostFree(s->stage_host);
+    hipFree(s->stage_dev);
+    hipStreamDestroy(s->stream);
+    delete s;
+    return DGX_OK;
+}
+
+const char *dgx_server_last_error(const dgx_server *s) { return s ? s->err.c_str() : "null server"; }
+
+int dgx_server_submit(dgx_server *s, const dgx_request *r, int64_t *id)
+{
+    if (!s || !r || !id) return DGX_ERR_INVALID;
+    Request q;
+    q.r = *r;
+    q.bytes = dgx_request_bytes(r, s->method);
+    if (q.bytes > s->budget || !(r->final_time > 0.0) || !(r->cfl > 0.0) || !(r->wave_speed > 0.0)) {
+        s->err = "request cannot be served (exceeds the HBM budget or has no positive time step / final time)";
+        return DGX_ERR_INVALID;
+    }
+    q.id = s->next_id++;
+    q.submitted = Clock::now();
+    s->queue.push_back(q);
+    *id = q.id;
+    return DGX_OK;
+}
+
+int dgx_server_step(dgx_server *s, int *n_active, int *n_queued, int *n_done)
+{
+    if (!s) return DGX_ERR_INVALID;
+    int rc = server_admit(s);
+    if (!rc && !s->active.empty()) rc = server_advance(s);
+    if (!rc) rc = server_retire(s);
+    if (n_active) *n_active = (int)s->active.size();
+    if (n_queued) *n_queued = (int)s->queue.size();
+    if (n_done) *n_done = (int)s->done.size();
+    return rc;
+}
+
+int dgx_server_result(dgx_server *s, int64_t id, double *host_u, double *latency_s, int64_t *steps,
+                      double *dof_updates)
+{
+    if (!s) return DGX_ERR_INVALID;
+    auto it = s->done.find(id);
+    if (it == s->done.end()) return DGX_ERR_STATE;
+    if (host_u) std::memcpy(host_u, it->second.u.data(), sizeof(double) * it->second.u.size());
+    if (latency_s) *latency_s = it->second.latency;
+    if (steps) *steps = it->second.steps;
+    if (dof_updates) *dof_updates = it->second.dof_updates;
+    return DGX_OK;
+}
+
+int dgx_server_release(dgx_server *s, int64_t id)
+{
+    if (!s) return DGX_ERR_INVALID;
+    return s->done.erase(id) ? DGX_OK : DGX_ERR_STATE;
+}
+
+int dgx_server_memory(const dgx_server *s, int64_t *in_use, int64_t *budget)
+{
+    if (!s) return DGX_ERR_INVALID;
+    if (in_use) *in_use = s->used;
+    if (budget) *budget = s->budget;
     return DGX_OK;
 }
 
