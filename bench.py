@@ -28,6 +28,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "line_kernel_tgv_p3_32cubed_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the mixed stage kernel from the committed rocprofv3 counter
+    summary (read requests by size + WRITE_SIZE, calibrated on a plain copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def workload_params(precision):
@@ -79,6 +93,94 @@ def run_gpu(precision, steps, warmup):
                 ndofs=d.ndofs, stages=stages, ke=ke)
 
 
+MACH_VORTEX = 1.195228609334  # sqrt(2/gamma): p_inf = 1/(gamma M^2) = 1/2 for the isentropic vortex
+
+
+def batched_vortex(n_inst=256, steps=20, seed=2024, precision="mixed"):
+    """Batched uneven instances: 2D isentropic vortex on [-5,5]^2, N ~ U{8..64} elements per side,
+    p ~ U{2..6}, strength ~ U[1,5], advection ~ U[0.5,1]^2 (seeded), CH split form + EC/Roe surface
+    flux, RK4; all instances advance with one graph launch per step.  Returns DOF-updates/s over
+    the timed steps."""
+    import numpy as np
+    import philip_amd as P
+    rng = np.random.default_rng(seed)
+    prec = P.MIXED if precision == "mixed" else P.FP64
+    kws, ics = [], []
+    for _ in range(n_inst):
+        kws.append(dict(dim=2, poly_degree=int(rng.integers(2, 7)), n_cells_1d=int(rng.integers(8, 65)),
+                        domain_left=-5.0, domain_right=5.0, mach_inf=MACH_VORTEX, use_split_form=1,
+                        two_point_flux=P.CH, conv_flux=P.TWO_POINT_ROE, precision=prec))
+        ics.append((float(rng.uniform(1, 5)), *(float(x) for x in rng.uniform(0.5, 1.0, 2))))
+    b = P.Batch(kws)
+    dts = []
+    for d, ic in zip(b.instances, ics):
+        d.set_initial_condition_vortex(*ic)
+        dts.append(d.constant_time_step(0.05) / 2.0)
+    b.allocate("rk4", dts)
+    b.step_in_time(2)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step_in_time(steps)
+    b.synchronize()
+    t = time.perf_counter() - t0
+    total = sum(d.ndofs for d in b.instances)
+    b.close()
+    return dict(value=total * 4 * steps / t, unit="DOF-updates/s", instances=n_inst, dof_total=total,
+                steps=steps, ms_per_step=1e3 * t / steps, precision=precision,
+                workload="256 independent 2D vortex solves, N~U{8..64}, p~U{2..6}, RK4, one graph launch per step")
+
+
+def serving(n_req=48, rate=40.0, seed=77, precision="mixed", time_scale=0.1):
+    """Serving stream: Poisson arrivals (rate req/s) of parametric solves, 70% 2D vortex
+    (N~U{8..64}, p~U{2..6}) and 30% 3D Euler TGV (M~U[0.05,0.3], p~U{2..5}, 4^3..16^3), final
+    time ~U[0.5,2] x time_scale; HBM budget 64 GiB.  Latency = arrival to result on the host."""
+    import numpy as np
+    import philip_amd as P
+    rng = np.random.default_rng(seed)
+    prec = P.MIXED if precision == "mixed" else P.FP64
+    reqs = []
+    for _ in range(n_req):
+        tf = float(rng.uniform(0.5, 2.0)) * time_scale
+        if rng.uniform() < 0.7:
+            kw = dict(dim=2, poly_degree=int(rng.integers(2, 7)), n_cells_1d=int(rng.integers(8, 65)),
+                      domain_left=-5.0, domain_right=5.0, mach_inf=MACH_VORTEX, use_split_form=1,
+                      two_point_flux=P.CH, conv_flux=P.TWO_POINT_ROE, precision=prec)
+            vort = (float(rng.uniform(1, 5)), *(float(x) for x in rng.uniform(0.5, 1.0, 2)))
+            reqs.append(P.request(tf, 0.05, 2.0, P.IC_VORTEX, vort, **kw))
+        else:
+            m = float(rng.uniform(0.05, 0.3))
+            kw = dict(dim=3, poly_degree=int(rng.integers(2, 6)), n_cells_1d=int(rng.choice([4, 8, 16])),
+                      mach_inf=m, use_split_form=1, two_point_flux=P.CH, conv_flux=P.TWO_POINT_ROE, precision=prec)
+            reqs.append(P.request(tf, 0.1, 1.0 + 1.0 / m, P.IC_TGV, **kw))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    s = P.Server(64 << 30, max_active=256)
+    ids, delay = [], []
+    t0 = time.perf_counter()
+    k = 0
+    done = 0
+    while done < n_req:
+        now = time.perf_counter() - t0
+        while k < n_req and arrivals[k] <= now:
+            ids.append(s.submit(reqs[k]))
+            delay.append(now - arrivals[k])  # host polling delay, added to the latency
+            k += 1
+        a, q, done = s.step()
+        if a == 0 and q == 0 and k < n_req:
+            time.sleep(max(0.0, arrivals[k] - (time.perf_counter() - t0)))
+    wall = time.perf_counter() - t0
+    lat, dofu = [], 0.0
+    for rid, dl in zip(ids, delay):
+        _, l, _, du = s.result(rid)
+        lat.append(l + dl)
+        dofu += du
+    s.close()
+    lat = np.array(lat)
+    return dict(p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                throughput=dofu / wall, unit="DOF-updates/s", requests=n_req, rate_per_s=rate, wall_s=wall,
+                precision=precision,
+                workload=f"70% 2D vortex / 30% 3D TGV, final time U[0.5,2]x{time_scale}, RK4, HBM budget 64 GiB")
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of known size (read nbytes + write nbytes per launch): the
     reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to HBM bytes."""
@@ -105,6 +207,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances line")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream line")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -136,7 +240,8 @@ def main():
                    "dof": ndofs, "stages_per_step": r["stages"], "precision": args.precision,
                    "kinetic_energy_after_run": r["ke"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if args.precision == "mixed" else None,
                      "kernel": "line_kernel<3,4,mixed>", "avg_launch_ms": r["avg_ms"],
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
@@ -144,6 +249,10 @@ def main():
         r64 = run_gpu("fp64", max(5, args.steps // 4), 2)
         line["fp64_path"] = {"value": r64["value"], "avg_launch_ms": r64["avg_ms"],
                              "roofline_frac": bytes_per_launch / (r64["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if not args.no_batch:
+        line["batched"] = batched_vortex()
+    if not args.no_serving:
+        line["serving"] = serving()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_threads)
     print(json.dumps(line), flush=True)
