@@ -202,85 +202,146 @@ int launch_aux(dgx_ctx *c, const double *u)
 // ---------------------------------------------------------------------------
 constexpr int IQ_THREADS = 256;
 
-template <int D>
+// Sum-factorised evaluation: per cell, density and momentum and their
+// reference gradients at the QGauss(nx)^dim points are built direction by
+// direction (x pass in LDS, y pass per quadrature column in registers, z pass
+// per point), then 1/2 rho |u|^2, 1/2 rho |omega|^2 and 1/2 |u|^2 are
+// accumulated with the tensor weights times det J.
+template <int D, int N>
 __global__ void __launch_bounds__(IQ_THREADS)
-integrated_kernel(const double *__restrict__ u, const double *__restrict__ ops, int n, int nx, int ncells,
-                  double inv_h, double detJ, double *__restrict__ partial)
+integrated_kernel(const double *__restrict__ u, const double *__restrict__ ops, int nx, int ncells, double inv_h,
+                  double detJ, double *__restrict__ partial)
 {
     constexpr int NS = D + 2;
+    constexpr int NV = D + 1;  // density and momentum
+    constexpr int n = N;
     extern __shared__ double smem[];
-    const int Np = (D == 3) ? n * n * n : n * n;
-    double *sU = smem;                       // [NS][Np]
-    double *sB = sU + NS * Np;               // [nx][n]
-    double *sG = sB + nx * n;                // [nx][n]
+    constexpr int Np = (D == 3) ? n * n * n : n * n;
+    constexpr int nr = (D == 3) ? n * n : n;  // nodes per x line
+    double *sU = smem;                       // [NV][Np]
+    double *sB = sU + NV * Np;               // [nx][n] values
+    double *sG = sB + nx * n;                // [nx][n] derivatives
     double *sW = sG + nx * n;                // [nx]
-    double *red = sW + nx;                   // [3][IQ_THREADS]
+    double *sT = sW + nx;                    // [NV][2][nx][nr]: x pass with B and G
+    double *red = sT + NV * 2 * nx * nr;     // [3][IQ_THREADS]
     const int e = blockIdx.x;
-    for (int t = threadIdx.x; t < NS * Np; t += blockDim.x) {
-        const int s = t / Np, i = t % Np;
-        sU[t] = u[((long)e * NS + s) * Np + i];
+    for (int t = threadIdx.x; t < NV * Np; t += blockDim.x) {
+        const int sv = t / Np, i = t % Np;
+        sU[t] = u[((long)e * NS + sv) * Np + i];
     }
     for (int t = threadIdx.x; t < 2 * nx * n + nx; t += blockDim.x) sB[t] = ops[t];
     __syncthreads();
-    const int nq = (D == 3) ? nx * nx * nx : nx * nx;
+    // x pass: T[s][k][qx][r] = sum_a {B,G}[qx][a] u[s][a + n r]
+    for (int t = threadIdx.x; t < NV * 2 * nx * nr; t += blockDim.x) {
+        const int r = t % nr, qx = (t / nr) % nx, k = (t / (nr * nx)) % 2, sv = t / (nr * nx * 2);
+        const double *m = (k ? sG : sB) + qx * n;
+        double acc = 0.0;
+        for (int a = 0; a < n; ++a) acc += m[a] * sU[sv * Np + a + n * r];
+        sT[t] = acc;
+    }
+    __syncthreads();
     double ke = 0.0, ens = 0.0, ike = 0.0;
-    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-        const int qx = q % nx, qy = (q / nx) % nx, qz = (D == 3) ? q / (nx * nx) : 0;
-        double val[NS], gr[NS][D];
-        for (int s = 0; s < NS; ++s) {
-            val[s] = 0.0;
-            for (int k = 0; k < D; ++k) gr[s][k] = 0.0;
-        }
-        for (int i = 0; i < Np; ++i) {
-            const int a = i % n, b = (i / n) % n, cz = (D == 3) ? i / (n * n) : 0;
-            const double bx = sB[qx * n + a], by = sB[qy * n + b], bz = (D == 3) ? sB[qz * n + cz] : 1.0;
-            const double gx = sG[qx * n + a], gy = sG[qy * n + b], gz = (D == 3) ? sG[qz * n + cz] : 0.0;
-            const double phi = bx * by * bz;
-            const double dphi[3] = {gx * by * bz, bx * gy * bz, bx * by * gz};
-            for (int s = 0; s < NS; ++s) {
-                const double us = sU[s * Np + i];
-                val[s] += phi * us;
-                for (int k = 0; k < D; ++k) gr[s][k] += dphi[k] * us;
+    for (int col = threadIdx.x; col < nx * nx; col += blockDim.x) {
+        const int qx = col % nx, qy = col / nx;
+        // y pass for this (qx, qy): BB, BG (B in x, G in y), GB (G in x) per state and z node
+        double BB[NV][N], BG[NV][N], GB[NV][N];
+        constexpr int nz = (D == 3) ? n : 1;
+        for (int sv = 0; sv < NV; ++sv)
+            for (int c = 0; c < nz; ++c) {
+                double bb = 0.0, bg = 0.0, gb = 0.0;
+                for (int b = 0; b < n; ++b) {
+                    const double tb = sT[((sv * 2 + 0) * nx + qx) * nr + b + n * c];
+                    const double tg = sT[((sv * 2 + 1) * nx + qx) * nr + b + n * c];
+                    bb += sB[qy * n + b] * tb;
+                    bg += sG[qy * n + b] * tb;
+                    gb += sB[qy * n + b] * tg;
+                }
+                BB[sv][c] = bb;
+                BG[sv][c] = bg;
+                GB[sv][c] = gb;
             }
+        const int nqz = (D == 3) ? nx : 1;
+        for (int qz = 0; qz < nqz; ++qz) {
+            double val[NV], gr[NV][D];
+            for (int sv = 0; sv < NV; ++sv) {
+                if (D == 3) {
+                    double v = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+                    for (int c = 0; c < n; ++c) {
+                        const double bz = sB[qz * n + c], dz = sG[qz * n + c];
+                        v += bz * BB[sv][c];
+                        gx += bz * GB[sv][c];
+                        gy += bz * BG[sv][c];
+                        gz += dz * BB[sv][c];
+                    }
+                    val[sv] = v;
+                    gr[sv][0] = gx;
+                    gr[sv][1] = gy;
+                    gr[sv][D - 1] = gz;
+                } else {
+                    val[sv] = BB[sv][0];
+                    gr[sv][0] = GB[sv][0];
+                    gr[sv][1] = BG[sv][0];
+                }
+            }
+            const double rho = val[0];
+            double vel[D], vel2 = 0.0;
+            for (int k = 0; k < D; ++k) {
+                vel[k] = val[1 + k] / rho;
+                vel2 = vel2 + vel[k] * vel[k];
+            }
+            double vg[D][D];
+            for (int a2 = 0; a2 < D; ++a2)
+                for (int b2 = 0; b2 < D; ++b2)
+                    vg[a2][b2] = (gr[1 + a2][b2] * inv_h - vel[a2] * (gr[0][b2] * inv_h)) / rho;
+            double om2;
+            if (D == 2) {
+                const double oz = vg[1][0] - vg[0][1];
+                om2 = oz * oz;
+            } else {
+                const double ox = vg[2 % D][1 % D] - vg[1 % D][2 % D];
+                const double oy = vg[0][2 % D] - vg[2 % D][0];
+                const double oz = vg[1][0] - vg[0][1];
+                om2 = ox * ox + oy * oy + oz * oz;
+            }
+            double w = sW[qx] * sW[qy];
+            if (D == 3) w *= sW[qz];
+            w *= detJ;
+            ke += 0.5 * rho * vel2 * w;
+            ens += 0.5 * rho * om2 * w;
+            ike += 0.5 * vel2 * w;
         }
-        const double rho = val[0];
-        double vel[D], vel2 = 0.0;
-        for (int k = 0; k < D; ++k) {
-            vel[k] = val[1 + k] / rho;
-            vel2 = vel2 + vel[k] * vel[k];
-        }
-        double vg[D][D];
-        for (int a2 = 0; a2 < D; ++a2)
-            for (int b2 = 0; b2 < D; ++b2)
-                vg[a2][b2] = (gr[1 + a2][b2] * inv_h - vel[a2] * (gr[0][b2] * inv_h)) / rho;
-        double om2;
-        if (D == 2) {
-            const double oz = vg[1][0] - vg[0][1];
-            om2 = oz * oz;
-        } else {
-            const double ox = vg[2 % D][1 % D] - vg[1 % D][2 % D];
-            const double oy = vg[0][2 % D] - vg[2 % D][0];
-            const double oz = vg[1][0] - vg[0][1];
-            om2 = ox * ox + oy * oy + oz * oz;
-        }
-        double w = sW[qx] * sW[qy];
-        if (D == 3) w *= sW[qz];
-        w *= detJ;
-        ke += 0.5 * rho * vel2 * w;
-        ens += 0.5 * rho * om2 * w;
-        ike += 0.5 * vel2 * w;
     }
     red[threadIdx.x] = ke;
     red[IQ_THREADS + threadIdx.x] = ens;
     red[2 * IQ_THREADS + threadIdx.x] = ike;
     __syncthreads();
-    for (int s = IQ_THREADS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int f = 0; f < 3; ++f) red[f * IQ_THREADS + threadIdx.x] += red[f * IQ_THREADS + threadIdx.x + s];
+    for (int sh = IQ_THREADS / 2; sh > 0; sh >>= 1) {
+        if ((int)threadIdx.x < sh)
+            for (int f = 0; f < 3; ++f) red[f * IQ_THREADS + threadIdx.x] += red[f * IQ_THREADS + threadIdx.x + sh];
         __syncthreads();
     }
     if (threadIdx.x == 0)
         for (int f = 0; f < 3; ++f) partial[(long)e * 3 + f] = red[f * IQ_THREADS];
+}
+
+// Fixed-order reduction of the per-cell partials (deterministic): thread t sums
+// cells t, t+256, ... in order, then a tree over the threads.
+__global__ void __launch_bounds__(IQ_THREADS)
+integrated_sum_kernel(const double *__restrict__ partial, int ncells, double *__restrict__ out3)
+{
+    __shared__ double red[3][IQ_THREADS];
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int e = threadIdx.x; e < ncells; e += IQ_THREADS)
+        for (int f = 0; f < 3; ++f) s[f] += partial[(long)e * 3 + f];
+    for (int f = 0; f < 3; ++f) red[f][threadIdx.x] = s[f];
+    __syncthreads();
+    for (int sh = IQ_THREADS / 2; sh > 0; sh >>= 1) {
+        if ((int)threadIdx.x < sh)
+            for (int f = 0; f < 3; ++f) red[f][threadIdx.x] += red[f][threadIdx.x + sh];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int f = 0; f < 3; ++f) out3[f] = red[f][0];
 }
 
 int launch_integrated(dgx_ctx *c, const double *u, int overint, double *out4)
@@ -306,31 +367,39 @@ int launch_integrated(dgx_ctx *c, const double *u, int overint, double *out4)
         hipFree(c->iq_ops);
         if (hipMalloc(&c->iq_ops, sizeof(double) * h.size()) != hipSuccess) return DGX_ERR_HIP;
         hipMemcpy(c->iq_ops, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice);
-        if (!c->iq_partial) {
-            if (hipMalloc(&c->iq_partial, sizeof(double) * 3 * c->ncells) != hipSuccess) return DGX_ERR_HIP;
+        if (!c->iq_partial) {  // per-cell partials, then the 3 sums
+            if (hipMalloc(&c->iq_partial, sizeof(double) * (3 * (size_t)c->ncells + 3)) != hipSuccess)
+                return DGX_ERR_HIP;
         }
         c->iq_overint = overint;
     }
-    const size_t shmem = sizeof(double) * ((size_t)c->ns * c->Np + 2 * nx * n + nx + 3 * IQ_THREADS);
+    const int nr = (c->dim == 3) ? n * n : n;
+    const size_t shmem = sizeof(double) * ((size_t)(c->dim + 1) * c->Np + 2 * nx * n + nx +
+                                           (size_t)(c->dim + 1) * 2 * nx * nr + 3 * IQ_THREADS);
     const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
-    if (c->dim == 3)
-        hipLaunchKernelGGL(integrated_kernel<3>, dim3(c->ncells), dim3(IQ_THREADS), shmem, c->stream, u, c->iq_ops, n, nx,
-                           c->ncells, 1.0 / c->h, detJ, c->iq_partial);
-    else
-        hipLaunchKernelGGL(integrated_kernel<2>, dim3(c->ncells), dim3(IQ_THREADS), shmem, c->stream, u, c->iq_ops, n, nx,
-                           c->ncells, 1.0 / c->h, detJ, c->iq_partial);
+    switch (c->dim * 16 + n) {
+#define IQ_CASE(DD, NN)                                                                                          \
+    case DD * 16 + NN:                                                                                            \
+        hipLaunchKernelGGL((integrated_kernel<DD, NN>), dim3(c->ncells), dim3(IQ_THREADS), shmem, c->stream, u,   \
+                           c->iq_ops, nx, c->ncells, 1.0 / c->h, detJ, c->iq_partial);                            \
+        break;
+        IQ_CASE(3, 2) IQ_CASE(3, 3) IQ_CASE(3, 4) IQ_CASE(3, 5) IQ_CASE(3, 6) IQ_CASE(3, 7)
+        IQ_CASE(2, 2) IQ_CASE(2, 3) IQ_CASE(2, 4) IQ_CASE(2, 5) IQ_CASE(2, 6) IQ_CASE(2, 7)
+#undef IQ_CASE
+    default: c->err = "unsupported polynomial degree"; return DGX_ERR_UNSUPPORTED;
+    }
     int rc = check_launch(c, "integrated_kernel");
     if (rc) return rc;
-    std::vector<double> part((size_t)3 * c->ncells);
-    if (hipMemcpyAsync(part.data(), c->iq_partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream) !=
-            hipSuccess ||
+    double *sums = c->iq_partial + 3 * (size_t)c->ncells;
+    hipLaunchKernelGGL(integrated_sum_kernel, dim3(1), dim3(IQ_THREADS), 0, c->stream, c->iq_partial, c->ncells, sums);
+    rc = check_launch(c, "integrated_sum_kernel");
+    if (rc) return rc;
+    double s[3] = {0, 0, 0};
+    if (hipMemcpyAsync(s, sums, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
         c->err = "integrated quantities copy failed";
         return DGX_ERR_HIP;
     }
-    double s[3] = {0, 0, 0};
-    for (int e = 0; e < c->ncells; ++e)
-        for (int f = 0; f < 3; ++f) s[f] += part[(size_t)e * 3 + f];
     double vol = 1.0;
     for (int k = 0; k < c->dim; ++k) vol *= (c->P.domain_right - c->P.domain_left);
     out4[0] = s[0] / vol;
