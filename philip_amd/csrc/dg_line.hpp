@@ -34,8 +34,10 @@
 namespace dgx {
 
 // Issue the loads of the low-storage RK registers S2, S3 at the start of the
-// kernel (held in registers through the line phase) instead of in the epilogue.
-constexpr bool LINE_PREFETCH_RK = false;
+// kernel (held in registers through the line phase) instead of in the epilogue:
+// on for the mixed path (its register budget keeps 4 waves/SIMD), off for FP64.
+constexpr bool LINE_PREFETCH_RK_F = true;
+constexpr bool LINE_PREFETCH_RK_D = false;
 // ES: elements per block relative to the 256-node default (ES > 0: ES times,
 // ES < 0: 1/|ES|); per-element strides do not depend on it.
 template <int D, int N, typename T, int ES = 1> struct LineGeo {
@@ -257,13 +259,22 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     const int le = tid / NP, i = tid % NP;
     const int e = blk0 + le;
     const bool active = (le < EPB) && (e < a.ncells);
+    // neighbour indices first: the trace loads depend on them
+    int nbv[NFI];
+#pragma unroll
+    for (int j = 0; j < NFI; ++j) {
+        const int t = tid + j * BLOCK;
+        const int fe = blk0 + t / LG::NFN;
+        nbv[j] = (t < LG::FITEMS && fe < a.ncells) ? a.nbr[fe * 2 * D + (t % LG::NFN) / NL] : 0;
+    }
     double ud[NS];
     double un[NFI][NS];
     int fdst[NFI];
     if (active)
         for (int s = 0; s < NS; ++s) ud[s] = a.u[((long)e * NS + s) * NP + i];
     double pf2[NS], pf3[NS];
-    const bool pf = LINE_PREFETCH_RK && active && a.mode == MODE_LSRK && !a.lsrk_first;
+    constexpr bool PREFETCH = FAST ? LINE_PREFETCH_RK_F : LINE_PREFETCH_RK_D;
+    const bool pf = PREFETCH && active && a.mode == MODE_LSRK && !a.lsrk_first;
     if (pf)
         for (int s = 0; s < NS; ++s) {
             const long idx = ((long)e * NS + s) * NP + i;
@@ -280,7 +291,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             const int fe = blk0 + fle;
             if (fe < a.ncells) {
                 const int f = r / NL, fn = r % NL;
-                const int nb = a.nbr[fe * 2 * D + f];
+                const int nb = nbv[j];
                 if (fn == 0) sNb[fle][f] = nb;
                 if (nb < blk0 || nb >= blk0 + EPB) {  // in-block neighbours are read from the node cache
                     const int fj = line_node<D, N>(f >> 1, fn, (f & 1) ? 0 : N - 1);
