@@ -281,8 +281,8 @@ int dgx_create(const dgx_params *p, dgx_ctx **out)
             if (hipMalloc(&c->aux_dev[k], bytes) != hipSuccess) return bad(DGX_ERR_HIP, "hipMalloc failed");
     if (hipMemcpy(c->nbr_dev, c->nbr_host.data(), sizeof(int) * c->nbr_host.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bad(DGX_ERR_HIP, "hipMemcpy failed");
-    hipMemset(c->u_dev, 0, bytes);
-    hipMemset(c->r_dev, 0, bytes);
+    if (hipMemset(c->u_dev, 0, bytes) != hipSuccess || hipMemset(c->r_dev, 0, bytes) != hipSuccess)
+        return bad(DGX_ERR_HIP, "hipMemset failed");
     *out = c;
     return DGX_OK;
 }
@@ -290,17 +290,17 @@ int dgx_create(const dgx_params *p, dgx_ctx **out)
 int dgx_destroy(dgx_ctx *c)
 {
     if (!c) return DGX_OK;
-    hipDeviceSynchronize();
-    hipFree(c->nbr_dev);
-    hipFree(c->u_dev);
-    hipFree(c->r_dev);
-    hipFree(c->minv_dev);
-    for (int k = 0; k < 3; ++k) hipFree(c->aux_dev[k]);
-    for (double *r : c->regs) hipFree(r);
-    hipFree(c->iq_ops);
-    hipFree(c->iq_partial);
-    if (c->timing.a) hipEventDestroy(c->timing.a);
-    if (c->timing.b) hipEventDestroy(c->timing.b);
+    (void)hipStreamSynchronize(c->stream);  // teardown: errors are not actionable here
+    (void)hipFree(c->nbr_dev);
+    (void)hipFree(c->u_dev);
+    (void)hipFree(c->r_dev);
+    (void)hipFree(c->minv_dev);
+    for (int k = 0; k < 3; ++k) (void)hipFree(c->aux_dev[k]);
+    for (double *r : c->regs) (void)hipFree(r);
+    (void)hipFree(c->iq_ops);
+    (void)hipFree(c->iq_partial);
+    if (c->timing.a) (void)hipEventDestroy(c->timing.a);
+    if (c->timing.b) (void)hipEventDestroy(c->timing.b);
     delete c;
     return DGX_OK;
 }
@@ -538,7 +538,7 @@ int dgx_allocate_ode_system(dgx_ctx *c, int method)
     if (c->batch_locked) return fail(c, DGX_ERR_STATE, "instance registers belong to a batch graph");
     Butcher B;
     Lsrk L;
-    for (double *r : c->regs) hipFree(r);
+    for (double *r : c->regs) (void)hipFree(r);
     c->regs.clear();
     int nregs;
     if (get_butcher(method, B)) nregs = 3 + B.s;   // acc, stage x2, k_j (only if referenced)
@@ -855,13 +855,13 @@ struct dgx_batch {
 static void batch_free_groups(dgx_batch *b)
 {
     for (BatchGroup &g : b->groups) {
-        hipFree(g.blocks);
-        hipFree(g.pcs);
-        for (StageArgs *a : g.args) hipFree(a);
+        (void)hipFree(g.blocks);
+        (void)hipFree(g.pcs);
+        for (StageArgs *a : g.args) (void)hipFree(a);
     }
     b->groups.clear();
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
 }
@@ -882,7 +882,7 @@ static int batch_blocks(dgx_batch *b, BatchGroup &g, int es)
         const dgx_ctx *c = b->inst[g.inst[slot]];
         for (int e = 0; e < c->ncells; e += epb) tab.push_back(BatchBlock{slot, e});
     }
-    hipFree(g.blocks);
+    (void)hipFree(g.blocks);
     g.blocks = nullptr;
     BHIPCHK(hipMalloc(&g.blocks, sizeof(BatchBlock) * tab.size()));
     BHIPCHK(hipMemcpy(g.blocks, tab.data(), sizeof(BatchBlock) * tab.size(), hipMemcpyHostToDevice));
@@ -933,9 +933,9 @@ static int batch_autotune(dgx_batch *b, BatchGroup &g, const Ops1D &ops)
             best = es;
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipFree(pa);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(pa);
     g_variant_cache.push_back({key, best});
     return batch_blocks(b, g, best);
 }
@@ -969,14 +969,14 @@ int dgx_batch_create(const dgx_params *params, int n, dgx_batch **out)
 int dgx_batch_destroy(dgx_batch *b)
 {
     if (!b) return DGX_OK;
-    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_free_groups(b);
     for (dgx_ctx *c : b->inst) {
         c->stream = nullptr;
         c->batch_locked = false;
         dgx_destroy(c);
     }
-    if (b->stream) hipStreamDestroy(b->stream);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return DGX_OK;
 }
@@ -1085,7 +1085,7 @@ int dgx_batch_allocate(dgx_batch *b, int method, const double *dt)
     hipGraph_t graph = nullptr;
     hipError_t ce = hipStreamEndCapture(b->stream, &graph);
     if (rc) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return rc;
     }
     if (ce != hipSuccess) {
@@ -1282,8 +1282,8 @@ static int server_advance(dgx_server *s)
         bytes = align256(bytes + sizeof(StageArgs) * g.members.size() * S);
     }
     if (bytes > s->stage_cap) {
-        hipHostFree(s->stage_host);
-        hipFree(s->stage_dev);
+        (void)hipHostFree(s->stage_host);
+        (void)hipFree(s->stage_dev);
         s->stage_host = nullptr;
         s->stage_dev = nullptr;
         const int64_t cap = std::max<int64_t>(bytes * 2, 1 << 20);
@@ -1402,14 +1402,14 @@ int dgx_server_create(int64_t hbm_budget_bytes, int max_active, int method, dgx_
 int dgx_server_destroy(dgx_server *s)
 {
     if (!s) return DGX_OK;
-    hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->stream);
     for (Request &q : s->active) {
         q.c->stream = nullptr;
         dgx_destroy(q.c);
     }
-    hipHostFree(s->stage_host);
-    hipFree(s->stage_dev);
-    hipStreamDestroy(s->stream);
+    (void)hipHostFree(s->stage_host);
+    (void)hipFree(s->stage_dev);
+    (void)hipStreamDestroy(s->stream);
     delete s;
     return DGX_OK;
 }

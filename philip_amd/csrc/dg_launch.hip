@@ -63,15 +63,20 @@ static int launch_stage_dn(dgx_ctx *c, const StageArgs &a)
 {
     using S = Geo<D, N>;
     const int blocks = (c->ncells + S::EPB - 1) / S::EPB;
-    if (c->timing.enabled) hipEventRecord(c->timing.a, c->stream);
+    if (c->timing.enabled && hipEventRecord(c->timing.a, c->stream) != hipSuccess) {
+        c->err = "hipEventRecord failed";
+        return DGX_ERR_HIP;
+    }
     stage_dn<D, N>(c, a, dim3(blocks), dim3(S::BLOCK));
     int rc = check_launch(c, "stage_kernel");
     if (rc) return rc;
     if (c->timing.enabled) {
-        hipEventRecord(c->timing.b, c->stream);
-        hipEventSynchronize(c->timing.b);
         float ms = 0.f;
-        hipEventElapsedTime(&ms, c->timing.a, c->timing.b);
+        if (hipEventRecord(c->timing.b, c->stream) != hipSuccess || hipEventSynchronize(c->timing.b) != hipSuccess ||
+            hipEventElapsedTime(&ms, c->timing.a, c->timing.b) != hipSuccess) {
+            c->err = "stage kernel timing failed";
+            return DGX_ERR_HIP;
+        }
         c->timing.sum_ms += ms;
         c->timing.n += 1;
     }
@@ -364,9 +369,13 @@ int launch_integrated(dgx_ctx *c, const double *u, int overint, double *out4)
             }
         }
         for (int q = 0; q < nx; ++q) h[2 * nx * n + q] = w[q];
-        hipFree(c->iq_ops);
-        if (hipMalloc(&c->iq_ops, sizeof(double) * h.size()) != hipSuccess) return DGX_ERR_HIP;
-        hipMemcpy(c->iq_ops, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice);
+        (void)hipFree(c->iq_ops);
+        c->iq_ops = nullptr;
+        if (hipMalloc(&c->iq_ops, sizeof(double) * h.size()) != hipSuccess ||
+            hipMemcpy(c->iq_ops, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            c->err = "integrated-quantity operators upload failed";
+            return DGX_ERR_HIP;
+        }
         if (!c->iq_partial) {  // per-cell partials, then the 3 sums
             if (hipMalloc(&c->iq_partial, sizeof(double) * (3 * (size_t)c->ncells + 3)) != hipSuccess)
                 return DGX_ERR_HIP;
