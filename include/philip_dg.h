@@ -144,6 +144,9 @@ double dgx_constant_time_step(const dgx_ctx *ctx, double cfl);
    with QGauss(p+1+overint): out[0]=kinetic energy, [1]=enstrophy, [2]=incompressible KE,
    [3]=vorticity-based dissipation rate; reductions in FP64 on the device. */
 int dgx_integrated_quantities(dgx_ctx *ctx, int overint, double *out4);
+/* PeriodicTurbulence::compute_current_integrated_numerical_entropy (periodic_turbulence.cpp:765-868):
+   sum over cells and GLL volume nodes of W det J (-rho ln(p rho^-gamma)), FP64 on the device. */
+int dgx_numerical_entropy(dgx_ctx *ctx, double *out);
 
 /* Blocks until the context's stream has finished all launched work. */
 int dgx_synchronize(dgx_ctx *ctx);

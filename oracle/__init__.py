@@ -95,6 +95,8 @@ def _load():
     lib.dgo_integrated_quantities.argtypes = [vp, dp, i, dp]
     lib.dgo_entropy_rate.restype = d
     lib.dgo_entropy_rate.argtypes = [vp, dp]
+    lib.dgo_numerical_entropy.restype = d
+    lib.dgo_numerical_entropy.argtypes = [vp, dp]
     lib.dgo_run.argtypes = [vp, i, d, d, dp, dp, i]
     lib.dgo_set_threads.argtypes = [i]
     return lib
@@ -300,6 +302,9 @@ class Oracle:
 
     def constant_time_step(self, cfl):
         return lib().dgo_constant_time_step(self.h, cfl)
+
+    def numerical_entropy(self, u):
+        return lib().dgo_numerical_entropy(self.h, _dp(np.ascontiguousarray(u, dtype=np.float64)))
 
     def vortex_time_step(self, cfl, speed):
         return lib().dgo_vortex_time_step(self.h, cfl, speed)

@@ -1499,7 +1499,7 @@ void dgo_apply_inverse_mass(const dgo_ctx *c, const double *in, double *out, int
 {
     /* apply_inverse_global_mass_matrix, Cartesian branch (dg_base.cpp:4208-4219); cDG/kDG => M^-1 */
     (void)aux;
-    const int ns = c->ns, Np = c->Np, n = c->n;
+    const int ns = c->ns, n = c->n;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(g_threads)
 #endif
@@ -1838,7 +1838,7 @@ void dgo_integrated_quantities(const dgo_ctx *c, const double *u, int overint, d
             for (int k = 0; k < d; ++k) vel2 = vel2 + vel[k] * vel[k];
             const double KE = 0.5 * st[0] * vel2;
             /* vorticity (navier_stokes.cpp:396-423) from primitive velocity gradient */
-            double vg[3][3];
+            double vg[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
             for (int a = 0; a < d; ++a)
                 for (int b = 0; b < d; ++b) vg[a][b] = (g[(1 + a) * d + b] - vel[a] * g[0 * d + b]) / st[0];
             double om[3] = {0, 0, 0};
@@ -1878,6 +1878,31 @@ double dgo_entropy_rate(dgo_ctx *c, const double *u)
             for (int s = 0; s < c->ns; ++s) acc += vs[s] * r[IDX(c, e, s, i)];
         }
     free(r);
+    return acc;
+}
+
+/* PeriodicTurbulence::compute_current_integrated_numerical_entropy
+   (periodic_turbulence.cpp:765-868): sum over cells and volume quadrature points of
+   w_q det J * (-rho s), s = ln(p rho^-gamma) (euler.cpp:437-447, 511-532), the solution
+   interpolated to the volume quadrature (identity for collocated GLL). */
+double dgo_numerical_entropy(const dgo_ctx *c, const double *u)
+{
+    const int ns = c->ns, Np = c->Np, Nq = (c->dim == 3) ? c->nq * c->nq * c->nq : c->nq * c->nq;
+    double *uq = (double *)malloc(sizeof(double) * ns * Nq);
+    double acc = 0.0;
+    for (int e = 0; e < c->ncells; ++e) {
+        for (int st = 0; st < ns; ++st) vol_apply(c, c->chi, c->nq, c->n, u + IDX(c, e, st, 0), uq + st * Nq, 0, 1.0);
+        for (int q = 0; q < Nq; ++q) {
+            double us[NS_MAX];
+            for (int st = 0; st < ns; ++st) us[st] = uq[st * Nq + q];
+            const double rho = us[0];
+            const double p = dgo_pressure(c, us);
+            const double ent = log(p * pow(rho, -c->gam));
+            acc += (-rho * ent) * c->wvol[q] * c->detJ;
+        }
+    }
+    free(uq);
+    (void)Np;
     return acc;
 }
 

@@ -17,7 +17,7 @@
  *                            src/numerical_flux/viscous_numerical_flux.cpp:144-328
  *   RK / low-storage RK      src/ode_solver/runge_kutta_ode_solver.cpp:15-110,
  *                            src/ode_solver/low_storage_runge_kutta_ode_solver.cpp:24-222,
- *                            src/ode_solver/runge_kutta_methods/*.cpp
+ *                            src/ode_solver/runge_kutta_methods/ (all tableaus)
  *   TGV / vortex / dt / KE   src/physics/initial_conditions/initial_condition_function.cpp:261-336,:643-690,
  *                            src/flow_solver/flow_solver_cases/periodic_turbulence.cpp:115-125,:439-671,
  *                            src/flow_solver/flow_solver.cpp:554-610
@@ -142,6 +142,8 @@ void dgo_integrated_quantities(const dgo_ctx *c, const double *u, int overint, d
 /* numerical entropy  sum_i w_i J_i (-rho s)/(gamma-1) at volume nodes (mass-lumped, reference
    periodic_entropy_tests style uses M-weighted v.u; see oracle source) */
 double dgo_entropy_rate(dgo_ctx *c, const double *u);
+/* integrated numerical entropy  sum w_q detJ (-rho ln(p rho^-gamma))  (periodic_turbulence.cpp:765-868) */
+double dgo_numerical_entropy(const dgo_ctx *c, const double *u);
 
 /* full flow-solver loop; returns number of steps; ke_hist gets KE after every step (may be NULL) */
 int dgo_run(dgo_ctx *c, int method, double dt_const, double final_time, double *u,

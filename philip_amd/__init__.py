@@ -26,7 +26,8 @@ EXPORTED = [
     "dgx_get_neighbors", "dgx_assemble_residual", "dgx_apply_inverse_mass_to_rhs",
     "dgx_assemble_residual_device", "dgx_rhs_device", "dgx_apply_inverse_mass_device",
     "dgx_allocate_ode_system", "dgx_step_in_time", "dgx_current_time", "dgx_current_iteration",
-    "dgx_set_track_error_estimate", "dgx_constant_time_step", "dgx_integrated_quantities", "dgx_set_stream",
+    "dgx_set_track_error_estimate", "dgx_constant_time_step", "dgx_integrated_quantities",
+    "dgx_numerical_entropy", "dgx_set_stream",
     "dgx_enable_timing", "dgx_kernel_timing", "dgx_set_initial_condition_tgv",
     "dgx_set_initial_condition_vortex", "dgx_synchronize",
     "dgx_batch_create", "dgx_batch_destroy", "dgx_batch_last_error", "dgx_batch_size", "dgx_batch_instance",
@@ -115,6 +116,7 @@ def load():
     lib.dgx_constant_time_step.restype = d
     lib.dgx_constant_time_step.argtypes = [vp, d]
     lib.dgx_integrated_quantities.argtypes = [vp, i, dp]
+    lib.dgx_numerical_entropy.argtypes = [vp, dp]
     lib.dgx_set_stream.argtypes = [vp, vp]
     lib.dgx_enable_timing.argtypes = [vp, i]
     lib.dgx_kernel_timing.argtypes = [vp, dp, P(i64)]
@@ -268,6 +270,11 @@ class DG:
         self._chk(load().dgx_integrated_quantities(self.h, overint, _dp(o)), "integrated_quantities")
         return dict(kinetic_energy=o[0], enstrophy=o[1], incompressible_kinetic_energy=o[2],
                     vorticity_dissipation=o[3])
+
+    def numerical_entropy(self):
+        o = np.zeros(1)
+        self._chk(load().dgx_numerical_entropy(self.h, _dp(o)), "numerical_entropy")
+        return float(o[0])
 
     def enable_timing(self, on=True):
         self._chk(load().dgx_enable_timing(self.h, int(on)), "enable_timing")

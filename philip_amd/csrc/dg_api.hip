@@ -774,6 +774,12 @@ int dgx_integrated_quantities(dgx_ctx *c, int overint, double *out4)
     return launch_integrated(c, c->u_dev, overint, out4);
 }
 
+int dgx_numerical_entropy(dgx_ctx *c, double *out)
+{
+    if (!c || !out) return DGX_ERR_INVALID;
+    return launch_entropy(c, c->u_dev, out);
+}
+
 int dgx_synchronize(dgx_ctx *c)
 {
     if (!c) return DGX_ERR_INVALID;

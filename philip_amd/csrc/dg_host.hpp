@@ -85,4 +85,5 @@ int launch_line_batch(int dim, int n, int precision, int es, const BatchBlock *b
                       const StageArgs *args, const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb);
 int launch_aux(dgx_ctx *c, const double *u);
 int launch_integrated(dgx_ctx *c, const double *u, int overint, double *host_out4);
+int launch_entropy(dgx_ctx *c, const double *u, double *host_out);
 }

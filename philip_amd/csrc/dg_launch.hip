@@ -329,6 +329,46 @@ integrated_kernel(const double *__restrict__ u, const double *__restrict__ ops, 
         for (int f = 0; f < 3; ++f) partial[(long)e * 3 + f] = red[f * IQ_THREADS];
 }
 
+// Integrated numerical entropy on collocated GLL nodes (periodic_turbulence.cpp:765-868,
+// euler.cpp:437-447): per cell sum_i W_i det J (-rho_i ln(p_i rho_i^-gamma)); the three
+// partial slots of a cell hold (entropy, 0, 0) so the fixed-order reduction below applies.
+__global__ void __launch_bounds__(IQ_THREADS)
+entropy_kernel(const double *__restrict__ u, int n, int dim, int ncells, double detJ, double gam, Ops1D ops,
+               double *__restrict__ partial)
+{
+    __shared__ double red[IQ_THREADS];
+    const int e = blockIdx.x;
+    const int Np = (dim == 3) ? n * n * n : n * n, ns = dim + 2;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < Np; i += blockDim.x) {
+        double us[5];
+        for (int s = 0; s < ns; ++s) us[s] = u[((long)e * ns + s) * Np + i];
+        const double rho = us[0];
+        double vel2 = 0.0;
+        for (int k = 0; k < dim; ++k) {
+            const double v = us[1 + k] / rho;
+            vel2 = vel2 + v * v;
+        }
+        const double p = (gam - 1.0) * (us[ns - 1] - 0.5 * rho * vel2);
+        const double ent = log(p * pow(rho, -gam));
+        const int a = i % n, b = (i / n) % n, c = i / (n * n);
+        double W = ops.w[a] * ops.w[b];
+        if (dim == 3) W *= ops.w[c];
+        acc += (-rho * ent) * W * detJ;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int sh = IQ_THREADS / 2; sh > 0; sh >>= 1) {
+        if ((int)threadIdx.x < sh) red[threadIdx.x] += red[threadIdx.x + sh];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[(long)e * 3] = red[0];
+        partial[(long)e * 3 + 1] = 0.0;
+        partial[(long)e * 3 + 2] = 0.0;
+    }
+}
+
 // Fixed-order reduction of the per-cell partials (deterministic): thread t sums
 // cells t, t+256, ... in order, then a tree over the threads.
 __global__ void __launch_bounds__(IQ_THREADS)
@@ -347,6 +387,31 @@ integrated_sum_kernel(const double *__restrict__ partial, int ncells, double *__
     }
     if (threadIdx.x == 0)
         for (int f = 0; f < 3; ++f) out3[f] = red[f][0];
+}
+
+int launch_entropy(dgx_ctx *c, const double *u, double *out)
+{
+    if (!c->iq_partial && hipMalloc(&c->iq_partial, sizeof(double) * (3 * (size_t)c->ncells + 3)) != hipSuccess) {
+        c->err = "hipMalloc failed";
+        return DGX_ERR_HIP;
+    }
+    const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
+    hipLaunchKernelGGL(entropy_kernel, dim3(c->ncells), dim3(IQ_THREADS), 0, c->stream, u, c->n, c->dim, c->ncells,
+                       detJ, c->P.gamma_gas, c->ops, c->iq_partial);
+    int rc = check_launch(c, "entropy_kernel");
+    if (rc) return rc;
+    double *sums = c->iq_partial + 3 * (size_t)c->ncells;
+    hipLaunchKernelGGL(integrated_sum_kernel, dim3(1), dim3(IQ_THREADS), 0, c->stream, c->iq_partial, c->ncells, sums);
+    rc = check_launch(c, "integrated_sum_kernel");
+    if (rc) return rc;
+    double s3[3];
+    if (hipMemcpyAsync(s3, sums, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        c->err = "numerical entropy copy failed";
+        return DGX_ERR_HIP;
+    }
+    *out = s3[0];
+    return DGX_OK;
 }
 
 int launch_integrated(dgx_ctx *c, const double *u, int overint, double *out4)

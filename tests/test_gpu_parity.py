@@ -195,6 +195,17 @@ def test_integrated_quantities_match_oracle():
         assert abs(qd[k] - qo[k]) <= 1e-13 * abs(qo[k]), k
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_numerical_entropy_matches_oracle(dim):
+    kw = dict(dim=dim, poly_degree=3, n_cells_1d=4) if dim == 3 else dict(
+        dim=2, poly_degree=4, n_cells_1d=5, domain_left=-5.0, domain_right=5.0, mach_inf=1.195228609334)
+    o, d = make_pair(**kw)
+    u = perturbed(o.ic_tgv() if dim == 3 else o.ic_vortex(strength=3.0, u_inf=0.5, v_inf=0.4), o, seed=dim)
+    d.set_solution(u)
+    ref = o.numerical_entropy(u)
+    assert abs(d.numerical_entropy() - ref) <= 1e-13 * abs(ref)
+
+
 def test_reference_golden_nsfr_gll_on_device():
     """The reference's own golden value, reproduced by the device path end to end
     (viscous_taylor_green_vortex_energy_check_nsfr_gll_flux_nodes_quick.prm:59-60,
