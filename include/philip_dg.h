@@ -147,6 +147,12 @@ int dgx_integrated_quantities(dgx_ctx *ctx, int overint, double *out4);
 /* PeriodicTurbulence::compute_current_integrated_numerical_entropy (periodic_turbulence.cpp:765-868):
    sum over cells and GLL volume nodes of W det J (-rho ln(p rho^-gamma)), FP64 on the device. */
 int dgx_numerical_entropy(dgx_ctx *ctx, double *out);
+/* PeriodicTurbulence::update_maximum_local_wave_speed (periodic_turbulence.cpp:371-422):
+   max of |u| + c over the QGauss(p+1+overint) points of every cell. */
+int dgx_max_wave_speed(dgx_ctx *ctx, int overint, double *out);
+/* PeriodicTurbulence::get_adaptive_time_step (periodic_turbulence.cpp:359-367):
+   dt = cfl * L / (n_dofs/nstate)^(1/dim) / max wave speed (overint 10, as the reference). */
+int dgx_adaptive_time_step(dgx_ctx *ctx, double cfl, double *dt);
 
 /* Blocks until the context's stream has finished all launched work. */
 int dgx_synchronize(dgx_ctx *ctx);

@@ -97,6 +97,10 @@ def _load():
     lib.dgo_entropy_rate.argtypes = [vp, dp]
     lib.dgo_numerical_entropy.restype = d
     lib.dgo_numerical_entropy.argtypes = [vp, dp]
+    lib.dgo_max_wave_speed.restype = d
+    lib.dgo_max_wave_speed.argtypes = [vp, dp, i]
+    lib.dgo_adaptive_time_step.restype = d
+    lib.dgo_adaptive_time_step.argtypes = [vp, dp, d]
     lib.dgo_run.argtypes = [vp, i, d, d, dp, dp, i]
     lib.dgo_set_threads.argtypes = [i]
     return lib
@@ -305,6 +309,12 @@ class Oracle:
 
     def numerical_entropy(self, u):
         return lib().dgo_numerical_entropy(self.h, _dp(np.ascontiguousarray(u, dtype=np.float64)))
+
+    def max_wave_speed(self, u, overint=10):
+        return lib().dgo_max_wave_speed(self.h, _dp(np.ascontiguousarray(u, dtype=np.float64)), overint)
+
+    def adaptive_time_step(self, u, cfl):
+        return lib().dgo_adaptive_time_step(self.h, _dp(np.ascontiguousarray(u, dtype=np.float64)), cfl)
 
     def vortex_time_step(self, cfl, speed):
         return lib().dgo_vortex_time_step(self.h, cfl, speed)

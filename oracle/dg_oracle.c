@@ -1881,6 +1881,49 @@ double dgo_entropy_rate(dgo_ctx *c, const double *u)
     return acc;
 }
 
+/* PeriodicTurbulence::update_maximum_local_wave_speed (periodic_turbulence.cpp:371-422):
+   max over cells and QGauss(p+1+overint) points of |u| + c (Euler::max_convective_eigenvalue,
+   euler.cpp:1033-1046), the solution interpolated from the GLL support points. */
+double dgo_max_wave_speed(const dgo_ctx *c, const double *u, int overint)
+{
+    const int d = c->dim, ns = c->ns, n = c->n, Np = c->Np;
+    const int nx = c->p + 1 + overint;
+    double xq[64], wq[64];
+    dgo_gauss_legendre(nx, xq, wq);
+    double *B = (double *)malloc(sizeof(double) * nx * n);
+    double val[MAXN], der[MAXN];
+    for (int q = 0; q < nx; ++q) {
+        dgo_lagrange_values(n, c->xs, xq[q], val, der);
+        for (int j = 0; j < n; ++j) B[q * n + j] = val[j];
+    }
+    const int nqv = (d == 3) ? nx * nx * nx : nx * nx;
+    double *sq = (double *)malloc(sizeof(double) * ns * nqv);
+    double *coef = (double *)malloc(sizeof(double) * Np);
+    double vmax = 0.0;
+    for (int e = 0; e < c->ncells; ++e) {
+        for (int st = 0; st < ns; ++st) {
+            for (int i = 0; i < Np; ++i) coef[i] = u[IDX(c, e, st, i)];
+            dgo_tensor_apply(d, B, nx, n, B, nx, n, B, nx, n, coef, sq + st * nqv, 0, 1.0);
+        }
+        for (int q = 0; q < nqv; ++q) {
+            double us[NS_MAX];
+            for (int st = 0; st < ns; ++st) us[st] = sq[st * nqv + q];
+            const double ev = dgo_max_convective_eigenvalue(c, us);
+            if (ev > vmax) vmax = ev;
+        }
+    }
+    free(B); free(sq); free(coef);
+    return vmax;
+}
+
+/* PeriodicTurbulence::get_adaptive_time_step (periodic_turbulence.cpp:359-367) */
+double dgo_adaptive_time_step(const dgo_ctx *c, const double *u, double cfl)
+{
+    const unsigned ndofs_per_state = (unsigned)(dgo_n_dofs(c) / c->ns);
+    const double dx = (c->P.domain_right - c->P.domain_left) / pow(ndofs_per_state, (1.0 / c->dim));
+    return cfl * dx / dgo_max_wave_speed(c, u, 10);
+}
+
 /* PeriodicTurbulence::compute_current_integrated_numerical_entropy
    (periodic_turbulence.cpp:765-868): sum over cells and volume quadrature points of
    w_q det J * (-rho s), s = ln(p rho^-gamma) (euler.cpp:437-447, 511-532), the solution

@@ -144,6 +144,10 @@ void dgo_integrated_quantities(const dgo_ctx *c, const double *u, int overint, d
 double dgo_entropy_rate(dgo_ctx *c, const double *u);
 /* integrated numerical entropy  sum w_q detJ (-rho ln(p rho^-gamma))  (periodic_turbulence.cpp:765-868) */
 double dgo_numerical_entropy(const dgo_ctx *c, const double *u);
+/* max |u| + c over QGauss(p+1+overint) points (periodic_turbulence.cpp:371-422) and the
+   adaptive step cfl * dx / max speed (periodic_turbulence.cpp:359-367) */
+double dgo_max_wave_speed(const dgo_ctx *c, const double *u, int overint);
+double dgo_adaptive_time_step(const dgo_ctx *c, const double *u, double cfl);
 
 /* full flow-solver loop; returns number of steps; ke_hist gets KE after every step (may be NULL) */
 int dgo_run(dgo_ctx *c, int method, double dt_const, double final_time, double *u,

@@ -27,7 +27,7 @@ EXPORTED = [
     "dgx_assemble_residual_device", "dgx_rhs_device", "dgx_apply_inverse_mass_device",
     "dgx_allocate_ode_system", "dgx_step_in_time", "dgx_current_time", "dgx_current_iteration",
     "dgx_set_track_error_estimate", "dgx_constant_time_step", "dgx_integrated_quantities",
-    "dgx_numerical_entropy", "dgx_set_stream",
+    "dgx_numerical_entropy", "dgx_max_wave_speed", "dgx_adaptive_time_step", "dgx_set_stream",
     "dgx_enable_timing", "dgx_kernel_timing", "dgx_set_initial_condition_tgv",
     "dgx_set_initial_condition_vortex", "dgx_synchronize",
     "dgx_batch_create", "dgx_batch_destroy", "dgx_batch_last_error", "dgx_batch_size", "dgx_batch_instance",
@@ -117,6 +117,8 @@ def load():
     lib.dgx_constant_time_step.argtypes = [vp, d]
     lib.dgx_integrated_quantities.argtypes = [vp, i, dp]
     lib.dgx_numerical_entropy.argtypes = [vp, dp]
+    lib.dgx_max_wave_speed.argtypes = [vp, i, dp]
+    lib.dgx_adaptive_time_step.argtypes = [vp, d, dp]
     lib.dgx_set_stream.argtypes = [vp, vp]
     lib.dgx_enable_timing.argtypes = [vp, i]
     lib.dgx_kernel_timing.argtypes = [vp, dp, P(i64)]
@@ -276,6 +278,16 @@ class DG:
         self._chk(load().dgx_numerical_entropy(self.h, _dp(o)), "numerical_entropy")
         return float(o[0])
 
+    def max_wave_speed(self, overint=10):
+        o = np.zeros(1)
+        self._chk(load().dgx_max_wave_speed(self.h, overint, _dp(o)), "max_wave_speed")
+        return float(o[0])
+
+    def adaptive_time_step(self, cfl):
+        o = np.zeros(1)
+        self._chk(load().dgx_adaptive_time_step(self.h, cfl, _dp(o)), "adaptive_time_step")
+        return float(o[0])
+
     def enable_timing(self, on=True):
         self._chk(load().dgx_enable_timing(self.h, int(on)), "enable_timing")
 
@@ -285,13 +297,15 @@ class DG:
         self._chk(load().dgx_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)), "kernel_timing")
         return ms.value, n.value
 
-    def run(self, method, dt, final_time):
-        """FlowSolver::run time loop (flow_solver.cpp:554-610) with a constant dt."""
+    def run(self, method, dt, final_time, adaptive_cfl=None):
+        """FlowSolver::run time loop (flow_solver.cpp:554-610): constant dt, or with adaptive_cfl the
+        step from the device's maximum wave speed before every step (flow_solver.cpp:502-504,
+        603-604); the last step is clipped to the final time."""
         self.allocate_ode_system(method)
         t = self.current_time
         steps = 0
         while t < final_time:
-            step = dt
+            step = self.adaptive_time_step(adaptive_cfl) if adaptive_cfl is not None else dt
             if t + step > final_time:
                 step = final_time - t
             self.step_in_time(step, 1)

@@ -780,6 +780,24 @@ int dgx_numerical_entropy(dgx_ctx *c, double *out)
     return launch_entropy(c, c->u_dev, out);
 }
 
+int dgx_max_wave_speed(dgx_ctx *c, int overint, double *out)
+{
+    if (!c || !out || overint < 0) return DGX_ERR_INVALID;
+    return launch_wave_speed(c, c->u_dev, overint, out);
+}
+
+int dgx_adaptive_time_step(dgx_ctx *c, double cfl, double *dt)
+{
+    if (!c || !dt) return DGX_ERR_INVALID;
+    double vmax = 0.0;
+    int rc = launch_wave_speed(c, c->u_dev, 10, &vmax);
+    if (rc) return rc;
+    const unsigned ndofs_per_state = (unsigned)(c->ndofs / c->ns);
+    const double dx = (c->P.domain_right - c->P.domain_left) / std::pow((double)ndofs_per_state, (1.0 / c->dim));
+    *dt = cfl * dx / vmax;
+    return DGX_OK;
+}
+
 int dgx_synchronize(dgx_ctx *c)
 {
     if (!c) return DGX_ERR_INVALID;

@@ -86,4 +86,5 @@ int launch_line_batch(int dim, int n, int precision, int es, const BatchBlock *b
 int launch_aux(dgx_ctx *c, const double *u);
 int launch_integrated(dgx_ctx *c, const double *u, int overint, double *host_out4);
 int launch_entropy(dgx_ctx *c, const double *u, double *host_out);
+int launch_wave_speed(dgx_ctx *c, const double *u, int overint, double *host_out);
 }

@@ -329,6 +329,91 @@ integrated_kernel(const double *__restrict__ u, const double *__restrict__ ops, 
         for (int f = 0; f < 3; ++f) partial[(long)e * 3 + f] = red[f * IQ_THREADS];
 }
 
+// Maximum local wave speed |u| + c over the QGauss(nx)^dim points of every cell
+// (periodic_turbulence.cpp:371-422, euler.cpp:1033-1046): all states interpolated
+// sum-factorised (x pass in LDS, y and z passes in registers); per-cell maxima.
+template <int D, int N>
+__global__ void __launch_bounds__(IQ_THREADS)
+wave_speed_kernel(const double *__restrict__ u, const double *__restrict__ ops, int nx, double gam,
+                  double *__restrict__ cellmax)
+{
+    constexpr int NS = D + 2, n = N;
+    constexpr int Np = (D == 3) ? n * n * n : n * n;
+    constexpr int nr = (D == 3) ? n * n : n;
+    extern __shared__ double smem[];
+    double *sU = smem;             // [NS][Np]
+    double *sB = sU + NS * Np;     // [nx][n]
+    double *sT = sB + nx * n;      // [NS][nx][nr]
+    double *red = sT + NS * nx * nr;
+    const int e = blockIdx.x;
+    for (int t = threadIdx.x; t < NS * Np; t += blockDim.x) sU[t] = u[(long)e * NS * Np + t];
+    for (int t = threadIdx.x; t < nx * n; t += blockDim.x) sB[t] = ops[t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < NS * nx * nr; t += blockDim.x) {
+        const int r = t % nr, qx = (t / nr) % nx, sv = t / (nr * nx);
+        double acc = 0.0;
+        for (int a = 0; a < n; ++a) acc += sB[qx * n + a] * sU[sv * Np + a + n * r];
+        sT[t] = acc;
+    }
+    __syncthreads();
+    double vmax = 0.0;
+    for (int col = threadIdx.x; col < nx * nx; col += blockDim.x) {
+        const int qx = col % nx, qy = col / nx;
+        constexpr int nz = (D == 3) ? n : 1;
+        double Y[NS][nz];
+        for (int sv = 0; sv < NS; ++sv)
+            for (int c = 0; c < nz; ++c) {
+                double acc = 0.0;
+                for (int b = 0; b < n; ++b) acc += sB[qy * n + b] * sT[(sv * nx + qx) * nr + b + n * c];
+                Y[sv][c] = acc;
+            }
+        const int nqz = (D == 3) ? nx : 1;
+        for (int qz = 0; qz < nqz; ++qz) {
+            double us[NS];
+            for (int sv = 0; sv < NS; ++sv) {
+                if (D == 3) {
+                    double acc = 0.0;
+                    for (int c = 0; c < n; ++c) acc += sB[qz * n + c] * Y[sv][c];
+                    us[sv] = acc;
+                } else {
+                    us[sv] = Y[sv][0];
+                }
+            }
+            const double rho = us[0];
+            double vel2 = 0.0;
+            for (int k = 0; k < D; ++k) {
+                const double v = us[1 + k] / rho;
+                vel2 = vel2 + v * v;
+            }
+            const double p = (gam - 1.0) * (us[NS - 1] - 0.5 * rho * vel2);
+            const double ev = sqrt(vel2) + sqrt(gam * p / rho);
+            vmax = ev > vmax ? ev : vmax;
+        }
+    }
+    red[threadIdx.x] = vmax;
+    __syncthreads();
+    for (int sh = IQ_THREADS / 2; sh > 0; sh >>= 1) {
+        if ((int)threadIdx.x < sh) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + sh]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cellmax[e] = red[0];
+}
+
+__global__ void __launch_bounds__(IQ_THREADS)
+max_reduce_kernel(const double *__restrict__ v, int n, double *__restrict__ out)
+{
+    __shared__ double red[IQ_THREADS];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += IQ_THREADS) m = fmax(m, v[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int sh = IQ_THREADS / 2; sh > 0; sh >>= 1) {
+        if ((int)threadIdx.x < sh) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + sh]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
 // Integrated numerical entropy on collocated GLL nodes (periodic_turbulence.cpp:765-868,
 // euler.cpp:437-447): per cell sum_i W_i det J (-rho_i ln(p_i rho_i^-gamma)); the three
 // partial slots of a cell hold (entropy, 0, 0) so the fixed-order reduction below applies.
@@ -414,7 +499,9 @@ int launch_entropy(dgx_ctx *c, const double *u, double *out)
     return DGX_OK;
 }
 
-int launch_integrated(dgx_ctx *c, const double *u, int overint, double *out4)
+// QGauss(p+1+overint) interpolation operators (values, derivatives, weights) and the per-cell
+// partial buffer shared by the reduction kernels.
+static int iq_operators(dgx_ctx *c, int overint)
 {
     const int n = c->n, nx = c->p + 1 + overint;
     if (nx > 32) {
@@ -447,6 +534,46 @@ int launch_integrated(dgx_ctx *c, const double *u, int overint, double *out4)
         }
         c->iq_overint = overint;
     }
+    return DGX_OK;
+}
+
+int launch_wave_speed(dgx_ctx *c, const double *u, int overint, double *out)
+{
+    const int n = c->n, nx = c->p + 1 + overint;
+    int rc = iq_operators(c, overint);
+    if (rc) return rc;
+    const int nr = (c->dim == 3) ? n * n : n;
+    const size_t shmem = sizeof(double) * ((size_t)c->ns * c->Np + nx * n + (size_t)c->ns * nx * nr + IQ_THREADS);
+    switch (c->dim * 16 + n) {
+#define WS_CASE(DD, NN)                                                                                         \
+    case DD * 16 + NN:                                                                                           \
+        hipLaunchKernelGGL((wave_speed_kernel<DD, NN>), dim3(c->ncells), dim3(IQ_THREADS), shmem, c->stream, u,  \
+                           c->iq_ops, nx, c->P.gamma_gas, c->iq_partial);                                        \
+        break;
+        WS_CASE(3, 2) WS_CASE(3, 3) WS_CASE(3, 4) WS_CASE(3, 5) WS_CASE(3, 6) WS_CASE(3, 7)
+        WS_CASE(2, 2) WS_CASE(2, 3) WS_CASE(2, 4) WS_CASE(2, 5) WS_CASE(2, 6) WS_CASE(2, 7)
+#undef WS_CASE
+    default: c->err = "unsupported polynomial degree"; return DGX_ERR_UNSUPPORTED;
+    }
+    rc = check_launch(c, "wave_speed_kernel");
+    if (rc) return rc;
+    double *m = c->iq_partial + 3 * (size_t)c->ncells;
+    hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(IQ_THREADS), 0, c->stream, c->iq_partial, c->ncells, m);
+    rc = check_launch(c, "max_reduce_kernel");
+    if (rc) return rc;
+    if (hipMemcpyAsync(out, m, sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        c->err = "wave speed copy failed";
+        return DGX_ERR_HIP;
+    }
+    return DGX_OK;
+}
+
+int launch_integrated(dgx_ctx *c, const double *u, int overint, double *out4)
+{
+    const int n = c->n, nx = c->p + 1 + overint;
+    int rc0 = iq_operators(c, overint);
+    if (rc0) return rc0;
     const int nr = (c->dim == 3) ? n * n : n;
     const size_t shmem = sizeof(double) * ((size_t)(c->dim + 1) * c->Np + 2 * nx * n + nx +
                                            (size_t)(c->dim + 1) * 2 * nx * nr + 3 * IQ_THREADS);

@@ -206,6 +206,38 @@ def test_numerical_entropy_matches_oracle(dim):
     assert abs(d.numerical_entropy() - ref) <= 1e-13 * abs(ref)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_max_wave_speed_and_adaptive_step_match_oracle(dim):
+    kw = dict(dim=3, poly_degree=3, n_cells_1d=4, mach_inf=0.2) if dim == 3 else dict(
+        dim=2, poly_degree=5, n_cells_1d=4, domain_left=-5.0, domain_right=5.0, mach_inf=1.195228609334)
+    o, d = make_pair(**kw)
+    u = perturbed(o.ic_tgv() if dim == 3 else o.ic_vortex(strength=3.0, u_inf=0.5, v_inf=0.4), o, seed=5)
+    d.set_solution(u)
+    for overint in (0, 10):
+        ref = o.max_wave_speed(u, overint)
+        assert abs(d.max_wave_speed(overint) - ref) <= 1e-14 * ref
+    ref_dt = o.adaptive_time_step(u, 0.2)
+    assert abs(d.adaptive_time_step(0.2) - ref_dt) <= 1e-14 * ref_dt
+
+
+def test_adaptive_run_matches_oracle_steps():
+    kw = dict(dim=3, poly_degree=2, n_cells_1d=4, use_split_form=1, two_point_flux=O.CH, conv_flux=O.TWO_POINT_ROE)
+    o, d = make_pair(**kw)
+    u = o.ic_tgv()
+    d.set_solution(u)
+    steps = d.run("ssprk3", None, 0.02, adaptive_cfl=0.1)
+    t, n = 0.0, 0
+    while t < 0.02:
+        dt = o.adaptive_time_step(u, 0.1)
+        if t + dt > 0.02:
+            dt = 0.02 - t
+        o.step("ssprk3", dt, u)
+        t += dt
+        n += 1
+    assert steps == n
+    assert rel(d.solution(), u) <= 1e-11
+
+
 def test_reference_golden_nsfr_gll_on_device():
     """The reference's own golden value, reproduced by the device path end to end
     (viscous_taylor_green_vortex_energy_check_nsfr_gll_flux_nodes_quick.prm:59-60,

@@ -79,3 +79,27 @@ def test_gl_flux_nodes_interpolate_to_gauss_points():
     assert abs(o.numerical_entropy(u) - ref) <= 1e-12 * abs(ref)
     # and it differs from the nodal (GLL) rule, so the interpolation is exercised
     assert abs(entropy_numpy(o, u, gl=False) - ref) > 1e-9 * abs(ref)
+
+
+def test_max_wave_speed_uniform_state_and_independent_interpolation():
+    o = O.Oracle(dim=3, poly_degree=2, n_cells_1d=2, mach_inf=0.2)
+    gam = 1.4
+    rho, vel, p = 1.1, np.array([0.3, -0.4, 0.1]), 9.0
+    u = np.zeros((o.ncells, o.ns, o.Np))
+    u[:, 0], u[:, 1:4], u[:, 4] = rho, (rho * vel)[None, :, None], p / (gam - 1) + 0.5 * rho * vel @ vel
+    exact = np.linalg.norm(vel) + np.sqrt(gam * p / rho)
+    assert abs(o.max_wave_speed(u.ravel()) - exact) <= 1e-14 * exact
+    # non-uniform: numpy interpolation of the GLL nodal values to QGauss(p+11) points
+    w = o.ic_tgv()
+    n, nx = o.n, o.n + 10
+    xs, _ = gll(n)
+    xq, _ = L.leggauss(nx)
+    A = lagrange_matrix(xs, 0.5 * (xq + 1))
+    U = w.reshape(o.ncells, o.ns, n, n, n)
+    Uq = np.einsum("ia,jb,kc,esabc->esijk", A, A, A, U)
+    v = Uq[:, 1:4] / Uq[:, 0:1]
+    pq = (gam - 1) * (Uq[:, 4] - 0.5 * Uq[:, 0] * (v ** 2).sum(1))
+    ref = float((np.sqrt((v ** 2).sum(1)) + np.sqrt(gam * pq / Uq[:, 0])).max())
+    assert abs(o.max_wave_speed(w) - ref) <= 1e-13 * ref
+    h = 2 * np.pi / (o.ndofs / o.ns) ** (1 / 3)
+    assert abs(o.adaptive_time_step(w, 0.3) - 0.3 * h / ref) <= 1e-13 * 0.3 * h / ref
