@@ -133,6 +133,9 @@ int dgx_allocate_ode_system(dgx_ctx *ctx, int method);
 int dgx_step_in_time(dgx_ctx *ctx, double dt, int n_steps);
 double dgx_current_time(const dgx_ctx *ctx);
 int64_t dgx_current_iteration(const dgx_ctx *ctx);
+/* Restores the solver clock when restarting from saved data (FlowSolver::run with
+   restart_computation_from_file, flow_solver.cpp:463-481: time and iteration continue). */
+int dgx_set_time(dgx_ctx *ctx, double time, int64_t iteration);
 /* 3S*+ methods: also track the embedded register (extra FSAL residual per step,
    low_storage_runge_kutta_ode_solver.cpp:63-69); off by default because it does
    not change the solution at a fixed time step. */

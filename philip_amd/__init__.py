@@ -25,7 +25,7 @@ EXPORTED = [
     "dgx_set_solution", "dgx_get_solution", "dgx_get_right_hand_side", "dgx_get_auxiliary_solution",
     "dgx_get_neighbors", "dgx_assemble_residual", "dgx_apply_inverse_mass_to_rhs",
     "dgx_assemble_residual_device", "dgx_rhs_device", "dgx_apply_inverse_mass_device",
-    "dgx_allocate_ode_system", "dgx_step_in_time", "dgx_current_time", "dgx_current_iteration",
+    "dgx_allocate_ode_system", "dgx_step_in_time", "dgx_current_time", "dgx_current_iteration", "dgx_set_time",
     "dgx_set_track_error_estimate", "dgx_constant_time_step", "dgx_integrated_quantities",
     "dgx_numerical_entropy", "dgx_max_wave_speed", "dgx_adaptive_time_step", "dgx_set_stream",
     "dgx_enable_timing", "dgx_kernel_timing", "dgx_set_initial_condition_tgv",
@@ -112,6 +112,7 @@ def load():
     lib.dgx_current_time.argtypes = [vp]
     lib.dgx_current_iteration.restype = i64
     lib.dgx_current_iteration.argtypes = [vp]
+    lib.dgx_set_time.argtypes = [vp, d, i64]
     lib.dgx_set_track_error_estimate.argtypes = [vp, i]
     lib.dgx_constant_time_step.restype = d
     lib.dgx_constant_time_step.argtypes = [vp, d]
@@ -297,11 +298,31 @@ class DG:
         self._chk(load().dgx_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)), "kernel_timing")
         return ms.value, n.value
 
-    def run(self, method, dt, final_time, adaptive_cfl=None):
+    def set_time(self, time, iteration):
+        self._chk(load().dgx_set_time(self.h, time, iteration), "set_time")
+
+    def unsteady_row(self, numerical_entropy=False):
+        """One row of the unsteady data table (periodic_turbulence.cpp:923-975): the quantities this
+        device path computes, in the reference's column names."""
+        q = self.integrated()
+        row = {"time": self.current_time}
+        if numerical_entropy:
+            row["numerical_entropy"] = self.numerical_entropy()
+        row["kinetic_energy"] = q["kinetic_energy"]
+        row["enstrophy"] = q["enstrophy"]
+        if self.p.pde == NAVIER_STOKES:
+            row["eps_vorticity"] = q["vorticity_dissipation"]
+        row["incompressible_kinetic_energy"] = q["incompressible_kinetic_energy"]
+        return row
+
+    def run(self, method, dt, final_time, adaptive_cfl=None, table=None, allocate=True):
         """FlowSolver::run time loop (flow_solver.cpp:554-610): constant dt, or with adaptive_cfl the
         step from the device's maximum wave speed before every step (flow_solver.cpp:502-504,
-        603-604); the last step is clipped to the final time."""
-        self.allocate_ode_system(method)
+        603-604); the last step is clipped to the final time.  table: list that receives one
+        unsteady-data row per step.  allocate=False continues with the registers already allocated
+        (a restarted run)."""
+        if allocate:
+            self.allocate_ode_system(method)
         t = self.current_time
         steps = 0
         while t < final_time:
@@ -311,7 +332,51 @@ class DG:
             self.step_in_time(step, 1)
             t = self.current_time
             steps += 1
+            if table is not None:
+                table.append(self.unsteady_row())
         return steps
+
+    def save_restart(self, path, table=None):
+        """Restart data (flow_solver.cpp:365-426 writes the solution, the time and the data table):
+        solution in the reference DoF order, time, iteration, the parameters, the table so far."""
+        import json
+        params = {f: getattr(self.p, f) for f, _ in Params._fields_}
+        cols = sorted({k for r in (table or []) for k in r})
+        rows = np.array([[r.get(c, np.nan) for c in cols] for r in (table or [])], dtype=np.float64)
+        np.savez(path, solution=self.solution(), time=np.float64(self.current_time),
+                 iteration=np.int64(self.current_iteration), params=np.array(json.dumps(params)),
+                 table_columns=np.array(cols, dtype=str), table_rows=rows.reshape(len(table or []), len(cols)))
+
+    def load_restart(self, path):
+        """Restart from save_restart data (flow_solver.cpp:58-100): checks the discretisation matches,
+        restores the solution and the clock; returns the saved data-table rows."""
+        import json
+        with np.load(path, allow_pickle=False) as z:
+            saved = json.loads(str(z["params"]))
+            for k in ("dim", "poly_degree", "n_cells_1d", "domain_left", "domain_right"):
+                if saved[k] != getattr(self.p, k):
+                    raise DeviceError(f"restart data is for {k}={saved[k]}, this solver has {getattr(self.p, k)}")
+            self.set_solution(z["solution"])
+            self.set_time(float(z["time"]), int(z["iteration"]))
+            cols = [str(c) for c in z["table_columns"]]
+            return [dict(zip(cols, map(float, r))) for r in z["table_rows"]]
+
+
+def write_unsteady_table(path, rows):
+    """Unsteady data table as text (periodic_turbulence.cpp:971-975 writes a dealii::TableHandler):
+    a header of column names, then one row per step, values in scientific notation with 16
+    digits (flow_solver_case_base.cpp:229-237)."""
+    cols = list(rows[0]) if rows else []
+    with open(path, "w") as f:
+        f.write(" ".join(cols) + "\n")
+        for r in rows:
+            f.write(" ".join(f"{r[c]:.16e}" for c in cols) + "\n")
+
+
+def read_unsteady_table(path):
+    with open(path) as f:
+        cols = f.readline().split()
+        return [dict(zip(cols, map(float, line.split()))) for line in f if line.strip()]
 
 
 class Batch:

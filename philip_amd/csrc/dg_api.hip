@@ -701,6 +701,14 @@ double dgx_constant_time_step(const dgx_ctx *c, double cfl)
     return cfl * dx;
 }
 
+int dgx_set_time(dgx_ctx *c, double time, int64_t iteration)
+{
+    if (!c || iteration < 0) return DGX_ERR_INVALID;
+    c->current_time = time;
+    c->iteration = iteration;
+    return DGX_OK;
+}
+
 // Initial conditions interpolated at the GLL support points of each cell
 // (set_initial_condition.cpp:45-51): TGV (initial_condition_function.cpp:261-336),
 // isentropic vortex with parameterised strength/advection (:643-690).
