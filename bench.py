@@ -190,6 +190,30 @@ def serving(n_req=48, rate=40.0, seed=77, precision="mixed", time_scale=0.1):
                 workload=f"70% 2D vortex / 30% 3D TGV, final time U[0.5,2]x{time_scale}, RK4, HBM budget 64 GiB")
 
 
+def navier_stokes_tgv(precision="mixed", steps=3):
+    """Navier-Stokes TGV, Re 1600, Pr 0.71, M 0.1, p=5 on 16^3 hex (4,423,680 DOF), BR2, Ranocha
+    split form + EC/Roe surface flux, constant viscosity, classical RK4 (auxiliary-gradient kernel +
+    stage kernel per stage).  Returns DOF-updates/s over the timed steps."""
+    import philip_amd as P
+    d = P.DG(dim=3, poly_degree=5, n_cells_1d=16, pde=P.NAVIER_STOKES, diss_flux=P.BR2, use_constant_viscosity=1,
+             constant_viscosity=1.0, reynolds_inf=1600.0, prandtl=0.71, mach_inf=0.1, use_split_form=1,
+             two_point_flux=P.RA, conv_flux=P.TWO_POINT_ROE,
+             precision=P.MIXED if precision == "mixed" else P.FP64)
+    d.set_initial_condition_tgv()
+    dt = d.constant_time_step(0.01)
+    d.allocate_ode_system("rk4")
+    d.step_in_time(dt, 1)
+    d.synchronize()
+    t0 = time.perf_counter()
+    d.step_in_time(dt, steps)
+    d.synchronize()
+    t = time.perf_counter() - t0
+    n = d.ndofs
+    d.close()
+    return dict(value=n * 4 * steps / t, unit="DOF-updates/s", dof=n, ms_per_stage=1e3 * t / (4 * steps),
+                precision=precision, workload="NS TGV Re1600 p=5 16^3, BR2, Ranocha split + EC/Roe, RK4")
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of known size (read nbytes + write nbytes per launch): the
     reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to HBM bytes."""
@@ -261,6 +285,7 @@ def main():
                              "roofline_frac": bytes_per_launch / (r64["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
     if not args.no_batch:
         line["batched"] = batched_vortex()
+        line["navier_stokes"] = [navier_stokes_tgv("mixed"), navier_stokes_tgv("fp64")]
     if not args.no_serving:
         line["serving"] = serving()
     if not args.no_cpu_baseline:
