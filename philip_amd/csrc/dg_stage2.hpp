@@ -17,6 +17,8 @@
 // T is the flux arithmetic type: double (FP64 path) or float (mixed path,
 // FP32 fluxes and contractions, FP64 state and RK registers).
 #pragma once
+#include <type_traits>
+
 #include "dg_kernels.hpp"
 
 namespace dgx {
@@ -109,6 +111,7 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
         for (int s = 0; s < NS; ++s) sU[le][s][i] = ut[s];
         if (SPLIT) {
             const T rho = ut[0];
+#pragma unroll
             for (int k = 0; k < D; ++k) sP[le][k][i] = ut[1 + k] / rho;
             sP[le][D][i] = pressure<T, D>(ut, gamm1);
         }
@@ -117,6 +120,7 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
             if (NSV)
                 for (int s = 0; s < NS; ++s)
                     for (int d = 0; d < D; ++d) g[s * D + d] = T(a.aux[d][((long)e * NS + s) * NP + i]);
+#pragma unroll
             for (int k = 0; k < D; ++k) {
                 T f[NS];
                 for (int s = 0; s < NS; ++s) f[s] = T(0);
@@ -133,78 +137,87 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                 for (int s = 0; s < NS; ++s) sF[le][k][s][i] = f[s];
             }
         }
+#pragma unroll
         for (int k = 0; k < D; ++k)
             for (int s = 0; s < NS; ++s) sFace[le][k][s][i] = T(0);
     }
     __syncthreads();
 
     // ---------------- phase C: face-parallel numerical fluxes -----------
+    // Items ordered direction-major (direction, element, side, face node) so that
+    // the items of a wave mostly share one direction, a template parameter of the
+    // item code (no register array indexed per lane).
     constexpr int FITEMS = 2 * D * NL;
     for (int t = tid; t < EPB * FITEMS; t += G::BLOCK) {
-        const int fle = t / FITEMS, r = t % FITEMS;
+        const int k = t / (EPB * 2 * NL), r0 = t % (EPB * 2 * NL);
+        const int fle = r0 / (2 * NL), side = (r0 / NL) % 2, fn = r0 % NL;
         const int fe = blk0 + fle;
         if (fe >= a.ncells) continue;
-        const int f = r / NL, fn = r % NL;
-        const int k = f / 2, side = f % 2;
-        const int fi = line_node<D, N>(k, fn, side ? N - 1 : 0);
-        const int nb = a.nbr[fe * 2 * D + f];
-        const int fj = line_node<D, N>(k, fn, side ? 0 : N - 1);
-        const T sgn = side ? T(1) : T(-1);
-        T us[NS], un[NS], unt[NS];
-        double und[NS];
-        for (int s = 0; s < NS; ++s) {
-            us[s] = sU[fle][s][fi];
-            und[s] = a.u[((long)nb * NS + s) * NP + fj];
-            un[s] = T(und[s]);
-        }
-        if (SPLIT && RT) roundtrip<T, D>(un, pc, unt);
-        else
-            for (int s = 0; s < NS; ++s) unt[s] = un[s];
-        // reference orientation: the lower cell index is the interior side; selected by value so
-        // the flux evaluation itself is not under a divergent branch
-        const bool own = fe < nb;
-        T ua[NS], ub[NS];
-        for (int s = 0; s < NS; ++s) {
-            ua[s] = own ? us[s] : unt[s];
-            ub[s] = own ? unt[s] : us[s];
-        }
-        const T sg = own ? sgn : -sgn;
-        const Prim<T, D> pa = to_prim<T, D>(ua, gamm1);
-        const Prim<T, D> pb = to_prim<T, D>(ub, gamm1);
-        T fs[NS];
-        numerical_flux<T, D>(ua, pa, ub, pb, k, sg, pc, fs);
-        const T fsgn = own ? T(1) : T(-1);
-        T ft[NS];
-        for (int s = 0; s < NS; ++s) ft[s] = -(fs[s] * fsgn);
-        if (!SPLIT)
-            for (int s = 0; s < NS; ++s) ft[s] += sgn * sF[fle][k][s][fi];
-        if (NSV) {
-            T ui[NS], gi[NS * D], gn[NS * D], jump[NS * D];
+        auto item = [&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            const int fi = line_node<D, N>(K, fn, side ? N - 1 : 0);
+            const int nb = a.nbr[fe * 2 * D + 2 * K + side];
+            const int fj = line_node<D, N>(K, fn, side ? 0 : N - 1);
+            const T sgn = side ? T(1) : T(-1);
+            T us[NS], un[NS], unt[NS];
+            double und[NS];
             for (int s = 0; s < NS; ++s) {
-                ui[s] = T(a.u[((long)fe * NS + s) * NP + fi]);
-                for (int d = 0; d < D; ++d) {
-                    gi[s * D + d] = T(a.aux[d][((long)fe * NS + s) * NP + fi]);
-                    gn[s * D + d] = T(a.aux[d][((long)nb * NS + s) * NP + fj]);
-                    jump[s * D + d] = (d == k) ? (ui[s] - un[s]) * sgn : (ui[s] - un[s]) * T(0);
+                us[s] = sU[fle][s][fi];
+                und[s] = a.u[((long)nb * NS + s) * NP + fj];
+                un[s] = T(und[s]);
+            }
+            if (SPLIT && RT) roundtrip<T, D>(un, pc, unt);
+            else
+                for (int s = 0; s < NS; ++s) unt[s] = un[s];
+            // reference orientation: the lower cell index is the interior side; selected by value so
+            // the flux evaluation itself is not under a divergent branch
+            const bool own = fe < nb;
+            T ua[NS], ub[NS];
+            for (int s = 0; s < NS; ++s) {
+                ua[s] = own ? us[s] : unt[s];
+                ub[s] = own ? unt[s] : us[s];
+            }
+            const T sg = own ? sgn : -sgn;
+            const Prim<T, D> pa = to_prim<T, D>(ua, gamm1);
+            const Prim<T, D> pb = to_prim<T, D>(ub, gamm1);
+            T fs[NS];
+            numerical_flux<T, D>(ua, pa, ub, pb, K, sg, pc, fs);
+            const T fsgn = own ? T(1) : T(-1);
+            T ft[NS];
+            for (int s = 0; s < NS; ++s) ft[s] = -(fs[s] * fsgn);
+            if (!SPLIT)
+                for (int s = 0; s < NS; ++s) ft[s] += sgn * sF[fle][K][s][fi];
+            if (NSV) {
+                T ui[NS], gi[NS * D], gn[NS * D], jump[NS * D];
+                for (int s = 0; s < NS; ++s) {
+                    ui[s] = T(a.u[((long)fe * NS + s) * NP + fi]);
+                    for (int d = 0; d < D; ++d) {
+                        gi[s * D + d] = T(a.aux[d][((long)fe * NS + s) * NP + fi]);
+                        gn[s * D + d] = T(a.aux[d][((long)nb * NS + s) * NP + fj]);
+                        jump[s * D + d] = (d == K) ? (ui[s] - un[s]) * sgn : (ui[s] - un[s]) * T(0);
+                    }
                 }
+                T fa[NS], fb[NS];
+                viscous_flux_dir<T, D>(ui, gi, K, pc, fa);
+                viscous_flux_dir<T, D>(un, gn, K, pc, fb);
+                T fvs[NS];
+                for (int s = 0; s < NS; ++s) fvs[s] = T(0.5) * (fa[s] * sgn + fb[s] * sgn);
+                if (pc.diss_flux == 0) {
+                    T ja[NS], jb[NS];
+                    viscous_flux_dir<T, D>(ui, jump, K, pc, ja);
+                    viscous_flux_dir<T, D>(un, jump, K, pc, jb);
+                    const T pen = T(a.penalty);
+                    for (int s = 0; s < NS; ++s) fvs[s] = fvs[s] - pen * (T(0.5) * (ja[s] * sgn + jb[s] * sgn));
+                }
+                for (int s = 0; s < NS; ++s) ft[s] -= fvs[s];
+                if (SPLIT)
+                    for (int s = 0; s < NS; ++s) ft[s] += sgn * fa[s];
             }
-            T fa[NS], fb[NS];
-            viscous_flux_dir<T, D>(ui, gi, k, pc, fa);
-            viscous_flux_dir<T, D>(un, gn, k, pc, fb);
-            T fvs[NS];
-            for (int s = 0; s < NS; ++s) fvs[s] = T(0.5) * (fa[s] * sgn + fb[s] * sgn);
-            if (pc.diss_flux == 0) {
-                T ja[NS], jb[NS];
-                viscous_flux_dir<T, D>(ui, jump, k, pc, ja);
-                viscous_flux_dir<T, D>(un, jump, k, pc, jb);
-                const T pen = T(a.penalty);
-                for (int s = 0; s < NS; ++s) fvs[s] = fvs[s] - pen * (T(0.5) * (ja[s] * sgn + jb[s] * sgn));
-            }
-            for (int s = 0; s < NS; ++s) ft[s] -= fvs[s];
-            if (SPLIT)
-                for (int s = 0; s < NS; ++s) ft[s] += sgn * fa[s];
-        }
-        for (int s = 0; s < NS; ++s) sFace[fle][k][s][fi] = ft[s];
+            for (int s = 0; s < NS; ++s) sFace[fle][K][s][fi] = ft[s];
+        };
+        if (k == 0) item(std::integral_constant<int, 0>{});
+        else if (k == 1) item(std::integral_constant<int, 1>{});
+        else if (D == 3) item(std::integral_constant<int, (D == 3 ? 2 : 1)>{});
     }
 
     // ---------------- phases B/D: volume ---------------------------------
@@ -213,6 +226,7 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
     for (int s = 0; s < NS; ++s) acc[s] = T(0);
     if (active) node_coords<D, N>(i, c);
     if (SPLIT) {
+#pragma unroll
         for (int k = 0; k < D; ++k) {
             __syncthreads();  // previous direction's sPF consumed; phase C done
             constexpr int PITEMS = NL * NPAIR;
@@ -258,6 +272,7 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
     }
     if (!active) return;
     if (!SPLIT || NSV) {
+#pragma unroll
         for (int k = 0; k < D; ++k)
             for (int l = 0; l < N; ++l) {
                 const int j = node_with<D, N>(i, k, l);
@@ -266,6 +281,7 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
             }
     }
     const T inv_wb = T(ops.inv_wb);
+#pragma unroll
     for (int k = 0; k < D; ++k)
         for (int s = 0; s < NS; ++s) acc[s] += inv_wb * sFace[le][k][s][i];
 
