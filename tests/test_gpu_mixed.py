@@ -81,3 +81,49 @@ def test_fp64_ke_history_config2_scheme():
         worst = max(worst, abs(d.integrated()["kinetic_energy"] - ke_o) / ke_o)
     log(f"FP64 KE history worst rel diff over 10 LSRK3 steps: {worst:.3e}")
     assert worst <= 1e-12
+
+
+def test_mixed_entropy_drift_config2_scheme():
+    """Entropy-conserving split form: the numerical-entropy change over 10 LSRK3 steps in the mixed
+    path against the FP64 oracle's (the drift relative to |S_0|; bar 1e-6 of |S_0|)."""
+    kw = dict(n_cells_1d=8, **CFG2)
+    o = O.Oracle(**kw)
+    d = P.DG(precision=P.MIXED, **kw)
+    u = o.ic_tgv()
+    d.set_solution(u)
+    s0 = o.numerical_entropy(u)
+    assert abs(d.numerical_entropy() - s0) <= 1e-13 * abs(s0)
+    dt = o.constant_time_step(0.01)
+    d.allocate_ode_system("rk3_2_5f_3sstarplus")
+    worst = 0.0
+    for _ in range(10):
+        o.step("rk3_2_5f_3sstarplus", dt, u)
+        d.step_in_time(dt, 1)
+        worst = max(worst, abs(d.numerical_entropy() - o.numerical_entropy(u)) / abs(s0))
+    log(f"mixed numerical-entropy drift vs FP64 oracle over 10 LSRK3 steps: {worst:.3e} of |S0|")
+    assert worst <= 1e-6
+
+
+def test_mixed_navier_stokes_enstrophy_and_dissipation():
+    """Navier-Stokes TGV (Re 1600, BR2, Ranocha split form + EC/Roe, RK4): enstrophy and the
+    vorticity-based dissipation rate of the mixed path against the FP64 oracle over 5 steps
+    (bar 1e-5 relative; measured values printed)."""
+    kw = dict(dim=3, poly_degree=3, n_cells_1d=4, pde=O.NAVIER_STOKES, diss_flux=O.BR2, use_constant_viscosity=1,
+              constant_viscosity=1.0, reynolds_inf=1600.0, prandtl=0.71, mach_inf=0.1, use_split_form=1,
+              two_point_flux=O.RA, conv_flux=O.TWO_POINT_ROE)
+    o = O.Oracle(**kw)
+    d = P.DG(precision=P.MIXED, **kw)
+    u = o.ic_tgv()
+    d.set_solution(u)
+    dt = o.constant_time_step(0.01)
+    d.allocate_ode_system("rk4")
+    worst_e, worst_eps = 0.0, 0.0
+    for _ in range(5):
+        o.step("rk4", dt, u)
+        d.step_in_time(dt, 1)
+        qo, qd = o.integrated(u), d.integrated()
+        worst_e = max(worst_e, abs(qd["enstrophy"] - qo["enstrophy"]) / qo["enstrophy"])
+        worst_eps = max(worst_eps, abs(qd["vorticity_dissipation"] - qo["vorticity_dissipation"])
+                        / qo["vorticity_dissipation"])
+    log(f"mixed NS enstrophy rel err {worst_e:.3e}, dissipation rate rel err {worst_eps:.3e} over 5 RK4 steps")
+    assert worst_e <= 1e-5 and worst_eps <= 1e-5
