@@ -289,15 +289,16 @@ integrated_kernel(const double *__restrict__ u, const double *__restrict__ ops, 
                 }
             }
             const double rho = val[0];
+            const double irho = 1.0 / rho;  // one division per point, reused
             double vel[D], vel2 = 0.0;
             for (int k = 0; k < D; ++k) {
-                vel[k] = val[1 + k] / rho;
+                vel[k] = val[1 + k] * irho;
                 vel2 = vel2 + vel[k] * vel[k];
             }
             double vg[D][D];
             for (int a2 = 0; a2 < D; ++a2)
                 for (int b2 = 0; b2 < D; ++b2)
-                    vg[a2][b2] = (gr[1 + a2][b2] * inv_h - vel[a2] * (gr[0][b2] * inv_h)) / rho;
+                    vg[a2][b2] = (gr[1 + a2][b2] * inv_h - vel[a2] * (gr[0][b2] * inv_h)) * irho;
             double om2;
             if (D == 2) {
                 const double oz = vg[1][0] - vg[0][1];
