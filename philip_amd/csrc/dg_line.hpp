@@ -278,8 +278,8 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     if (pf)
         for (int s = 0; s < NS; ++s) {
             const long idx = ((long)e * NS + s) * NP + i;
-            pf2[s] = a.s2[idx];
-            pf3[s] = a.s3[idx];
+            pf2[s] = ld_stream(a.s2 + idx);  // streamed once: keep L2 for neighbour traces
+            pf3[s] = ld_stream(a.s3 + idx);
         }
     // neighbour face traces (one item per face node), loaded together with the own state
 #pragma unroll
