@@ -129,7 +129,10 @@ Ops1D build_ops(int n)
         for (int b = 0; b < n; ++b) o.Sk[a][b] = (w[a] * o.D[a][b] - w[b] * o.D[b][a]) / w[a];
     o.inv_wb = 1.0 / w[0];
     for (int a = 0; a < MAXN1D; ++a)
-        for (int b = 0; b < MAXN1D; ++b) o.Skf[a][b] = (float)o.Sk[a][b];
+        for (int b = 0; b < MAXN1D; ++b) {
+            o.Skf[a][b] = (float)o.Sk[a][b];
+            o.Df[a][b] = (float)o.D[a][b];
+        }
     o.inv_wbf = (float)o.inv_wb;
     return o;
 }

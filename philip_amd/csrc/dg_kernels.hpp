@@ -27,6 +27,7 @@ struct Ops1D {
     double w[MAXN1D];           // GLL weights on [0,1]
     double inv_wb;              // 1 / w_0 (boundary weight)
     float Skf[MAXN1D][MAXN1D];  // Sk rounded to float (mixed path)
+    float Df[MAXN1D][MAXN1D];   // D rounded to float (mixed path)
     float inv_wbf;              // inv_wb rounded to float
 };
 

@@ -33,8 +33,18 @@ static void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
     using LG = LineGeo<D, N, T>;
     const dim3 lgrid((c->ncells + LG::EPB - 1) / LG::EPB), lblock(LG::BLOCK);
     if constexpr (D == 3) {
-        if (split && nsv)
-            hipLaunchKernelGGL((stage2_kernel<D, N, true, true, RT, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
+        if (split && nsv) {
+            // mixed: line kernel with the viscous terms; FP64: stage2_kernel (the FP64 line
+            // variant needs ~85 KB of LDS per block at p=5 and ran slower at one block per CU)
+            if constexpr (std::is_same<T, float>::value) {
+                using LV = LineGeoK<D, N, T, true>;
+                const dim3 vgrid((c->ncells + LV::EPB - 1) / LV::EPB), vblock(LV::BLOCK);
+                hipLaunchKernelGGL((line_kernel<D, N, RT, T, true>), vgrid, vblock, 0, c->stream, a, c->ops, c->pc);
+            } else {
+                hipLaunchKernelGGL((stage2_kernel<D, N, true, true, RT, T>), grid, block, 0, c->stream, a, c->ops,
+                                   c->pc);
+            }
+        }
         else if (!split && nsv)
             hipLaunchKernelGGL((stage2_kernel<D, N, false, true, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
         else if (split)

@@ -40,7 +40,7 @@ constexpr bool LINE_PREFETCH_RK_F = true;
 constexpr bool LINE_PREFETCH_RK_D = false;
 // ES: elements per block relative to the 256-node default (ES > 0: ES times,
 // ES < 0: 1/|ES|); per-element strides do not depend on it.
-template <int D, int N, typename T, int ES = 1> struct LineGeo {
+template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
     using G = Geo<D, N>;
     static constexpr int NP = G::NP, NS = D + 2, NL = G::NL;
     static constexpr int EPB = ES > 0 ? G::EPB * ES : (G::EPB / -ES > 0 ? G::EPB / -ES : 1);
@@ -52,14 +52,40 @@ template <int D, int N, typename T, int ES = 1> struct LineGeo {
     // neighbouring elements fall in different LDS banks
     static constexpr int pad1(int x) { return x + (33 - x % 32) % 32; }
     static constexpr int ESQ = pad1(NQ * NP);
-    static constexpr int ESV = pad1(D * NS * NP);
+    // Navier-Stokes lines are shared by LS items (the pairs and the viscous columns split evenly)
+    static constexpr int LS = V ? (N >= 7 ? 3 : (N >= 5 ? 2 : 1)) : 1;
+    static constexpr int ESV = pad1(LS * D * NS * NP);  // line partials [direction][part][state][node]
     static constexpr int NFN = 2 * D * NL;           // face nodes per element
     static constexpr int ESN = pad1(NS * NFN);       // neighbour traces [state][face][face node]
-    static constexpr int ITEMS = EPB * D * NL;
+    static constexpr int ITEMS = EPB * D * NL * LS;
     static constexpr int FITEMS = EPB * NFN;
     static constexpr int ESF = pad1(2 * D * NS * NL);   // face terms [face][state][face node]
-    static constexpr int SBUF = (ESQ + ESN > ESV + ESF) ? EPB * (ESQ + ESN) : EPB * (ESV + ESF);
+    // Navier-Stokes (V, mixed path only): own viscous flux in every direction
+    // [direction][state][node] and the neighbours' viscous flux along the face direction at
+    // their face nodes [state][face][face node]
+    static_assert(!V || FAST, "the Navier-Stokes line kernel is the mixed-precision path");
+    static constexpr int ESFV = V ? pad1(D * NS * NP) : 0;
+    static constexpr int ESNV = V ? ESN : 0;
+    static constexpr int EIN = ESQ + ESN + ESFV + ESNV;
+    static constexpr int SBUF = (EIN > ESV + ESF) ? EPB * EIN : EPB * (ESV + ESF);
 };
+
+// Elements-per-block variant of the single-solve line kernel: 1 for Euler; for
+// Navier-Stokes (V) the 256-node default block halved until its LDS fits 64 KB
+// (down to one element).
+template <int D, int N, typename T, bool V> constexpr int line_es()
+{
+    if constexpr (V) {
+        using L1 = LineGeo<D, N, T, 1, true>;
+        constexpr int W = (L1::EIN > L1::ESV + L1::ESF ? L1::EIN : L1::ESV + L1::ESF) * (int)sizeof(T);
+        int div = 1;
+        while (Geo<D, N>::EPB / div > 1 && (Geo<D, N>::EPB / div) * W > 65536) div *= 2;
+        return div == 1 ? 1 : -div;
+    } else {
+        return 1;
+    }
+}
+template <int D, int N, typename T, bool V> using LineGeoK = LineGeo<D, N, T, line_es<D, N, T, V>(), V>;
 
 // ---- mixed path: node cache in float ------------------------------------
 
@@ -104,7 +130,16 @@ __device__ __forceinline__ void nodef_select(bool first, const NodeF<D> &x, cons
     o.h = first ? x.h : y.h;
 }
 
-template <int D, int N, int K>
+// Pairs of part H of LS: the N(N-1)/2 pairs a < b of a line, in the order (0,1), (0,2), ..,
+// (1,2), .., split into LS contiguous ranges (the line's work shared by LS items).
+template <int N, int LS, int H> __device__ __forceinline__ constexpr bool pair_in_part(int p0, int p1)
+{
+    constexpr int NPAIR = N * (N - 1) / 2;
+    const int idx = p0 * (2 * N - p0 - 1) / 2 + (p1 - p0 - 1);
+    return idx >= H * NPAIR / LS && idx < (H + 1) * NPAIR / LS;
+}
+
+template <int D, int N, int K, int LS = 1, int H = 0>
 __device__ __forceinline__ void pairs_f(const Ops1D &ops, const PhysConst &pc, const FastConst &fc, const float *sq,
                                         int line, float (&acc)[N][D + 2])
 {
@@ -116,6 +151,7 @@ __device__ __forceinline__ void pairs_f(const Ops1D &ops, const PhysConst &pc, c
     for (int p0 = 0; p0 < N; ++p0) {
 #pragma unroll
         for (int p1 = p0 + 1; p1 < N; ++p1) {
+            if (!pair_in_part<N, LS, H>(p0, p1)) continue;
             NodeF<D> A, B;
             nodef_load<D>(sq + line_node<D, N>(K, line, p0), NP, A);
             nodef_load<D>(sq + line_node<D, N>(K, line, p1), NP, B);
@@ -130,26 +166,77 @@ __device__ __forceinline__ void pairs_f(const Ops1D &ops, const PhysConst &pc, c
     }
 }
 
+// ---- Navier-Stokes terms (either precision) --------------------------------
+
+// Viscous part of the volume term along one line: node a gets -sum_b D[a][b] F_v,K(u_b, q_b)
+// (the conservative D-divergence of strong_dg.cpp:1386-1433 for the dissipative flux); sfv
+// holds the element's viscous fluxes [direction][state][node].
+template <int D, int N, int K, typename T, int LS = 1, int H = 0>
+__device__ __forceinline__ void visc_line(const Ops1D &ops, const T *sfv, int line, T (&acc)[N][D + 2])
+{
+    constexpr int NP = Geo<D, N>::NP, NS = D + 2;
+#pragma unroll
+    for (int b = H * N / LS; b < (H + 1) * N / LS; ++b) {
+        const int nb = line_node<D, N>(K, line, b);
+        T fv[NS];
+        for (int s = 0; s < NS; ++s) fv[s] = sfv[(K * NS + s) * NP + nb];
+#pragma unroll
+        for (int a2 = 0; a2 < N; ++a2) {
+            const T dab = std::is_same<T, float>::value ? T(ops.Df[a2][b]) : T(ops.D[a2][b]);
+            for (int s = 0; s < NS; ++s) acc[a2][s] -= dab * fv[s];
+        }
+    }
+}
+
+// Viscous numerical flux of a face node along sgn e_K (viscous_numerical_flux.cpp:144-328)
+// from the one-sided fluxes fa = F_v(ui, qi).e_K, fb = F_v(un, qn).e_K:
+// BR2  fvs = {{F_v}} . n;  SIPG also - penalty {{F_v(u, [[u]])}} . n.
+template <int D, int K, typename T>
+__device__ __forceinline__ void face_visc(const PhysConst &pc, T pen, const T *ui, const T *un, const T *fa,
+                                          const T *fb, T sgn, T (&fvs)[D + 2])
+{
+    constexpr int NS = D + 2;
+    for (int s = 0; s < NS; ++s) fvs[s] = T(0.5) * (fa[s] * sgn + fb[s] * sgn);
+    if (pc.diss_flux == 0) {
+        T jump[NS * D], ja[NS], jb[NS];
+        for (int s = 0; s < NS; ++s)
+            for (int d = 0; d < D; ++d) jump[s * D + d] = (d == K) ? (ui[s] - un[s]) * sgn : (ui[s] - un[s]) * T(0);
+        viscous_flux_dir<T, D>(ui, jump, K, pc, ja);
+        viscous_flux_dir<T, D>(un, jump, K, pc, jb);
+        for (int s = 0; s < NS; ++s) fvs[s] = fvs[s] - pen * (T(0.5) * (ja[s] * sgn + jb[s] * sgn));
+    }
+}
+
+// Block data a face item reads (LDS pointers; element strides from LineGeo).
+template <typename T> struct FaceIn {
+    const T *sq;    // node caches of the block
+    const T *sfv;   // viscous flux of the block's nodes, every direction (Navier-Stokes)
+    const T *tr;    // this face node's neighbour trace, stride tst
+    const T *trv;   // the neighbour's viscous flux along the face direction there (Navier-Stokes)
+    int tst;
+};
+
 // Surface term of element e at face node fn of its face (K, side): the
 // numerical flux with the reference orientation (the lower cell index is the
 // interior side), times -1/w_0 for the interior and +1/w_0 for the exterior
 // side, so the neighbour's term is the negative.  nbl >= 0: the neighbour is
 // element nbl of this block (node cache in LDS); otherwise its trace is tr
 // (state-major, stride tst).
-template <int D, int N, int K>
-__device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, const FastConst &fc, const float *sqb,
-                                       int le, int side, int fn, int e, int nb, int nbl, const float *tr, int tst,
-                                       float (&out)[D + 2])
+template <int D, int N, int K, bool V>
+__device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, const FastConst &fc,
+                                       const FaceIn<float> &in, int le, int side, int fn, int e, int nb, int nbl,
+                                       float pen, float (&out)[D + 2], float (&out_nb)[D + 2])
 {
-    using LG = LineGeo<D, N, float>;
+    using LG = LineGeo<D, N, float, 1, V>;
     constexpr int NP = LG::NP, NS = D + 2;
     const int fi = line_node<D, N>(K, fn, side ? N - 1 : 0);
+    const int fj = line_node<D, N>(K, fn, side ? 0 : N - 1);
     NodeF<D> qi, qn, L, R;
-    nodef_load<D>(sqb + le * LG::ESQ + fi, NP, qi);
+    nodef_load<D>(in.sq + le * LG::ESQ + fi, NP, qi);
     if (nbl >= 0) {
-        nodef_load<D>(sqb + nbl * LG::ESQ + line_node<D, N>(K, fn, side ? 0 : N - 1), NP, qn);
+        nodef_load<D>(in.sq + nbl * LG::ESQ + fj, NP, qn);
     } else {
-        for (int s = 0; s < NS; ++s) qn.u[s] = tr[s * tst];
+        for (int s = 0; s < NS; ++s) qn.u[s] = in.tr[s * in.tst];
         node_fill<D>(qn, fc);
     }
     const bool own = e < nb;
@@ -158,13 +245,33 @@ __device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, co
     const float sgn = side ? 1.f : -1.f;
     float fs[NS];
     num_flux_fast<D, K>(L, R, own ? sgn : -sgn, fc, pc, fs);
-    const float w = own ? -ops.inv_wbf : ops.inv_wbf;
-    for (int s = 0; s < NS; ++s) out[s] = w * fs[s];
+    if constexpr (!V) {
+        const float w = own ? -ops.inv_wbf : ops.inv_wbf;
+        for (int s = 0; s < NS; ++s) {
+            out[s] = w * fs[s];
+            out_nb[s] = -out[s];
+        }
+    } else {
+        // the viscous surface term: the average is antisymmetric between the two sides, the
+        // interior-flux part (strong form) is not, so both sides' values are returned
+        float fa[NS], fb[NS], fvs[NS];
+        for (int s = 0; s < NS; ++s) {
+            fa[s] = in.sfv[le * LG::ESFV + (K * NS + s) * NP + fi];
+            fb[s] = (nbl >= 0) ? in.sfv[nbl * LG::ESFV + (K * NS + s) * NP + fj] : in.trv[s * in.tst];
+        }
+        face_visc<D, K, float>(pc, pen, qi.u, qn.u, fa, fb, sgn, fvs);
+        const float fsgn = own ? 1.f : -1.f;
+        for (int s = 0; s < NS; ++s) {
+            const float A = -(fs[s] * fsgn) - fvs[s];
+            out[s] = ops.inv_wbf * (A + sgn * fa[s]);
+            out_nb[s] = ops.inv_wbf * ((-sgn * fb[s]) - A);
+        }
+    }
 }
 
 // ---- FP64 path ------------------------------------------------------------
 
-template <int D, int N, int K>
+template <int D, int N, int K, int LS = 1, int H = 0>
 __device__ __forceinline__ void pairs_d(const Ops1D &ops, const PhysConst &pc, const double *sq, int line,
                                         double (&acc)[N][D + 2])
 {
@@ -176,6 +283,7 @@ __device__ __forceinline__ void pairs_d(const Ops1D &ops, const PhysConst &pc, c
     for (int p0 = 0; p0 < N; ++p0) {
 #pragma unroll
         for (int p1 = p0 + 1; p1 < N; ++p1) {
+            if (!pair_in_part<N, LS, H>(p0, p1)) continue;
             const int na = line_node<D, N>(K, line, p0), nbn = line_node<D, N>(K, line, p1);
             double u1[NS], u2[NS];
             Prim<double, D> q1, q2;
@@ -203,9 +311,8 @@ __device__ __forceinline__ void pairs_d(const Ops1D &ops, const PhysConst &pc, c
 }
 
 template <int D, int N, int K>
-__device__ __forceinline__ void face_d(const Ops1D &ops, const PhysConst &pc, const double *sqb, int le, int side,
-                                       int fn, int e, int nb, int nbl, const double *tr, int tst,
-                                       double (&out)[D + 2])
+__device__ __forceinline__ void face_d(const Ops1D &ops, const PhysConst &pc, const FaceIn<double> &in, int le,
+                                       int side, int fn, int e, int nb, int nbl, double (&out)[D + 2])
 {
     using LG = LineGeo<D, N, double>;
     constexpr int NP = LG::NP, NS = D + 2;
@@ -213,9 +320,9 @@ __device__ __forceinline__ void face_d(const Ops1D &ops, const PhysConst &pc, co
     const int fj = line_node<D, N>(K, fn, side ? 0 : N - 1);
     double us[NS], unt[NS];
     for (int s = 0; s < NS; ++s) {
-        us[s] = sqb[le * LG::ESQ + s * NP + fi];
+        us[s] = in.sq[le * LG::ESQ + s * NP + fi];
         // in-block neighbour: its (round-tripped) state is in the node cache
-        unt[s] = (nbl >= 0) ? sqb[nbl * LG::ESQ + s * NP + fj] : tr[s * tst];
+        unt[s] = (nbl >= 0) ? in.sq[nbl * LG::ESQ + s * NP + fj] : in.tr[s * in.tst];
     }
     const bool own = e < nb;
     double ua[NS], ub[NS];
@@ -233,11 +340,13 @@ __device__ __forceinline__ void face_d(const Ops1D &ops, const PhysConst &pc, co
 }
 
 // One block's work: elements blk0 .. blk0 + EPB - 1 of the solve described by a.
-template <int D, int N, bool RT, typename T, int ES>
+// V: Navier-Stokes (split convective form plus the viscous volume and surface
+// terms, with the lifted gradient a.aux computed beforehand).
+template <int D, int N, bool RT, typename T, int ES, bool V = false>
 __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops, const PhysConst &pc, const int blk0)
 {
-    using LG = LineGeo<D, N, T, ES>;
-    constexpr int NP = LG::NP, NS = LG::NS, NL = LG::NL, EPB = LG::EPB, BLOCK = LG::BLOCK;
+    using LG = LineGeo<D, N, T, ES, V>;
+    constexpr int NP = LG::NP, NS = LG::NS, NL = LG::NL, EPB = LG::EPB, BLOCK = LG::BLOCK, LS = LG::LS;
     constexpr bool FAST = LG::FAST;
     constexpr int NFI = (LG::FITEMS + BLOCK - 1) / BLOCK;  // face-trace loads per thread
     constexpr int NFS = EPB * 2 * D;                       // face sides of the block
@@ -246,7 +355,9 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     // result in registers, so the line and face partials reuse their storage.
     __shared__ T sBuf[LG::SBUF];
     T *const sQ = sBuf;                       // per-node cache [element][field][node]
-    T *const sN = sBuf + EPB * LG::ESQ;       // neighbour face traces [element][state][face][face node]
+    T *const sN = sQ + EPB * LG::ESQ;         // neighbour face traces [element][state][face][face node]
+    T *const sFv = sN + EPB * LG::ESN;        // V: viscous flux [element][direction][state][node]
+    T *const sNv = sFv + EPB * LG::ESFV;      // V: neighbours' viscous flux, as the traces
     T *const sV = sBuf;                       // line partials [element][direction][state][node]
     T *const sF = sBuf + EPB * LG::ESV;       // face terms [element][face][state][face node]
     __shared__ int sNb[EPB][2 * D];           // face neighbours
@@ -254,6 +365,8 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
 
     const int tid = threadIdx.x;
     const FastConst fc = fast_const(pc);
+    // a neighbour is read from the block's own node cache when it is another element of the block
+    auto in_block = [&](int nb, int fe) { return nb >= blk0 && nb < blk0 + EPB && nb != fe; };
 
     // ---------------- phase A: loads first, then the node cache ----------
     const int le = tid / NP, i = tid % NP;
@@ -293,7 +406,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
                 const int f = r / NL, fn = r % NL;
                 const int nb = nbv[j];
                 if (fn == 0) sNb[fle][f] = nb;
-                if (nb < blk0 || nb >= blk0 + EPB) {  // in-block neighbours are read from the node cache
+                if (!in_block(nb, fe)) {
                     const int fj = line_node<D, N>(f >> 1, fn, (f & 1) ? 0 : N - 1);
                     for (int s = 0; s < NS; ++s) un[j][s] = a.u[((long)nb * NS + s) * NP + fj];
                     fdst[j] = fle * LG::ESN + r;
@@ -333,6 +446,39 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             for (int s = 0; s < NS; ++s) dst[s * LG::NFN] = unt[s];
         }
     }
+    if constexpr (V) {
+        // viscous fluxes from the lifted gradient: the own nodes' in every direction, the
+        // neighbours' face nodes' along the face direction
+        if (active) {
+            T g[NS * D], ut[NS], fv[D][NS];
+#pragma unroll
+            for (int x = 0; x < NS * D; ++x) g[x] = T(a.aux[x % D][((long)e * NS + x / D) * NP + i]);
+            for (int s = 0; s < NS; ++s) ut[s] = sQ[le * LG::ESQ + s * NP + i];
+            viscous_flux_all<T, D>(ut, g, pc, fv);
+#pragma unroll
+            for (int k = 0; k < D; ++k)
+                for (int s = 0; s < NS; ++s) sFv[le * LG::ESFV + (k * NS + s) * NP + i] = fv[k][s];
+        }
+#pragma unroll
+        for (int j = 0; j < NFI; ++j) {
+            if (fdst[j] < 0) continue;
+            const int t = tid + j * BLOCK;
+            const int f = (t % LG::NFN) / NL, fn = t % NL;
+            const int fj = line_node<D, N>(f >> 1, fn, (f & 1) ? 0 : N - 1);
+            const long nb = nbv[j];
+            T g[NS * D], ur[NS], fv[D][NS];
+#pragma unroll
+            for (int x = 0; x < NS * D; ++x) g[x] = T(a.aux[x % D][(nb * NS + x / D) * NP + fj]);
+            for (int s = 0; s < NS; ++s) ur[s] = T(un[j][s]);
+            viscous_flux_all<T, D>(ur, g, pc, fv);
+            T fk[NS];
+            for (int s = 0; s < NS; ++s) fk[s] = fv[0][s];
+#pragma unroll
+            for (int k = 1; k < D; ++k)
+                for (int s = 0; s < NS; ++s) fk[s] = (f >> 1) == k ? fv[k][s] : fk[s];
+            for (int s = 0; s < NS; ++s) sNv[fdst[j] + s * LG::NFN] = fk[s];
+        }
+    }
     __syncthreads();
 
     // ---------------- faces evaluated in this block ----------------------
@@ -350,7 +496,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             const int fle = r >> 1, f = 2 * kk + (r & 1);
             if (blk0 + fle < a.ncells) {
                 const int nb = sNb[fle][f];
-                want = !(nb >= blk0 && nb < blk0 + EPB && nb < blk0 + fle);
+                want = !(in_block(nb, blk0 + fle) && nb < blk0 + fle);
             }
         }
         const unsigned long long m = __ballot(want);
@@ -359,49 +505,61 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     }
 
     // ---------------- phase L: pair-line items, then face items ----------
-    constexpr int PITEMS = LG::ITEMS;                          // (direction, element, line)
+    constexpr int PITEMS = LG::ITEMS;                          // (direction, part, element, line)
     constexpr int JP = (PITEMS + BLOCK - 1) / BLOCK;           // slots that can hold a line item
     constexpr int JT = (PITEMS + NFS * NL + BLOCK - 1) / BLOCK;
+    // V with in-block neighbours: the neighbour's side of a shared face is not the negated own
+    // side (the interior viscous flux is one-sided), so it is held separately
+    constexpr bool NBT = V && EPB > 1;
     const int items = PITEMS + nface * NL;
+    const T pen = T(a.penalty);
     T rp[JP][N][NS];
     T rf[JT][NS];
+    T rfn[NBT ? JT : 1][NS];
 #pragma unroll
     for (int j = 0; j < JT; ++j) {
         const int t = tid + j * BLOCK;
         if (j < JP && t < PITEMS) {
-            const int k = t / (EPB * NL), r = t % (EPB * NL);
+            // t = ((k * LS + h) * EPB + element) * NL + line: direction k, part h of the line
+            const int kh = t / (EPB * NL), r = t % (EPB * NL);
+            const int k = kh / LS, h = kh % LS;
             const int lle = r / NL, line = r % NL;
             if (blk0 + lle >= a.ncells) continue;
             const T *q = sQ + lle * LG::ESQ;
-            if constexpr (FAST) {
-                if (k == 0) pairs_f<D, N, 0>(ops, pc, fc, q, line, rp[j < JP ? j : 0]);
-                else if (k == 1) pairs_f<D, N, 1>(ops, pc, fc, q, line, rp[j < JP ? j : 0]);
-                else if (D == 3) pairs_f<D, N, (D == 3 ? 2 : 1)>(ops, pc, fc, q, line, rp[j < JP ? j : 0]);
-            } else {
-                if (k == 0) pairs_d<D, N, 0>(ops, pc, q, line, rp[j < JP ? j : 0]);
-                else if (k == 1) pairs_d<D, N, 1>(ops, pc, q, line, rp[j < JP ? j : 0]);
-                else if (D == 3) pairs_d<D, N, (D == 3 ? 2 : 1)>(ops, pc, q, line, rp[j < JP ? j : 0]);
-            }
+            auto pitem = [&](auto kc, auto hc) {
+                constexpr int K = decltype(kc)::value, H = decltype(hc)::value;
+                if constexpr (FAST) pairs_f<D, N, K, LS, H>(ops, pc, fc, q, line, rp[j < JP ? j : 0]);
+                else pairs_d<D, N, K, LS, H>(ops, pc, q, line, rp[j < JP ? j : 0]);
+                if constexpr (V) visc_line<D, N, K, T, LS, H>(ops, sFv + lle * LG::ESFV, line, rp[j < JP ? j : 0]);
+            };
+            auto pdir = [&](auto kc) {
+                if (h == 0) pitem(kc, std::integral_constant<int, 0>{});
+                else if (LS > 1 && h == 1) pitem(kc, std::integral_constant<int, (LS > 1 ? 1 : 0)>{});
+                else if (LS > 2) pitem(kc, std::integral_constant<int, (LS > 2 ? 2 : 0)>{});
+            };
+            if (k == 0) pdir(std::integral_constant<int, 0>{});
+            else if (k == 1) pdir(std::integral_constant<int, 1>{});
+            else if (D == 3) pdir(std::integral_constant<int, (D == 3 ? 2 : 1)>{});
         } else if (t >= PITEMS && t < items) {
             const int u = (t - PITEMS) / NL, fn = (t - PITEMS) % NL;
             const int id = sList[wv][u];
             const int k = id / (2 * EPB), r = id % (2 * EPB);
             const int fle = r >> 1, side = r & 1;
             const int nb = sNb[fle][2 * k + side];
-            const int nbl = (nb >= blk0 && nb < blk0 + EPB) ? nb - blk0 : -1;
-            const T *tr = sN + fle * LG::ESN + (2 * k + side) * NL + fn;
-            if constexpr (FAST) {
-                if (k == 0) face_f<D, N, 0>(ops, pc, fc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN, rf[j]);
-                else if (k == 1) face_f<D, N, 1>(ops, pc, fc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN, rf[j]);
-                else if (D == 3)
-                    face_f<D, N, (D == 3 ? 2 : 1)>(ops, pc, fc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN,
-                                                   rf[j]);
-            } else {
-                if (k == 0) face_d<D, N, 0>(ops, pc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN, rf[j]);
-                else if (k == 1) face_d<D, N, 1>(ops, pc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN, rf[j]);
-                else if (D == 3)
-                    face_d<D, N, (D == 3 ? 2 : 1)>(ops, pc, sQ, fle, side, fn, blk0 + fle, nb, nbl, tr, LG::NFN, rf[j]);
-            }
+            const int nbl = in_block(nb, blk0 + fle) ? nb - blk0 : -1;
+            const int fo = (2 * k + side) * NL + fn;
+            const FaceIn<T> in{sQ, sFv, sN + fle * LG::ESN + fo, sNv + fle * LG::ESNV + fo, LG::NFN};
+            auto fitem = [&](auto kc) {
+                constexpr int K = decltype(kc)::value;
+                if constexpr (FAST)
+                    face_f<D, N, K, V>(ops, pc, fc, in, fle, side, fn, blk0 + fle, nb, nbl, pen, rf[j],
+                                       rfn[NBT ? j : 0]);
+                else
+                    face_d<D, N, K>(ops, pc, in, fle, side, fn, blk0 + fle, nb, nbl, rf[j]);
+            };
+            if (k == 0) fitem(std::integral_constant<int, 0>{});
+            else if (k == 1) fitem(std::integral_constant<int, 1>{});
+            else if (D == 3) fitem(std::integral_constant<int, (D == 3 ? 2 : 1)>{});
         }
     }
     __syncthreads();  // node cache and traces consumed; their storage now takes the partials
@@ -409,10 +567,11 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     for (int j = 0; j < JT; ++j) {
         const int t = tid + j * BLOCK;
         if (j < JP && t < PITEMS) {
-            const int k = t / (EPB * NL), r = t % (EPB * NL);
+            const int kh = t / (EPB * NL), r = t % (EPB * NL);
+            const int k = kh / LS;
             const int lle = r / NL, line = r % NL;
             if (blk0 + lle >= a.ncells) continue;
-            T *v = sV + lle * LG::ESV + k * NS * NP;
+            T *v = sV + lle * LG::ESV + kh * NS * NP;
 #pragma unroll
             for (int p = 0; p < N; ++p) {
                 const int n = (k == 0) ? line_node<D, N>(0, line, p)
@@ -427,9 +586,9 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             const int nb = sNb[fle][f];
             T *w = sF + fle * LG::ESF + f * NS * NL + fn;
             for (int s = 0; s < NS; ++s) w[s * NL] = rf[j][s];
-            if (nb >= blk0 && nb < blk0 + EPB) {  // the shared face's term for the neighbour
+            if (in_block(nb, blk0 + fle)) {  // the shared face's term for the neighbour
                 T *w2 = sF + (nb - blk0) * LG::ESF + (f ^ 1) * NS * NL + fn;
-                for (int s = 0; s < NS; ++s) w2[s * NL] = -rf[j][s];
+                for (int s = 0; s < NS; ++s) w2[s * NL] = NBT ? rfn[NBT ? j : 0][s] : -rf[j][s];
             }
         }
     }
@@ -443,7 +602,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     const T *v = sV + le * LG::ESV + i;
     for (int s = 0; s < NS; ++s) {
         T xs = v[s * NP];
-        for (int k = 1; k < D; ++k) xs += v[(k * NS + s) * NP];
+        for (int k = 1; k < D * LS; ++k) xs += v[(k * NS + s) * NP];
         acc[s] = xs;
     }
     const T *fb = sF + le * LG::ESF;
@@ -457,11 +616,11 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
 }
 
 // One solve: blocks in XCD-aware order over the element groups.
-template <int D, int N, bool RT, typename T>
-__global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK))
+template <int D, int N, bool RT, typename T, bool V = false>
+__global__ void __launch_bounds__((LineGeoK<D, N, T, V>::BLOCK))
 line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
-    line_block<D, N, RT, T, 1>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
+    line_block<D, N, RT, T, line_es<D, N, T, V>(), V>(a, ops, pc, xcd_block() * LineGeoK<D, N, T, V>::EPB);
 }
 
 // Many independent solves of one (dim, p, precision) in one launch: block b

@@ -331,4 +331,53 @@ __device__ __forceinline__ void viscous_flux_dir(const T *u, const T *g, int fd,
     f[D + 1] = fe + q;
 }
 
+// Dissipative flux in every direction at once, f[k] = F_v(u, grad u) . e_k: the primitive
+// gradient, viscosity and divergence are shared between the directions; each direction's
+// arithmetic is that of viscous_flux_dir.
+template <typename T, int D>
+__device__ __forceinline__ void viscous_flux_all(const T *u, const T *g, const PhysConst &pc, T (&f)[D][D + 2])
+{
+    const T gam = T(pc.gam), gamm1 = T(pc.gamm1), mach2 = T(pc.mach2);
+    const T rho = u[0];
+    T vel[D];
+    for (int k = 0; k < D; ++k) vel[k] = u[1 + k] / rho;
+    const T p = pressure<T, D>(u, gamm1);
+    T pg[D + 2][D];
+    for (int k = 0; k < D; ++k) pg[0][k] = g[0 * D + k];
+    for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b) pg[1 + a][b] = (g[(1 + a) * D + b] - vel[a] * g[0 * D + b]) / rho;
+    for (int a = 0; a < D; ++a) {
+        pg[D + 1][a] = g[(D + 1) * D + a];
+        for (int b = 0; b < D; ++b) pg[D + 1][a] -= T(0.5) * (vel[b] * g[(1 + b) * D + a] + u[1 + b] * pg[1 + b][a]);
+        pg[D + 1][a] *= gamm1;
+    }
+    const T T_ = gam * mach2 * (p / rho);
+    T mu;
+    if (pc.use_constant_viscosity) mu = T(1) * T(pc.mu_const);
+    else {
+        const T tr = T(pc.temperature_ratio);
+        mu = ((T(1) + tr) / (T_ + tr)) * dpow<T>(T_, T(1.5));
+    }
+    const T mu_hat = mu / T(pc.reynolds);
+    T divv = T(0);
+    for (int k = 0; k < D; ++k) divv += T(0.5) * (pg[1 + k][k] + pg[1 + k][k]);
+    const T mu2 = (T(-2) / T(3)) * mu_hat;
+    const T kappa = mu_hat / (gamm1 * mach2 * T(pc.prandtl));
+#pragma unroll
+    for (int fd = 0; fd < D; ++fd) {
+        T tau_fd[D];
+        for (int a = 0; a < D; ++a) {
+            tau_fd[a] = T(2) * mu_hat * (T(0.5) * (pg[1 + a][fd] + pg[1 + fd][a]));
+            if (a == fd) tau_fd[a] += mu2 * divv;
+        }
+        const T dT = (gam * mach2 * pg[D + 1][fd] - T_ * pg[0][fd]) / rho;
+        const T q = -kappa * dT;
+        f[fd][0] = T(0);
+        for (int a = 0; a < D; ++a) f[fd][1 + a] = -tau_fd[a];
+        T fe = T(0);
+        for (int a = 0; a < D; ++a) fe -= vel[a] * tau_fd[a];
+        f[fd][D + 1] = fe + q;
+    }
+}
+
 } // namespace dgx
