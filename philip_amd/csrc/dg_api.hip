@@ -280,8 +280,12 @@ int dgx_create(const dgx_params *p, dgx_ctx **out)
         hipMalloc(&c->u_dev, bytes) != hipSuccess || hipMalloc(&c->r_dev, bytes) != hipSuccess)
         return bad(DGX_ERR_HIP, "hipMalloc failed");
     if (p->pde == DGX_NAVIER_STOKES)
-        for (int k = 0; k < c->dim; ++k)
-            if (hipMalloc(&c->aux_dev[k], bytes) != hipSuccess) return bad(DGX_ERR_HIP, "hipMalloc failed");
+        for (int k = 0; k < c->dim; ++k) {
+            const hipError_t he = (p->precision == DGX_MIXED)
+                                      ? hipMalloc(&c->auxf_dev[k], sizeof(float) * (size_t)c->ndofs)
+                                      : hipMalloc(&c->aux_dev[k], bytes);
+            if (he != hipSuccess) return bad(DGX_ERR_HIP, "hipMalloc failed");
+        }
     if (hipMemcpy(c->nbr_dev, c->nbr_host.data(), sizeof(int) * c->nbr_host.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bad(DGX_ERR_HIP, "hipMemcpy failed");
     if (hipMemset(c->u_dev, 0, bytes) != hipSuccess || hipMemset(c->r_dev, 0, bytes) != hipSuccess)
@@ -298,7 +302,10 @@ int dgx_destroy(dgx_ctx *c)
     (void)hipFree(c->u_dev);
     (void)hipFree(c->r_dev);
     (void)hipFree(c->minv_dev);
-    for (int k = 0; k < 3; ++k) (void)hipFree(c->aux_dev[k]);
+    for (int k = 0; k < 3; ++k) {
+        (void)hipFree(c->aux_dev[k]);
+        (void)hipFree(c->auxf_dev[k]);
+    }
     for (double *r : c->regs) (void)hipFree(r);
     (void)hipFree(c->iq_ops);
     (void)hipFree(c->iq_partial);
@@ -342,9 +349,17 @@ int dgx_get_auxiliary_solution(const dgx_ctx *cc, int idim, double *host_q)
 {
     dgx_ctx *c = const_cast<dgx_ctx *>(cc);
     if (!c || !host_q || idim < 0 || idim >= c->dim) return DGX_ERR_INVALID;
-    if (!c->aux_dev[idim]) return fail(c, DGX_ERR_STATE, "no auxiliary solution (Euler)");
-    HIPCHK(hipMemcpyAsync(host_q, c->aux_dev[idim], sizeof(double) * c->ndofs, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!c->aux_dev[idim] && !c->auxf_dev[idim]) return fail(c, DGX_ERR_STATE, "no auxiliary solution (Euler)");
+    if (c->aux_dev[idim]) {
+        HIPCHK(hipMemcpyAsync(host_q, c->aux_dev[idim], sizeof(double) * c->ndofs, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } else {  // mixed path: q is held in float
+        std::vector<float> qf((size_t)c->ndofs);
+        HIPCHK(hipMemcpyAsync(qf.data(), c->auxf_dev[idim], sizeof(float) * c->ndofs, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (size_t k = 0; k < qf.size(); ++k) host_q[k] = qf[k];
+    }
     return DGX_OK;
 }
 
@@ -361,7 +376,10 @@ static StageArgs base_args(dgx_ctx *c, const double *u)
 {
     StageArgs a{};
     a.u = u;
-    for (int k = 0; k < 3; ++k) a.aux[k] = c->aux_dev[k];
+    for (int k = 0; k < 3; ++k) {
+        a.aux[k] = c->aux_dev[k];
+        a.auxf[k] = c->auxf_dev[k];
+    }
     a.nbr = c->nbr_dev;
     a.ncells = c->ncells;
     a.inv_h = 1.0 / c->h;

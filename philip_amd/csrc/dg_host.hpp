@@ -51,7 +51,8 @@ struct dgx_ctx {
     int *nbr_dev = nullptr;
     double *u_dev = nullptr;       // DGBase::solution
     double *r_dev = nullptr;       // DGBase::right_hand_side
-    double *aux_dev[3] = {nullptr, nullptr, nullptr};
+    double *aux_dev[3] = {nullptr, nullptr, nullptr};   // lifted gradient, FP64 path
+    float *auxf_dev[3] = {nullptr, nullptr, nullptr};   // lifted gradient, mixed path
     double *minv_dev = nullptr;    // lifted right-hand side (dgx_apply_inverse_mass_to_rhs)
     // ODE system
     int method = -1;

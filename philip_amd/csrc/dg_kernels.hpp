@@ -13,6 +13,7 @@
 //   RK       src/ode_solver/runge_kutta_ode_solver.cpp:15-99,
 //            src/ode_solver/low_storage_runge_kutta_ode_solver.cpp:24-79
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include "dg_physics.hpp"
@@ -35,7 +36,8 @@ enum StageMode { MODE_RESIDUAL = 0, MODE_RHS = 1, MODE_LSRK = 2, MODE_BUTCHER = 
 
 struct StageArgs {
     const double *u;          // stage solution (read incl. neighbour face layers)
-    const double *aux[3];     // lifted gradient per direction (Navier-Stokes)
+    const double *aux[3];     // lifted gradient per direction (Navier-Stokes, FP64 path)
+    const float *auxf[3];     // the same stored in float (Navier-Stokes, mixed path)
     const int *nbr;           // [cell][2*dim]
     int ncells;
     double inv_h;             // 1/h
@@ -58,6 +60,14 @@ struct StageArgs {
     double b_dt;              // dt * b_i
     int first_stage, last_stage, stage;
 };
+
+// Lifted gradient component (state s, direction d) at flat index idx of the
+// layout [cell][state][node], in the precision of the path's fluxes.
+template <typename T> __device__ __forceinline__ T aux_at(const StageArgs &a, int d, long idx)
+{
+    if constexpr (std::is_same<T, float>::value) return a.auxf[d][idx];
+    else return T(a.aux[d][idx]);
+}
 
 template <int D, int N> struct Shape {
     static constexpr int NP = (D == 3) ? N * N * N : N * N;
@@ -189,10 +199,12 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs &a, const Ops1D &
 
 // Lifted-gradient (auxiliary) equation, collocated GLL:
 //   q_d = (1/h) [ D_d u + (1/w_0) (u_hat - u) n_d ]  on faces normal to d, u_hat = {{u}}.
-template <int D, int N>
+// Q: storage type of q (double for the FP64 path, float for the mixed path, whose
+// kernels evaluate the viscous fluxes in float).
+template <int D, int N, typename Q>
 __global__ void __launch_bounds__((Shape<D, N>::BLOCK))
-aux_kernel(const double *__restrict__ u, double *__restrict__ q0, double *__restrict__ q1,
-           double *__restrict__ q2, const int *__restrict__ nbr, int ncells, double inv_h, Ops1D ops)
+aux_kernel(const double *__restrict__ u, Q *__restrict__ q0, Q *__restrict__ q1,
+           Q *__restrict__ q2, const int *__restrict__ nbr, int ncells, double inv_h, Ops1D ops)
 {
     using S = Shape<D, N>;
     constexpr int NP = S::NP, NS = S::NS, EPB = S::EPB;
@@ -212,7 +224,7 @@ aux_kernel(const double *__restrict__ u, double *__restrict__ q0, double *__rest
     if (!active) return;
     int c[3];
     node_coords<D, N>(i, c);
-    double *qout[3] = {q0, q1, q2};
+    Q *qout[3] = {q0, q1, q2};
     for (int d = 0; d < D; ++d) {
         double g[NS];
         for (int s = 0; s < NS; ++s) {
@@ -231,7 +243,7 @@ aux_kernel(const double *__restrict__ u, double *__restrict__ q0, double *__rest
                 g[s] += ops.inv_wb * ((avg - ui[s]) * sgn);
             }
         }
-        for (int s = 0; s < NS; ++s) qout[d][((long)e * NS + s) * NP + i] = g[s] * inv_h;
+        for (int s = 0; s < NS; ++s) qout[d][((long)e * NS + s) * NP + i] = Q(g[s] * inv_h);
     }
 }
 

@@ -98,8 +98,12 @@ static int launch_aux_dn(dgx_ctx *c, const double *u)
 {
     using S = Shape<D, N>;
     const int blocks = (c->ncells + S::EPB - 1) / S::EPB;
-    hipLaunchKernelGGL((aux_kernel<D, N>), dim3(blocks), dim3(S::BLOCK), 0, c->stream, u, c->aux_dev[0],
-                       c->aux_dev[1], c->aux_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops);
+    if (c->P.precision == DGX_MIXED)
+        hipLaunchKernelGGL((aux_kernel<D, N, float>), dim3(blocks), dim3(S::BLOCK), 0, c->stream, u, c->auxf_dev[0],
+                           c->auxf_dev[1], c->auxf_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops);
+    else
+        hipLaunchKernelGGL((aux_kernel<D, N, double>), dim3(blocks), dim3(S::BLOCK), 0, c->stream, u, c->aux_dev[0],
+                           c->aux_dev[1], c->aux_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops);
     return check_launch(c, "aux_kernel");
 }
 

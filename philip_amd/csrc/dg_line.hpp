@@ -452,7 +452,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
         if (active) {
             T g[NS * D], ut[NS], fv[D][NS];
 #pragma unroll
-            for (int x = 0; x < NS * D; ++x) g[x] = T(a.aux[x % D][((long)e * NS + x / D) * NP + i]);
+            for (int x = 0; x < NS * D; ++x) g[x] = aux_at<T>(a, x % D, ((long)e * NS + x / D) * NP + i);
             for (int s = 0; s < NS; ++s) ut[s] = sQ[le * LG::ESQ + s * NP + i];
             viscous_flux_all<T, D>(ut, g, pc, fv);
 #pragma unroll
@@ -468,7 +468,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             const long nb = nbv[j];
             T g[NS * D], ur[NS], fv[D][NS];
 #pragma unroll
-            for (int x = 0; x < NS * D; ++x) g[x] = T(a.aux[x % D][(nb * NS + x / D) * NP + fj]);
+            for (int x = 0; x < NS * D; ++x) g[x] = aux_at<T>(a, x % D, (nb * NS + x / D) * NP + fj);
             for (int s = 0; s < NS; ++s) ur[s] = T(un[j][s]);
             viscous_flux_all<T, D>(ur, g, pc, fv);
             T fk[NS];

@@ -119,7 +119,7 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
             T g[NS * D];
             if (NSV)
                 for (int s = 0; s < NS; ++s)
-                    for (int d = 0; d < D; ++d) g[s * D + d] = T(a.aux[d][((long)e * NS + s) * NP + i]);
+                    for (int d = 0; d < D; ++d) g[s * D + d] = aux_at<T>(a, d, ((long)e * NS + s) * NP + i);
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 T f[NS];
@@ -192,8 +192,8 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                 for (int s = 0; s < NS; ++s) {
                     ui[s] = T(a.u[((long)fe * NS + s) * NP + fi]);
                     for (int d = 0; d < D; ++d) {
-                        gi[s * D + d] = T(a.aux[d][((long)fe * NS + s) * NP + fi]);
-                        gn[s * D + d] = T(a.aux[d][((long)nb * NS + s) * NP + fj]);
+                        gi[s * D + d] = aux_at<T>(a, d, ((long)fe * NS + s) * NP + fi);
+                        gn[s * D + d] = aux_at<T>(a, d, ((long)nb * NS + s) * NP + fj);
                         jump[s * D + d] = (d == K) ? (ui[s] - un[s]) * sgn : (ui[s] - un[s]) * T(0);
                     }
                 }
