@@ -129,8 +129,8 @@ static int line_batch_es(const BatchBlock *blocks, int nblocks, const StageArgs 
                          const Ops1D &ops, hipStream_t st, int *epb)
 {
     using LG = LineGeo<D, N, T, ES>;
-    if constexpr (LG::SBUF * sizeof(T) > 65536) {
-        return DGX_ERR_UNSUPPORTED;  // variant would not fit the per-block LDS budget
+    if constexpr (LG::SBUF * sizeof(T) > 160 * 1024 - 4096) {
+        return DGX_ERR_UNSUPPORTED;  // variant would not fit the 160 KB of LDS a block can hold
     } else {
         if (epb) *epb = LG::EPB;
         if (!blocks) return DGX_OK;  // query only

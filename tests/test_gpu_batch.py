@@ -55,6 +55,10 @@ def test_batch_matches_single_solves():
                        mach_inf=MACH_VORTEX, conv_flux=P.LXF), (3.0, 0.7, 0.8)))
     cases.append((dict(dim=3, poly_degree=3, n_cells_1d=4, use_split_form=1, two_point_flux=P.CH,
                        conv_flux=P.TWO_POINT_ROE, precision=P.MIXED), None))
+    # long 3D lines (shared by two or three items; the FP64 p=6 block holds ~135 KB of LDS)
+    for p_, prec in ((5, P.FP64), (6, P.FP64), (6, P.MIXED)):
+        cases.append((dict(dim=3, poly_degree=p_, n_cells_1d=2, use_split_form=1, two_point_flux=P.CH,
+                           conv_flux=P.TWO_POINT_ROE, precision=prec), None))
     log(f"batch of {len(cases)} instances")
     b = P.Batch([kw for kw, _ in cases])
     dts = []
