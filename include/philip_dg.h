@@ -104,7 +104,9 @@ int dgx_set_solution(dgx_ctx *ctx, const double *host_u);
 int dgx_get_solution(const dgx_ctx *ctx, double *host_u);
 /* DGBase::right_hand_side (dg_base.hpp:396): the un-lifted residual. */
 int dgx_get_right_hand_side(const dgx_ctx *ctx, double *host_r);
-/* DGBase::auxiliary_solution[idim] (dg_base.hpp:419), Navier-Stokes only. */
+/* DGBase::auxiliary_solution[idim] (dg_base.hpp:419), Navier-Stokes only.  FP64 contexts hold q
+   in double; mixed-precision contexts hold it in float (their fluxes are float) and return it
+   widened to double. */
 int dgx_get_auxiliary_solution(const dgx_ctx *ctx, int idim, double *host_q);
 /* integer connectivity used by the kernels: nbr[cell][2*dim] (deal.II face numbering). */
 int dgx_get_neighbors(const dgx_ctx *ctx, int32_t *host_nbr);
