@@ -45,6 +45,27 @@ def test_mixed_residual_error(scheme):
     assert e <= 5e-4
 
 
+@pytest.mark.parametrize("diss", [O.SIPG, O.BR2])
+@pytest.mark.parametrize("p", [2, 3, 5, 6])
+def test_mixed_navier_stokes_residual_error(diss, p):
+    """Split-form Navier-Stokes residual of the mixed path (line kernel with the viscous terms,
+    float lifted gradient) against the FP64 oracle on a perturbed TGV state: p=3 has elements
+    sharing faces inside a block, p=5 and p=6 split each line over two and three items."""
+    kw = dict(dim=3, poly_degree=p, n_cells_1d=4 if p <= 3 else 2, pde=O.NAVIER_STOKES, diss_flux=diss,
+              reynolds_inf=1600.0, prandtl=0.71, mach_inf=0.1, use_split_form=1, two_point_flux=O.RA,
+              conv_flux=O.TWO_POINT_ROE)
+    o = O.Oracle(**kw)
+    d = P.DG(precision=P.MIXED, **kw)
+    rng = np.random.default_rng(300 + p)
+    v = o.ic_tgv().reshape(o.ncells, o.ns, o.Np)
+    v = (v + 1e-3 * np.abs(v).max(axis=(0, 2), keepdims=True) * rng.standard_normal(v.shape)).ravel()
+    d.set_solution(v)
+    e = rel(d.assemble_residual(), o.residual(v))
+    eq = max(rel(d.auxiliary_solution(k), o.auxiliary(k)) for k in range(3))
+    log(f"mixed NS residual rel err {e:.3e}, lifted gradient {eq:.3e} (diss={diss}, p={p})")
+    assert e <= 5e-4 and eq <= 1e-6
+
+
 def test_mixed_ke_history_within_1e6():
     kw = dict(n_cells_1d=8, **CFG2)
     o = O.Oracle(**kw)
