@@ -1,5 +1,7 @@
 // Line-structured fused stage kernel for the split (two-point) form of the
-// Euler equations on collocated GLL nodes.
+// Euler equations on collocated GLL nodes, and of the Navier-Stokes equations
+// on the mixed path (V: the viscous volume and surface terms added to the same
+// items, navier_stokes.cpp / viscous_numerical_flux.cpp).
 //
 // The skew-symmetric Hadamard volume term (strong_dg.cpp:1017-1579, Chan 2019
 // eq. 15) couples node a only with the nodes b on its own grid line in each
@@ -18,9 +20,10 @@
 //                     phase the neighbours' face traces (one item per face
 //                     node) are loaded into LDS, so the line phase makes no
 //                     global loads
-//   L  line-parallel  pairs + faces of one line, sums kept in registers;
-//                     after a barrier they go to LDS over the (now dead)
-//                     node cache and traces
+//   L  line-parallel  pairs + faces of one line, sums kept in registers
+//                     (a long line's pairs shared by LS items); after a
+//                     barrier they go to LDS over the (now dead) node cache
+//                     and traces
 //   E  node-parallel  sum of the direction partials, FP64 epilogue
 // T = float: mixed path (per-node cache of derived quantities, float fluxes,
 // FP64 state and RK registers); T = double: FP64 path (the point physics of
