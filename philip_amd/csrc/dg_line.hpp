@@ -56,9 +56,12 @@ template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
     static constexpr int pad1(int x) { return x + (33 - x % 32) % 32; }
     static constexpr int ESQ = pad1(NQ * NP);
     // Long lines are shared by LS items (the pairs and the viscous columns split evenly), so
-    // that the line phase keeps most of the block busy: from p = 4 in 2D and for Navier-Stokes,
-    // from p = 5 for 3D Euler (at p = 4 the extra partials cost more than the balance gains)
-    static constexpr int LS = (V || D == 2) ? (N >= 7 ? 3 : (N >= 5 ? 2 : 1)) : (N >= 7 ? 3 : (N >= 6 ? 2 : 1));
+    // that the line phase keeps most of the block busy: from p = 4 in 2D and for Navier-Stokes;
+    // for 3D Euler at p = 5, and at p = 6 on the FP64 path only (at p = 4 the extra partials
+    // cost more than the balance gains; at p = 6 the mixed path gained nothing and its batched
+    // kernel's item loop was no longer unrolled, spilling the line sums to scratch)
+    static constexpr int LS = (V || D == 2) ? (N >= 7 ? 3 : (N >= 5 ? 2 : 1))
+                                            : (N >= 7 ? (FAST ? 1 : 3) : (N >= 6 ? 2 : 1));
     static constexpr int ESV = pad1(LS * D * NS * NP);  // line partials [direction][part][state][node]
     static constexpr int NFN = 2 * D * NL;           // face nodes per element
     static constexpr int ESN = pad1(NS * NFN);       // neighbour traces [state][face][face node]
