@@ -288,6 +288,8 @@ def main():
         line["navier_stokes"] = [navier_stokes_tgv("mixed"), navier_stokes_tgv("fp64")]
     if not args.no_serving:
         line["serving"] = serving()
+        # arrival rate swept at both precisions (same request stream per rate)
+        line["serving_sweep"] = [serving(rate=r, precision=pr) for pr in ("mixed", "fp64") for r in (20.0, 40.0, 80.0)]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_threads)
     print(json.dumps(line), flush=True)
