@@ -190,15 +190,19 @@ def serving(n_req=48, rate=40.0, seed=77, precision="mixed", time_scale=0.1):
                 workload=f"70% 2D vortex / 30% 3D TGV, final time U[0.5,2]x{time_scale}, RK4, HBM budget 64 GiB")
 
 
+def ns_tgv_params():
+    import philip_amd as P
+    return dict(dim=3, poly_degree=5, n_cells_1d=16, pde=P.NAVIER_STOKES, diss_flux=P.BR2, use_constant_viscosity=1,
+                constant_viscosity=1.0, reynolds_inf=1600.0, prandtl=0.71, mach_inf=0.1, use_split_form=1,
+                two_point_flux=P.RA, conv_flux=P.TWO_POINT_ROE)
+
+
 def navier_stokes_tgv(precision="mixed", steps=3):
     """Navier-Stokes TGV, Re 1600, Pr 0.71, M 0.1, p=5 on 16^3 hex (4,423,680 DOF), BR2, Ranocha
     split form + EC/Roe surface flux, constant viscosity, classical RK4 (auxiliary-gradient kernel +
     stage kernel per stage).  Returns DOF-updates/s over the timed steps."""
     import philip_amd as P
-    d = P.DG(dim=3, poly_degree=5, n_cells_1d=16, pde=P.NAVIER_STOKES, diss_flux=P.BR2, use_constant_viscosity=1,
-             constant_viscosity=1.0, reynolds_inf=1600.0, prandtl=0.71, mach_inf=0.1, use_split_form=1,
-             two_point_flux=P.RA, conv_flux=P.TWO_POINT_ROE,
-             precision=P.MIXED if precision == "mixed" else P.FP64)
+    d = P.DG(precision=P.MIXED if precision == "mixed" else P.FP64, **ns_tgv_params())
     d.set_initial_condition_tgv()
     dt = d.constant_time_step(0.01)
     d.allocate_ode_system("rk4")
@@ -212,6 +216,49 @@ def navier_stokes_tgv(precision="mixed", steps=3):
     d.close()
     return dict(value=n * 4 * steps / t, unit="DOF-updates/s", dof=n, ms_per_stage=1e3 * t / (4 * steps),
                 precision=precision, workload="NS TGV Re1600 p=5 16^3, BR2, Ranocha split + EC/Roe, RK4")
+
+
+def navier_stokes_history(final_time=2.0, every=100):
+    """The Navier-Stokes TGV above run to t = final_time (CFL 0.01, last step clipped) at both
+    precisions, enstrophy and the vorticity-based dissipation rate (QGauss(p+11)) every `every`
+    steps and at the end.  Reports each run's steps and wall time and the largest relative
+    difference of the mixed history from the FP64 one (the FP64 device path restates the oracle
+    to 1e-12 per stage; the oracle itself would need ~1 h of CPU time for this run)."""
+    import numpy as np
+    import philip_amd as P
+    hist, out = {}, {}
+    for prec in ("fp64", "mixed"):
+        d = P.DG(precision=P.MIXED if prec == "mixed" else P.FP64, **ns_tgv_params())
+        d.set_initial_condition_tgv()
+        dt = d.constant_time_step(0.01)
+        d.allocate_ode_system("rk4")
+        n_full = int(final_time / dt)
+        rows = []
+        t0 = time.perf_counter()
+        done = 0
+        while done < n_full:
+            n = min(every, n_full - done)
+            d.step_in_time(dt, n)
+            done += n
+            q = d.integrated()
+            rows.append((d.current_time, q["enstrophy"], q["vorticity_dissipation"]))
+        rem = final_time - d.current_time
+        if rem > 1e-12 * final_time:
+            d.step_in_time(rem, 1)
+            q = d.integrated()
+            rows.append((d.current_time, q["enstrophy"], q["vorticity_dissipation"]))
+        d.synchronize()
+        wall = time.perf_counter() - t0
+        hist[prec] = np.array(rows)
+        out[prec] = dict(steps=d.current_iteration, final_time=d.current_time, wall_s=wall,
+                         enstrophy=rows[-1][1], vorticity_dissipation=rows[-1][2])
+        d.close()
+    a, b = hist["mixed"], hist["fp64"]
+    out["max_rel_diff_enstrophy"] = float(np.max(np.abs(a[:, 1] - b[:, 1]) / np.abs(b[:, 1])))
+    out["max_rel_diff_dissipation"] = float(np.max(np.abs(a[:, 2] - b[:, 2]) / np.abs(b[:, 2])))
+    out["samples"] = len(a)
+    out["workload"] = f"NS TGV Re1600 p=5 16^3, BR2, Ranocha split + EC/Roe, RK4 to t={final_time}"
+    return out
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -242,6 +289,8 @@ def main():
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances line")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream line")
+    ap.add_argument("--no-ns-history", action="store_true",
+                    help="skip the Navier-Stokes TGV run to t=2 at both precisions")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -286,6 +335,8 @@ def main():
     if not args.no_batch:
         line["batched"] = batched_vortex()
         line["navier_stokes"] = [navier_stokes_tgv("mixed"), navier_stokes_tgv("fp64")]
+    if not args.no_ns_history:
+        line["navier_stokes_t2"] = navier_stokes_history()
     if not args.no_serving:
         line["serving"] = serving()
         # arrival rate swept at both precisions (same request stream per rate)
