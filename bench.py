@@ -102,6 +102,42 @@ def run_gpu(precision, steps, warmup):
                 ndofs=d.ndofs, stages=stages, ke=ke, pcie=pcie)
 
 
+def tgv_history(steps=1000, every=50):
+    """The headline workload (Euler TGV p=3 32^3, CH split + EC/Roe, RK3(2)5F[3S*+], CFL 0.01) run
+    for `steps` steps at both precisions; kinetic energy and numerical entropy every `every` steps.
+    Reports the largest relative difference of the mixed KE history from the FP64 one, each run's
+    entropy drift |S(t) - S(0)| / |S(0)| and the mixed run's entropy drift from the FP64 one."""
+    import numpy as np
+    import philip_amd as P
+    hist, out = {}, {}
+    for prec in ("fp64", "mixed"):
+        d = P.DG(**workload_params(P.FP64 if prec == "fp64" else P.MIXED))
+        d.set_initial_condition_tgv()
+        dt = d.constant_time_step(0.01)
+        d.allocate_ode_system("rk3_2_5f_3sstarplus")
+        rows = [(0.0, d.integrated()["kinetic_energy"], d.numerical_entropy())]
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            n = min(every, steps - done)
+            d.step_in_time(dt, n)
+            done += n
+            rows.append((d.current_time, d.integrated()["kinetic_energy"], d.numerical_entropy()))
+        d.synchronize()
+        wall = time.perf_counter() - t0
+        h = np.array(rows)
+        hist[prec] = h
+        out[prec] = dict(steps=steps, final_time=d.current_time, wall_s=wall, kinetic_energy=h[-1, 1],
+                         entropy_drift=float(np.max(np.abs(h[:, 2] - h[0, 2])) / abs(h[0, 2])))
+        d.close()
+    a, b = hist["mixed"], hist["fp64"]
+    out["max_rel_diff_kinetic_energy"] = float(np.max(np.abs(a[:, 1] - b[:, 1]) / np.abs(b[:, 1])))
+    out["max_entropy_diff_rel_S0"] = float(np.max(np.abs(a[:, 2] - b[:, 2])) / abs(b[0, 2]))
+    out["samples"] = len(a)
+    out["workload"] = f"Euler TGV p=3 32^3, CH split + EC/Roe, RK3(2)5F[3S*+], {steps} steps"
+    return out
+
+
 MACH_VORTEX = 1.195228609334  # sqrt(2/gamma): p_inf = 1/(gamma M^2) = 1/2 for the isentropic vortex
 
 
@@ -290,7 +326,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances line")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream line")
     ap.add_argument("--no-ns-history", action="store_true",
-                    help="skip the Navier-Stokes TGV run to t=2 at both precisions")
+                    help="skip the long runs at both precisions (Euler TGV 1000 steps, Navier-Stokes TGV to t=2)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -336,6 +372,7 @@ def main():
         line["batched"] = batched_vortex()
         line["navier_stokes"] = [navier_stokes_tgv("mixed"), navier_stokes_tgv("fp64")]
     if not args.no_ns_history:
+        line["tgv_1000_steps"] = tgv_history()
         line["navier_stokes_t2"] = navier_stokes_history()
     if not args.no_serving:
         line["serving"] = serving()
