@@ -141,7 +141,7 @@ def tgv_history(steps=1000, every=50):
 MACH_VORTEX = 1.195228609334  # sqrt(2/gamma): p_inf = 1/(gamma M^2) = 1/2 for the isentropic vortex
 
 
-def batched_vortex(n_inst=256, steps=20, seed=2024, precision="mixed"):
+def batched_vortex(n_inst=256, steps=200, seed=2024, precision="mixed"):
     """Batched uneven instances: 2D isentropic vortex on [-5,5]^2, N ~ U{8..64} elements per side,
     p ~ U{2..6}, strength ~ U[1,5], advection ~ U[0.5,1]^2 (seeded), CH split form + EC/Roe surface
     flux, RK4; all instances advance with one graph launch per step.  Returns DOF-updates/s over
