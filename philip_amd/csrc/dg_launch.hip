@@ -39,7 +39,10 @@ static void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
             if constexpr (std::is_same<T, float>::value) {
                 using LV = LineGeoK<D, N, T, true>;
                 const dim3 vgrid((c->ncells + LV::EPB - 1) / LV::EPB), vblock(LV::BLOCK);
-                hipLaunchKernelGGL((line_kernel<D, N, RT, T, true>), vgrid, vblock, 0, c->stream, a, c->ops, c->pc);
+                if constexpr (N == 6)
+                    hipLaunchKernelGGL((line_kernel_ns_w4<D, N>), vgrid, vblock, 0, c->stream, a, c->ops, c->pc);
+                else
+                    hipLaunchKernelGGL((line_kernel_ns<D, N>), vgrid, vblock, 0, c->stream, a, c->ops, c->pc);
             } else {
                 hipLaunchKernelGGL((stage2_kernel<D, N, true, true, RT, T>), grid, block, 0, c->stream, a, c->ops,
                                    c->pc);

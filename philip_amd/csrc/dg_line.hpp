@@ -624,11 +624,29 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
 }
 
 // One solve: blocks in XCD-aware order over the element groups.
-template <int D, int N, bool RT, typename T, bool V = false>
-__global__ void __launch_bounds__((LineGeoK<D, N, T, V>::BLOCK))
+template <int D, int N, bool RT, typename T>
+__global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK))
 line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
-    line_block<D, N, RT, T, line_es<D, N, T, V>(), V>(a, ops, pc, xcd_block() * LineGeoK<D, N, T, V>::EPB);
+    line_block<D, N, RT, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
+}
+
+// Navier-Stokes, mixed path.  line_kernel_ns_w4 is held to 4 waves/SIMD (128 VGPRs; a few
+// values spill): used where that measured faster, p=5 (142 -> 122 us per stage at 16^3); at
+// p=2..4 and p=6 it was not (p=3: 93 -> 98 us), so those keep the compiler's choice.
+template <int D, int N>
+__global__ void __launch_bounds__((LineGeoK<D, N, float, true>::BLOCK))
+line_kernel_ns(StageArgs a, Ops1D ops, PhysConst pc)
+{
+    line_block<D, N, false, float, line_es<D, N, float, true>(), true>(a, ops, pc,
+                                                                       xcd_block() * LineGeoK<D, N, float, true>::EPB);
+}
+template <int D, int N>
+__global__ void __launch_bounds__((LineGeoK<D, N, float, true>::BLOCK)) __attribute__((amdgpu_waves_per_eu(4)))
+line_kernel_ns_w4(StageArgs a, Ops1D ops, PhysConst pc)
+{
+    line_block<D, N, false, float, line_es<D, N, float, true>(), true>(a, ops, pc,
+                                                                       xcd_block() * LineGeoK<D, N, float, true>::EPB);
 }
 
 // Many independent solves of one (dim, p, precision) in one launch: block b
