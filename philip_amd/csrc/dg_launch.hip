@@ -137,8 +137,12 @@ static int line_batch_es(const BatchBlock *blocks, int nblocks, const StageArgs 
     } else {
         if (epb) *epb = LG::EPB;
         if (!blocks) return DGX_OK;  // query only
-        hipLaunchKernelGGL((line_kernel_batch<D, N, RT, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks, args,
-                           pcs, ops);
+        if constexpr (D == 2 && std::is_same<T, float>::value)
+            hipLaunchKernelGGL((line_kernel_batch_w5<D, N, RT, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
+                               args, pcs, ops);
+        else
+            hipLaunchKernelGGL((line_kernel_batch<D, N, RT, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
+                               args, pcs, ops);
         return hipGetLastError() == hipSuccess ? DGX_OK : DGX_ERR_HIP;
     }
 }
