@@ -137,7 +137,7 @@ static int line_batch_es(const BatchBlock *blocks, int nblocks, const StageArgs 
     } else {
         if (epb) *epb = LG::EPB;
         if (!blocks) return DGX_OK;  // query only
-        if constexpr (D == 2 && std::is_same<T, float>::value)
+        if constexpr (D == 2 && N < 7 && std::is_same<T, float>::value)
             hipLaunchKernelGGL((line_kernel_batch_w5<D, N, RT, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
                                args, pcs, ops);
         else

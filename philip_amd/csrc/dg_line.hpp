@@ -664,7 +664,8 @@ line_kernel_batch(const BatchBlock *__restrict__ blocks, const StageArgs *__rest
     line_block<D, N, RT, T, ES>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
 }
 // The same held to 5 waves/SIMD (96 VGPRs, a few values spilled): used for the 2D mixed-path
-// groups, where it measured faster (256-instance vortex mix: 2.72 -> 2.63 ms per step).
+// groups up to p=5, where it measured faster (p=3: 64 -> 55 us, p=4: 141 -> 125 us per stage in
+// the 256-instance vortex mix); at p=6 it was slower (254 -> 263 us).
 template <int D, int N, bool RT, typename T, int ES>
 __global__ void __launch_bounds__((LineGeo<D, N, T, ES>::BLOCK)) __attribute__((amdgpu_waves_per_eu(5)))
 line_kernel_batch_w5(const BatchBlock *__restrict__ blocks, const StageArgs *__restrict__ args,
