@@ -332,8 +332,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
+        print("bench.py measures one GPU only: the solver runs on a single MI355X "
+              "(a sharded multi-GPU run is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     r = run_gpu(args.precision, args.steps, args.warmup)
