@@ -50,9 +50,12 @@ static void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
         }
         else if (!split && nsv)
             hipLaunchKernelGGL((stage2_kernel<D, N, false, true, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
-        else if (split)
-            hipLaunchKernelGGL((line_kernel<D, N, RT, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
-        else
+        else if (split) {
+            if constexpr (N == 7 && std::is_same<T, float>::value)
+                hipLaunchKernelGGL((line_kernel_w4<D, N, RT, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
+            else
+                hipLaunchKernelGGL((line_kernel<D, N, RT, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
+        } else
             hipLaunchKernelGGL((stage2_kernel<D, N, false, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
     } else {
         if (split)

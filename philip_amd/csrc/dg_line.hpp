@@ -630,6 +630,14 @@ line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
     line_block<D, N, RT, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
 }
+// The same held to 4 waves/SIMD: used for the mixed path at 3D p=6, where it measured faster
+// despite spilling (12^3 TGV: 129 -> 120 us per stage; p=3..5 unchanged).
+template <int D, int N, bool RT, typename T>
+__global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK)) __attribute__((amdgpu_waves_per_eu(4)))
+line_kernel_w4(StageArgs a, Ops1D ops, PhysConst pc)
+{
+    line_block<D, N, RT, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
+}
 
 // Navier-Stokes, mixed path.  line_kernel_ns_w4 is held to 4 waves/SIMD (128 VGPRs; a few
 // values spill): used where that measured faster, p=5 (142 -> 122 us per stage at 16^3); at
