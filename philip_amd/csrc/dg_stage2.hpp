@@ -87,8 +87,13 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
     __shared__ T sU[EPB][NS][NP];                       // state for pairs and faces
     __shared__ T sP[EPB][SPLIT ? D + 1 : 1][NP];          // velocity, pressure (split form)
     __shared__ T sF[EPB][NFLUX][NS][NP];                // volume fluxes for the D-divergence
-    __shared__ T sFace[EPB][D][NS][NP];                 // face terms per face direction
-    __shared__ T sPF[EPB][SPLIT ? NL * NPAIR : 1][NS];  // pair fluxes of one direction
+    // face terms per face direction (phase C) and, once they are summed into the node's
+    // registers, the pair fluxes of one direction (phase B) in the same storage
+    constexpr int PFN = SPLIT ? NL * NPAIR : 1;
+    constexpr int FPW = (D * NS * NP > PFN * NS) ? D * NS * NP : PFN * NS;
+    __shared__ T sFP[EPB * FPW];
+    T (*const sFace)[D][NS][NP] = reinterpret_cast<T (*)[D][NS][NP]>(sFP);
+    T (*const sPF)[PFN][NS] = reinterpret_cast<T (*)[PFN][NS]>(sFP);
 
     const int tid = threadIdx.x;
     const int blk0 = blockIdx.x * EPB;
@@ -220,15 +225,24 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
         else if (D == 3) item(std::integral_constant<int, (D == 3 ? 2 : 1)>{});
     }
 
-    // ---------------- phases B/D: volume ---------------------------------
+    // ---------------- surface terms into registers ----------------------
     int c[3] = {0, 0, 0};
     T acc[NS];
     for (int s = 0; s < NS; ++s) acc[s] = T(0);
-    if (active) node_coords<D, N>(i, c);
+    const T inv_wb = T(ops.inv_wb);
+    __syncthreads();  // phase C done
+    if (active) {
+        node_coords<D, N>(i, c);
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            for (int s = 0; s < NS; ++s) acc[s] += inv_wb * sFace[le][k][s][i];
+    }
+
+    // ---------------- phases B/D: volume ---------------------------------
     if (SPLIT) {
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            __syncthreads();  // previous direction's sPF consumed; phase C done
+            __syncthreads();  // face terms / previous direction's sPF consumed
             constexpr int PITEMS = NL * NPAIR;
             for (int t = tid; t < EPB * PITEMS; t += G::BLOCK) {
                 const int ple = t / PITEMS, r = t % PITEMS;
@@ -267,8 +281,6 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                 }
             }
         }
-    } else {
-        __syncthreads();  // phase C done
     }
     if (!active) return;
     if (!SPLIT || NSV) {
@@ -280,10 +292,6 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                 for (int s = 0; s < NS; ++s) acc[s] -= dk * sF[le][k][s][j];
             }
     }
-    const T inv_wb = T(ops.inv_wb);
-#pragma unroll
-    for (int k = 0; k < D; ++k)
-        for (int s = 0; s < NS; ++s) acc[s] += inv_wb * sFace[le][k][s][i];
 
     // ---------------- phase E: FP64 epilogue -----------------------------
     stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc);
