@@ -103,6 +103,9 @@ def _load():
     lib.dgo_adaptive_time_step.argtypes = [vp, dp, d]
     lib.dgo_run.argtypes = [vp, i, d, d, dp, dp, i]
     lib.dgo_set_threads.argtypes = [i]
+    lib.dgo_set_mutation.argtypes = [i]
+    lib.dgo_set_fsal.argtypes = [vp, i]
+    lib.dgo_residual_magnitude.argtypes = [vp, dp, dp]
     return lib
 
 
@@ -286,6 +289,18 @@ class Oracle:
         lib().dgo_assemble_residual(self.h, _dp(np.ascontiguousarray(u)), _dp(r))
         return r
 
+    def residual_magnitude(self, u):
+        """Per-DoF rounding scale of the residual: the sum of the absolute values of the terms
+        summed into each DoF (GLL nodes)."""
+        m = self.vec()
+        lib().dgo_residual_magnitude(self.h, _dp(np.ascontiguousarray(u, dtype=np.float64)), _dp(m))
+        return m
+
+    def set_fsal(self, on):
+        """3S*+ methods: evaluate the FSAL residual of the embedded error estimate (default on,
+        as the reference); it does not change the solution."""
+        lib().dgo_set_fsal(self.h, int(on))
+
     def inverse_mass(self, r):
         o = self.vec()
         lib().dgo_apply_inverse_mass(self.h, _dp(np.ascontiguousarray(r)), _dp(o), 0)
@@ -337,3 +352,11 @@ class Oracle:
 
 def set_threads(n):
     lib().dgo_set_threads(int(n))
+
+
+MUT_NONE, MUT_LXF_MIN, MUT_LXF_NO_HALF, MUT_ROE_NO_HALF, MUT_ROE_NO_HARTEN, MUT_ROE_NO_SHEAR = range(6)
+
+
+def set_mutation(m):
+    """Plant one of the DGO_MUT_* mistakes in the oracle (tests of its checks only)."""
+    lib().dgo_set_mutation(int(m))

@@ -63,10 +63,16 @@ struct dgo_ctx {
     double *aux_rhs[3];
     /* physics constants (src/physics/euler.cpp:13-67, navier_stokes.cpp:16-55) */
     double gam, gamm1, mach2, pinf, temperature_ratio;
+    int fsal;                     /* 3S*+ methods: evaluate the FSAL residual (reference: always) */
 };
 
 static int g_threads = 1;
 void dgo_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
+/* Planted mistakes for the tests that check the oracle's own checks (DGO_MUT_*): a test
+   switches one on, shows that the identity it relies on then fails, and switches it off. */
+static int g_mutation = DGO_MUT_NONE;
+void dgo_set_mutation(int m) { g_mutation = m; }
 
 /* ------------------------------------------------------------------------ */
 /* 1D quadrature (deal.II QGauss / QGaussLobatto on [0,1])                  */
@@ -490,6 +496,7 @@ dgo_ctx *dgo_create(const dgo_params *prm)
     c->detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
     c->cof = (c->dim == 3) ? c->h * c->h : c->h;
     c->one1[0] = 1.0;
+    c->fsal = 1;
 
     c->gam = prm->gamma_gas;
     c->gamm1 = c->gam - 1.0;
@@ -832,7 +839,7 @@ static void roe_dissipation(const dgo_ctx *c, const double *ui, const double *ue
     const double dVn = vnR - vnL;
     double dVt[3];
     for (int k = 0; k < d; ++k) dVt[k] = (vR[k] - vL[k]) - dVn * n[k];
-    for (int e = 0; e < 3; ++e) {
+    for (int e = 0; e < 3 && g_mutation != DGO_MUT_ROE_NO_HARTEN; ++e) {
         const double a1 = fabs(eig[e] - eL[e]), a2 = fabs(eR[e] - eig[e]);
         const double eps = a1 > a2 ? a1 : a2;
         if (eig[e] < eps) eig[e] = 0.5 * (eig[e] * eig[e] / eps + eps);
@@ -840,7 +847,7 @@ static void roe_dissipation(const dgo_ctx *c, const double *ui, const double *ue
     double co[4];
     co[0] = eig[0] * (dp - rho_ravg * c_ravg * dVn) / (2.0 * c2);
     co[1] = eig[1] * (drho - dp / c2);
-    co[2] = eig[1] * rho_ravg;
+    co[2] = (g_mutation == DGO_MUT_ROE_NO_SHEAR) ? 0.0 : eig[1] * rho_ravg;
     co[3] = eig[2] * (dp + rho_ravg * c_ravg * dVn) / (2.0 * c2);
     double A[NS_MAX];
     A[0] = co[0] * 1.0;
@@ -856,7 +863,8 @@ static void roe_dissipation(const dgo_ctx *c, const double *ui, const double *ue
     A[0] += co[3] * 1.0;
     for (int k = 0; k < d; ++k) A[1 + k] += co[3] * (v_ravg[k] + c_ravg * n[k]);
     A[ns - 1] += co[3] * (h_ravg + c_ravg * vn_ravg);
-    for (int s = 0; s < ns; ++s) out[s] = -0.5 * A[s];
+    const double half = (g_mutation == DGO_MUT_ROE_NO_HALF) ? 1.0 : 0.5;
+    for (int s = 0; s < ns; ++s) out[s] = -half * A[s];
 }
 
 void dgo_numerical_flux(const dgo_ctx *c, int which, const double *ui, const double *ue,
@@ -883,9 +891,12 @@ void dgo_numerical_flux(const dgo_ctx *c, int which, const double *ui, const dou
         }
     }
     if (which == DGO_LXF || which == DGO_TWO_POINT_LXF) {
+        /* LaxFriedrichsRiemannSolverDissipation (convective_numerical_flux.cpp:188-213) */
         const double ei = max_normal_eig(c, ui, n), ee = max_normal_eig(c, ue, n);
-        const double lam = (ei > ee) ? ei : ee;
-        for (int s = 0; s < ns; ++s) diss[s] = -0.5 * lam * (ue[s] - ui[s]);
+        double lam = (ei > ee) ? ei : ee;
+        if (g_mutation == DGO_MUT_LXF_MIN) lam = (ei > ee) ? ee : ei;
+        const double half = (g_mutation == DGO_MUT_LXF_NO_HALF) ? 1.0 : 0.5;
+        for (int s = 0; s < ns; ++s) diss[s] = -half * lam * (ue[s] - ui[s]);
     } else if (which == DGO_ROE || which == DGO_TWO_POINT_ROE) {
         roe_dissipation(c, ui, ue, n, diss);
     } else {
@@ -1495,6 +1506,94 @@ void dgo_assemble_residual(dgo_ctx *c, const double *u, double *rhs)
     }
 }
 
+void dgo_set_fsal(dgo_ctx *c, int on) { c->fsal = on != 0; }
+
+/* Rounding scale of the residual, per DoF (test infrastructure, collocated GLL nodes): the sum
+   of the absolute values of the terms the residual at that DoF is summed from -- two-point
+   volume fluxes times the skew operator (split form) or flux values times the derivative
+   operator (conservative form and the viscous flux), and at face nodes the numerical fluxes and
+   the interior fluxes of the strong form -- each with the node's quadrature weight and metric
+   factor W_i h^(d-1) (surface terms also 1/w_0).  A residual computed in another summation
+   order differs from the oracle's by a few units of FP64 rounding of this scale. */
+void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
+{
+    const long nd = dgo_n_dofs(c);
+    const int d = c->dim, ns = c->ns, Np = c->Np, n = c->n;
+    const int split = c->P.use_split_form, visc = (c->P.pde == DGO_NAVIER_STOKES);
+    memset(mag, 0, sizeof(double) * nd);
+    if (c->P.flux_nodes != DGO_GLL) return;
+    if (visc) assemble_auxiliary(c, u);
+    double D[MAXN][MAXN], Sk[MAXN][MAXN];
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            D[a][b] = c->fb_grad[a * n + b];
+            Sk[a][b] = c->skew[a * n + b] / c->wq[a];
+        }
+    const double hd1 = (d == 3) ? c->h * c->h : c->h;
+    const double penalty = penalty_scaling(c);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads)
+#endif
+    for (int e = 0; e < c->ncells; ++e) {
+        for (int i = 0; i < Np; ++i) {
+            int ci[3] = {i % n, (i / n) % n, (d == 3) ? i / (n * n) : 0};
+            double ui[NS_MAX], gi[NS_MAX * 3], acc[NS_MAX];
+            for (int s = 0; s < ns; ++s) {
+                ui[s] = u[IDX(c, e, s, i)];
+                acc[s] = 0.0;
+                for (int k = 0; k < d; ++k) gi[s * d + k] = visc ? c->aux[k][IDX(c, e, s, i)] : 0.0;
+            }
+            for (int k = 0; k < d; ++k) {
+                for (int b = 0; b < n; ++b) {
+                    int cj[3] = {ci[0], ci[1], ci[2]};
+                    cj[k] = b;
+                    const int j = cj[0] + n * (cj[1] + n * cj[2]);
+                    double uj[NS_MAX], gj[NS_MAX * 3], F[NS_MAX * 3];
+                    for (int s = 0; s < ns; ++s) {
+                        uj[s] = u[IDX(c, e, s, j)];
+                        for (int kk = 0; kk < d; ++kk) gj[s * d + kk] = visc ? c->aux[kk][IDX(c, e, s, j)] : 0.0;
+                    }
+                    if (split) {
+                        if (b != ci[k]) {
+                            dgo_two_point_flux(c, c->P.two_point_flux, ui, uj, F);
+                            for (int s = 0; s < ns; ++s) acc[s] += fabs(Sk[ci[k]][b] * F[s * d + k]);
+                        }
+                    } else {
+                        dgo_convective_flux(c, uj, F);
+                        for (int s = 0; s < ns; ++s) acc[s] += fabs(D[ci[k]][b] * F[s * d + k]);
+                    }
+                    if (visc) {
+                        dgo_dissipative_flux(c, uj, gj, F);
+                        for (int s = 0; s < ns; ++s) acc[s] += fabs(D[ci[k]][b] * F[s * d + k]);
+                    }
+                }
+                if (ci[k] != 0 && ci[k] != n - 1) continue;
+                /* face node: numerical fluxes and the interior fluxes of the strong form */
+                const int side = (ci[k] == n - 1), face = 2 * k + side;
+                const int nb = c->nbr[e * 2 * d + face];
+                int cj[3] = {ci[0], ci[1], ci[2]};
+                cj[k] = side ? 0 : n - 1;
+                const int j = cj[0] + n * (cj[1] + n * cj[2]);
+                double ue[NS_MAX], ge[NS_MAX * 3], nrm[3] = {0, 0, 0}, fl[NS_MAX], F[NS_MAX * 3];
+                for (int s = 0; s < ns; ++s) {
+                    ue[s] = u[IDX(c, nb, s, j)];
+                    for (int kk = 0; kk < d; ++kk) ge[s * d + kk] = visc ? c->aux[kk][IDX(c, nb, s, j)] : 0.0;
+                }
+                nrm[k] = side ? 1.0 : -1.0;
+                dgo_numerical_flux(c, c->P.conv_flux, ui, ue, nrm, fl);
+                dgo_convective_flux(c, ui, F);
+                for (int s = 0; s < ns; ++s) acc[s] += (fabs(fl[s]) + fabs(F[s * d + k])) / c->wq[0];
+                if (visc) {
+                    viscous_num_flux(c, ui, ue, gi, ge, nrm, penalty, fl);
+                    dgo_dissipative_flux(c, ui, gi, F);
+                    for (int s = 0; s < ns; ++s) acc[s] += (fabs(fl[s]) + fabs(F[s * d + k])) / c->wq[0];
+                }
+            }
+            for (int s = 0; s < ns; ++s) mag[IDX(c, e, s, i)] = c->wvol[i] * hd1 * acc[s];
+        }
+    }
+}
+
 void dgo_apply_inverse_mass(const dgo_ctx *c, const double *in, double *out, int aux)
 {
     /* apply_inverse_global_mass_matrix, Cartesian branch (dg_base.cpp:4208-4219); cDG/kDG => M^-1 */
@@ -1759,7 +1858,7 @@ void dgo_step(dgo_ctx *c, int m, double dt, double *u)
             }
             memcpy(r, S1, sizeof(double) * nd);
         }
-        if (L.plus) {
+        if (L.plus && c->fsal) {
             /* FSAL residual for the embedded estimate (:63-69); it only updates S4 */
             memcpy(u, r, sizeof(double) * nd);
             rhs_lifted(c, u, r, scratch);

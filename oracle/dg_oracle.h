@@ -45,6 +45,13 @@ enum {
 enum { DGO_EULER = 0, DGO_NAVIER_STOKES = 1 };
 enum { DGO_SIPG = 0, DGO_BR2 = 1 };
 enum { DGO_ORDER_DEALII = 0, DGO_ORDER_LEXICOGRAPHIC = 1 };
+/* planted mistakes (tests of the tests only): LxF wave speed min instead of max, LxF or Roe
+   dissipation without its factor 1/2, Roe without Harten's entropy fix, Roe without the shear
+   (tangential velocity jump) wave */
+enum {
+    DGO_MUT_NONE = 0, DGO_MUT_LXF_MIN = 1, DGO_MUT_LXF_NO_HALF = 2, DGO_MUT_ROE_NO_HALF = 3,
+    DGO_MUT_ROE_NO_HARTEN = 4, DGO_MUT_ROE_NO_SHEAR = 5
+};
 
 typedef struct {
     int dim;                 /* 2 or 3 */
@@ -154,6 +161,13 @@ int dgo_run(dgo_ctx *c, int method, double dt_const, double final_time, double *
             double *ke_hist, int max_hist);
 
 void dgo_set_threads(int n);
+/* planted mistake for the tests of the oracle's checks (DGO_MUT_*; DGO_MUT_NONE restores) */
+void dgo_set_mutation(int m);
+/* 3S*+ methods: whether dgo_step evaluates the FSAL residual of the embedded estimate
+   (default 1, as the reference); it does not change the solution */
+void dgo_set_fsal(dgo_ctx *c, int on);
+/* per-DoF rounding scale of the residual: sum of |terms| summed into each DoF (GLL only) */
+void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag);
 
 #ifdef __cplusplus
 }
