@@ -156,11 +156,36 @@ int dgx_numerical_entropy(dgx_ctx *ctx, double *out);
    max of |u| + c over the QGauss(p+1+overint) points of every cell. */
 int dgx_max_wave_speed(dgx_ctx *ctx, int overint, double *out);
 /* PeriodicTurbulence::get_adaptive_time_step (periodic_turbulence.cpp:359-367):
-   dt = cfl * L / (n_dofs/nstate)^(1/dim) / max wave speed (overint 10, as the reference). */
+   dt = cfl * L / (n_dofs/nstate)^(1/dim) / max wave speed (overint 10, as the reference).
+   DGX_ERR_NONPHYSICAL if the maximum wave speed is not a positive finite number. */
 int dgx_adaptive_time_step(dgx_ctx *ctx, double cfl, double *dt);
 
 /* Blocks until the context's stream has finished all launched work. */
 int dgx_synchronize(dgx_ctx *ctx);
+
+/* Non-physical states (Parameters::AllParameters::non_physical_behavior, all_parameters.h:304,
+   all_parameters.cpp:351-356; PhysicsBase::handle_non_physical_result, physics.cpp:298-317;
+   Euler::check_positive_quantity, euler.cpp:155-173).  The stage kernels flag every stage state
+   with a negative or NaN density or pressure.  The flag is read (and cleared) by
+   dgx_assemble_residual, dgx_step_in_time (once per call, after its steps), dgx_batch_synchronize,
+   a served request's retirement and dgx_check_state, and handled by the context's policy:
+     DGX_RETURN_BIG_NUMBER  (reference default) counted in dgx_nonphysical_events, status DGX_OK;
+                            the device does not substitute 1e100 for the offending pressure, so the
+                            affected residual entries are NaN where the reference's are ~1e100;
+     DGX_ABORT_RUN          the call returns DGX_ERR_NONPHYSICAL (the reference std::abort()s; the
+                            caller decides); dgx_assemble_residual then fills right_hand_side with
+                            ones, the reference's invalid-assembly fallback (dg_base.cpp:2850-2875);
+     DGX_PRINT_WARNING      a warning on stderr, status DGX_OK.
+   A served request that met a non-physical state makes dgx_server_result return
+   DGX_ERR_NONPHYSICAL (its outputs are still filled). */
+enum dgx_non_physical_behavior { DGX_RETURN_BIG_NUMBER = 0, DGX_ABORT_RUN = 1, DGX_PRINT_WARNING = 2 };
+int dgx_set_non_physical_behavior(dgx_ctx *ctx, int policy);
+/* Waits for the context stream, reads and clears the flag, applies the policy (for the
+   device-pointer entry points, which do not wait). */
+int dgx_check_state(dgx_ctx *ctx);
+/* Number of checks so far that found a non-physical state (all policies), plus adaptive time
+   steps refused for a non-positive or non-finite maximum wave speed. */
+int64_t dgx_nonphysical_events(const dgx_ctx *ctx);
 /* HIP stream the context launches on (as an opaque pointer); 0 = default stream. */
 int dgx_set_stream(dgx_ctx *ctx, void *hip_stream);
 /* Average duration (ms) of the last `dgx_step_in_time` stage kernels, measured with

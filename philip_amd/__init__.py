@@ -16,6 +16,8 @@ LXF, ROE, CENTRAL, TWO_POINT, TWO_POINT_LXF, TWO_POINT_ROE = 0, 1, 2, 3, 4, 5
 EULER, NAVIER_STOKES = 0, 1
 SIPG, BR2 = 0, 1
 FP64, MIXED = 0, 1
+RETURN_BIG_NUMBER, ABORT_RUN, PRINT_WARNING = 0, 1, 2   # non_physical_behavior (all_parameters.h:304)
+ERR_NONPHYSICAL = -3
 METHODS = {"euler": 0, "heun": 1, "ssprk3": 2, "rk4": 3,
            "rk4_3_5_3sstar": 10, "rk3_2_5f_3sstarplus": 11,
            "rk4_3_9f_3sstarplus": 12, "rk5_4_10f_3sstarplus": 13}
@@ -29,7 +31,8 @@ EXPORTED = [
     "dgx_set_track_error_estimate", "dgx_constant_time_step", "dgx_integrated_quantities",
     "dgx_numerical_entropy", "dgx_max_wave_speed", "dgx_adaptive_time_step", "dgx_set_stream",
     "dgx_enable_timing", "dgx_kernel_timing", "dgx_set_initial_condition_tgv",
-    "dgx_set_initial_condition_vortex", "dgx_synchronize",
+    "dgx_set_initial_condition_vortex", "dgx_synchronize", "dgx_set_non_physical_behavior", "dgx_check_state",
+    "dgx_nonphysical_events",
     "dgx_batch_create", "dgx_batch_destroy", "dgx_batch_last_error", "dgx_batch_size", "dgx_batch_instance",
     "dgx_batch_allocate", "dgx_batch_step_in_time", "dgx_batch_synchronize", "dgx_batch_variant",
     "dgx_request_bytes", "dgx_request_n_dofs", "dgx_server_create", "dgx_server_destroy", "dgx_server_last_error",
@@ -51,7 +54,9 @@ class Params(ctypes.Structure):
 
 
 class DeviceError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 _lib = None
@@ -124,6 +129,10 @@ def load():
     lib.dgx_enable_timing.argtypes = [vp, i]
     lib.dgx_kernel_timing.argtypes = [vp, dp, P(i64)]
     lib.dgx_synchronize.argtypes = [vp]
+    lib.dgx_set_non_physical_behavior.argtypes = [vp, i]
+    lib.dgx_check_state.argtypes = [vp]
+    lib.dgx_nonphysical_events.restype = i64
+    lib.dgx_nonphysical_events.argtypes = [vp]
     lib.dgx_apply_inverse_mass_device.argtypes = [vp, vp, vp]
     lib.dgx_batch_create.argtypes = [P(Params), i, P(vp)]
     lib.dgx_batch_destroy.argtypes = [vp]
@@ -188,7 +197,25 @@ class DG:
 
     def _chk(self, rc, what):
         if rc != 0:
-            raise DeviceError(f"{what}: code {rc}: {load().dgx_last_error(self.h).decode()}")
+            raise DeviceError(f"{what}: code {rc}: {load().dgx_last_error(self.h).decode()}", rc)
+
+    def set_non_physical_behavior(self, policy):
+        """RETURN_BIG_NUMBER (reference default), ABORT_RUN (calls return ERR_NONPHYSICAL, raised
+        here as DeviceError with .code == ERR_NONPHYSICAL) or PRINT_WARNING."""
+        self._chk(load().dgx_set_non_physical_behavior(self.h, policy), "set_non_physical_behavior")
+
+    def check_state(self):
+        """Waits for the device, reads and clears the non-physical flag, applies the policy."""
+        self._chk(load().dgx_check_state(self.h), "check_state")
+
+    @property
+    def nonphysical_events(self):
+        return load().dgx_nonphysical_events(self.h)
+
+    def right_hand_side(self):
+        r = np.empty(self.ndofs)
+        self._chk(load().dgx_get_right_hand_side(self.h, _dp(r)), "get_right_hand_side")
+        return r
 
     def close(self):
         if getattr(self, "h", None):
@@ -441,6 +468,7 @@ class Server:
         self.h = h
         self.method = m
         self.ndofs = {}
+        self.nonphysical = set()   # finished requests that met a non-physical state
 
     def _chk(self, rc, what):
         if rc != 0:
@@ -463,7 +491,9 @@ class Server:
         lat, dofu = ctypes.c_double(), ctypes.c_double()
         st = ctypes.c_int64()
         rc = load().dgx_server_result(self.h, rid, _dp(u), ctypes.byref(lat), ctypes.byref(st), ctypes.byref(dofu))
-        if rc != 0:
+        if rc == ERR_NONPHYSICAL:
+            self.nonphysical.add(rid)
+        elif rc != 0:
             return None
         return u, lat.value, st.value, dofu.value
 

@@ -31,7 +31,7 @@ static void legendre_pd(int n, double x, double *pn, double *dpn)
 void gauss_lobatto(int n, double *x, double *w)
 {
     const int N = n - 1;
-    std::vector<double> t(n);
+    double t[64];
     for (int i = 0; i < n; ++i) {
         double z = std::cos(M_PI * i / N);
         if (i > 0 && i < N) {
@@ -277,7 +277,8 @@ int dgx_create(const dgx_params *p, dgx_ctx **out)
     neighbors(c->dim, N, ijk, c->nbr_host);
     const size_t bytes = sizeof(double) * (size_t)c->ndofs;
     if (hipMalloc(&c->nbr_dev, sizeof(int) * c->nbr_host.size()) != hipSuccess ||
-        hipMalloc(&c->u_dev, bytes) != hipSuccess || hipMalloc(&c->r_dev, bytes) != hipSuccess)
+        hipMalloc(&c->u_dev, bytes) != hipSuccess || hipMalloc(&c->r_dev, bytes) != hipSuccess ||
+        hipMalloc(&c->status_dev, 256) != hipSuccess || hipMemset(c->status_dev, 0, 256) != hipSuccess)
         return bad(DGX_ERR_HIP, "hipMalloc failed");
     if (p->pde == DGX_NAVIER_STOKES)
         for (int k = 0; k < c->dim; ++k) {
@@ -302,6 +303,7 @@ int dgx_destroy(dgx_ctx *c)
     (void)hipFree(c->u_dev);
     (void)hipFree(c->r_dev);
     (void)hipFree(c->minv_dev);
+    (void)hipFree(c->status_dev);
     for (int k = 0; k < 3; ++k) {
         (void)hipFree(c->aux_dev[k]);
         (void)hipFree(c->auxf_dev[k]);
@@ -387,7 +389,29 @@ static StageArgs base_args(dgx_ctx *c, const double *u)
     const double pen = (double)(deg * (deg + 1)) / c->h * c->P.sipg_penalty_factor; // dg_base.cpp:397-413
     a.penalty = 0.5 * (pen + pen);
     a.resid_scale = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
+    a.status = c->status_dev;
     return a;
+}
+
+// Reads and clears the non-physical flag the stage kernels set since the last check (waits for
+// the context stream) and applies the context's policy, as PhysicsBase::handle_non_physical_result
+// (physics.cpp:298-317): abort_run -> DGX_ERR_NONPHYSICAL (the caller aborts), print_warning ->
+// a warning on stderr, return_big_number -> nothing.  *found tells whether the flag was set.
+static int check_nonphysical(dgx_ctx *c, const char *where, bool *found = nullptr)
+{
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, c->status_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (found) *found = flag != 0;
+    if (!flag) return DGX_OK;
+    HIPCHK(hipMemsetAsync(c->status_dev, 0, sizeof(int), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ++c->nonphys_events;
+    const std::string msg = std::string("Non-physical result has been detected (negative or NaN density or "
+                                        "pressure) in ") + where;
+    if (c->nonphys_policy == DGX_ABORT_RUN) return fail(c, DGX_ERR_NONPHYSICAL, msg);
+    if (c->nonphys_policy == DGX_PRINT_WARNING) std::fprintf(stderr, "WARNING: %s\n", msg.c_str());
+    return DGX_OK;
 }
 
 // right-hand side of the stage solution u (aux solve first for Navier-Stokes)
@@ -408,8 +432,14 @@ int dgx_assemble_residual(dgx_ctx *c)
     if (!c) return DGX_ERR_INVALID;
     int rc = residual_into(c, c->u_dev, c->r_dev, MODE_RESIDUAL);
     if (rc) return rc;
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return DGX_OK;
+    bool found = false;
+    rc = check_nonphysical(c, "assemble_residual", &found);
+    if (rc == DGX_ERR_NONPHYSICAL) {
+        // invalid assembly: right-hand side filled with ones (dg_base.cpp:2850-2875)
+        if (launch_fill(c, c->r_dev, c->ndofs, 1.0) != DGX_OK) return DGX_ERR_HIP;
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return rc;
 }
 
 int dgx_assemble_residual_device(dgx_ctx *c, const double *u_dev, double *r_dev)
@@ -709,7 +739,7 @@ int dgx_step_in_time(dgx_ctx *c, double dt, int n_steps)
         ++c->iteration;
         c->current_time += dt;
     }
-    return DGX_OK;
+    return check_nonphysical(c, "step_in_time");
 }
 
 double dgx_current_time(const dgx_ctx *c) { return c ? c->current_time : 0.0; }
@@ -821,6 +851,10 @@ int dgx_adaptive_time_step(dgx_ctx *c, double cfl, double *dt)
     double vmax = 0.0;
     int rc = launch_wave_speed(c, c->u_dev, 10, &vmax);
     if (rc) return rc;
+    if (!(vmax > 0.0) || !std::isfinite(vmax)) {
+        ++c->nonphys_events;
+        return fail(c, DGX_ERR_NONPHYSICAL, "adaptive time step: maximum wave speed is not a positive finite number");
+    }
     const unsigned ndofs_per_state = (unsigned)(c->ndofs / c->ns);
     const double dx = (c->P.domain_right - c->P.domain_left) / std::pow((double)ndofs_per_state, (1.0 / c->dim));
     *dt = cfl * dx / vmax;
@@ -833,6 +867,21 @@ int dgx_synchronize(dgx_ctx *c)
     HIPCHK(hipStreamSynchronize(c->stream));
     return DGX_OK;
 }
+
+int dgx_set_non_physical_behavior(dgx_ctx *c, int policy)
+{
+    if (!c || policy < DGX_RETURN_BIG_NUMBER || policy > DGX_PRINT_WARNING) return DGX_ERR_INVALID;
+    c->nonphys_policy = policy;
+    return DGX_OK;
+}
+
+int dgx_check_state(dgx_ctx *c)
+{
+    if (!c) return DGX_ERR_INVALID;
+    return check_nonphysical(c, "stage evaluation");
+}
+
+int64_t dgx_nonphysical_events(const dgx_ctx *c) { return c ? c->nonphys_events : 0; }
 
 int dgx_set_stream(dgx_ctx *c, void *s)
 {
@@ -1171,7 +1220,15 @@ int dgx_batch_synchronize(dgx_batch *b)
 {
     if (!b) return DGX_ERR_INVALID;
     BHIPCHK(hipStreamSynchronize(b->stream));
-    return DGX_OK;
+    int rc = DGX_OK;
+    for (dgx_ctx *c : b->inst) {  // every instance's non-physical flag, under its own policy
+        const int r = check_nonphysical(c, "batched step");
+        if (r && !rc) {
+            rc = r;
+            b->err = c->err;
+        }
+    }
+    return rc;
 }
 
 } // extern "C"
@@ -1197,6 +1254,7 @@ struct Result {
     std::vector<double> u;
     double latency = 0.0, dof_updates = 0.0;
     int64_t steps = 0;
+    bool nonphysical = false;   // a stage of this request met a non-physical state
 };
 
 int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
@@ -1405,6 +1463,12 @@ static int server_retire(dgx_server *s)
         SHIPCHK(hipMemcpyAsync(res.u.data(), q.c->u_dev, sizeof(double) * q.c->ndofs, hipMemcpyDeviceToHost,
                                s->stream));
         SHIPCHK(hipStreamSynchronize(s->stream));
+        bool found = false;
+        if (check_nonphysical(q.c, "served request", &found) == DGX_ERR_HIP) {
+            s->err = q.c->err;
+            return DGX_ERR_HIP;
+        }
+        res.nonphysical = found;
         res.latency = std::chrono::duration<double>(Clock::now() - q.submitted).count();
         res.steps = q.steps;
         res.dof_updates = (double)q.c->ndofs * s->B.s * q.steps;
@@ -1508,6 +1572,10 @@ int dgx_server_result(dgx_server *s, int64_t id, double *host_u, double *latency
     if (latency_s) *latency_s = it->second.latency;
     if (steps) *steps = it->second.steps;
     if (dof_updates) *dof_updates = it->second.dof_updates;
+    if (it->second.nonphysical) {
+        s->err = "request met a non-physical state (negative or NaN density or pressure)";
+        return DGX_ERR_NONPHYSICAL;
+    }
     return DGX_OK;
 }
 

@@ -54,6 +54,9 @@ struct dgx_ctx {
     double *aux_dev[3] = {nullptr, nullptr, nullptr};   // lifted gradient, FP64 path
     float *auxf_dev[3] = {nullptr, nullptr, nullptr};   // lifted gradient, mixed path
     double *minv_dev = nullptr;    // lifted right-hand side (dgx_apply_inverse_mass_to_rhs)
+    int *status_dev = nullptr;     // non-physical state flag written by the stage kernels
+    int nonphys_policy = 0;        // DGX_RETURN_BIG_NUMBER / DGX_ABORT_RUN / DGX_PRINT_WARNING
+    int64_t nonphys_events = 0;    // checks that found a non-physical state
     // ODE system
     int method = -1;
     bool track_error = false;
@@ -78,6 +81,7 @@ struct dgx_ctx {
 namespace dgx {
 int launch_stage(dgx_ctx *c, const StageArgs &a);
 int launch_inverse_mass(dgx_ctx *c, const double *in, double *out);
+int launch_fill(dgx_ctx *c, double *out, int64_t n, double value);
 // Batched split-form line kernel over the element groups `blocks` of several solves of one
 // (dim, n, precision); es = elements-per-block variant (1 or 2).  With blocks == nullptr only
 // reports the variant's elements per block in *epb.  DGX_ERR_UNSUPPORTED if the variant does

@@ -59,7 +59,19 @@ struct StageArgs {
     double a_next[4];         // a[i+1][j], j <= i
     double b_dt;              // dt * b_i
     int first_stage, last_stage, stage;
+    // set to 1 by any thread that finds a non-physical stage state (density or pressure
+    // negative or NaN; euler.cpp:155-173, 486-500); read and cleared by the host
+    int *status;
 };
+
+// Non-physical state check of one node (the reference checks every density and pressure it
+// computes, euler.cpp:155-173, 486-500; NaN also fails here).  A plain global store of the
+// same value from every offending lane: no atomics needed.
+template <typename T>
+__device__ __forceinline__ void flag_nonphysical(const StageArgs &a, T rho, T p)
+{
+    if (!(rho >= T(0)) || !(p >= T(0))) *a.status = 1;
+}
 
 // Lifted gradient component (state s, direction d) at flat index idx of the
 // layout [cell][state][node], in the precision of the path's fluxes.

@@ -1,6 +1,6 @@
-// Template dispatch of the DG kernels over (dim, nodes per direction, scheme),
-// launch bookkeeping (HIP-event timing of the stage kernels), and the
-// integrated-quantity reduction (kinetic energy, enstrophy).
+// Dispatch of the DG kernels over (dim, nodes per direction) to their per-N instantiations
+// (dg_inst_n<N>.hip), launch bookkeeping, the inverse mass, and the integrated-quantity
+// reductions (kinetic energy, enstrophy, numerical entropy, maximum wave speed).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -9,109 +9,16 @@
 #include <string>
 #include <vector>
 
-#include "dg_host.hpp"
-#include "dg_line.hpp"
-#include "dg_stage2.hpp"
+#include "dg_dispatch.hpp"
 
 namespace dgx {
 
-static int check_launch(dgx_ctx *c, const char *what)
-{
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        c->err = std::string(what) + ": " + hipGetErrorString(e);
-        return DGX_ERR_HIP;
-    }
-    return DGX_OK;
-}
-
-template <int D, int N, typename T, bool RT>
-static void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
-{
-    const bool split = c->P.use_split_form != 0;
-    const bool nsv = c->P.pde == DGX_NAVIER_STOKES;
-    using LG = LineGeo<D, N, T>;
-    const dim3 lgrid((c->ncells + LG::EPB - 1) / LG::EPB), lblock(LG::BLOCK);
-    if constexpr (D == 3) {
-        if (split && nsv) {
-            // mixed: line kernel with the viscous terms; FP64: stage2_kernel (the FP64 line
-            // variant needs ~85 KB of LDS per block at p=5 and ran slower at one block per CU)
-            if constexpr (std::is_same<T, float>::value) {
-                using LV = LineGeoK<D, N, T, true>;
-                const dim3 vgrid((c->ncells + LV::EPB - 1) / LV::EPB), vblock(LV::BLOCK);
-                if constexpr (N == 6)
-                    hipLaunchKernelGGL((line_kernel_ns_w4<D, N>), vgrid, vblock, 0, c->stream, a, c->ops, c->pc);
-                else
-                    hipLaunchKernelGGL((line_kernel_ns<D, N>), vgrid, vblock, 0, c->stream, a, c->ops, c->pc);
-            } else {
-                hipLaunchKernelGGL((stage2_kernel<D, N, true, true, RT, T>), grid, block, 0, c->stream, a, c->ops,
-                                   c->pc);
-            }
-        }
-        else if (!split && nsv)
-            hipLaunchKernelGGL((stage2_kernel<D, N, false, true, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
-        else if (split) {
-            if constexpr (N == 7 && std::is_same<T, float>::value)
-                hipLaunchKernelGGL((line_kernel_w4<D, N, RT, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
-            else
-                hipLaunchKernelGGL((line_kernel<D, N, RT, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
-        } else
-            hipLaunchKernelGGL((stage2_kernel<D, N, false, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
-    } else {
-        if (split)
-            hipLaunchKernelGGL((line_kernel<D, N, RT, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
-        else
-            hipLaunchKernelGGL((stage2_kernel<D, N, false, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
-    }
-}
-
-template <int D, int N>
-static void stage_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
-{
-    // FP64: fluxes in double with the reference's entropy-projection round trip;
-    // mixed: fluxes and contractions in float, state and RK registers in double
-    if (c->P.precision == DGX_MIXED) stage2_dn<D, N, float, false>(c, a, grid, block);
-    else stage2_dn<D, N, double, true>(c, a, grid, block);
-}
-
-template <int D, int N>
-static int launch_stage_dn(dgx_ctx *c, const StageArgs &a)
-{
-    using S = Geo<D, N>;
-    const int blocks = (c->ncells + S::EPB - 1) / S::EPB;
-    if (c->timing.enabled && hipEventRecord(c->timing.a, c->stream) != hipSuccess) {
-        c->err = "hipEventRecord failed";
-        return DGX_ERR_HIP;
-    }
-    stage_dn<D, N>(c, a, dim3(blocks), dim3(S::BLOCK));
-    int rc = check_launch(c, "stage_kernel");
-    if (rc) return rc;
-    if (c->timing.enabled) {
-        float ms = 0.f;
-        if (hipEventRecord(c->timing.b, c->stream) != hipSuccess || hipEventSynchronize(c->timing.b) != hipSuccess ||
-            hipEventElapsedTime(&ms, c->timing.a, c->timing.b) != hipSuccess) {
-            c->err = "stage kernel timing failed";
-            return DGX_ERR_HIP;
-        }
-        c->timing.sum_ms += ms;
-        c->timing.n += 1;
-    }
-    return DGX_OK;
-}
-
-template <int D, int N>
-static int launch_aux_dn(dgx_ctx *c, const double *u)
-{
-    using S = Shape<D, N>;
-    const int blocks = (c->ncells + S::EPB - 1) / S::EPB;
-    if (c->P.precision == DGX_MIXED)
-        hipLaunchKernelGGL((aux_kernel<D, N, float>), dim3(blocks), dim3(S::BLOCK), 0, c->stream, u, c->auxf_dev[0],
-                           c->auxf_dev[1], c->auxf_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops);
-    else
-        hipLaunchKernelGGL((aux_kernel<D, N, double>), dim3(blocks), dim3(S::BLOCK), 0, c->stream, u, c->aux_dev[0],
-                           c->aux_dev[1], c->aux_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops);
-    return check_launch(c, "aux_kernel");
-}
+DGX_KERNELS_N(extern, 2)
+DGX_KERNELS_N(extern, 3)
+DGX_KERNELS_N(extern, 4)
+DGX_KERNELS_N(extern, 5)
+DGX_KERNELS_N(extern, 6)
+DGX_KERNELS_N(extern, 7)
 
 #define DISPATCH_N(FN, ...)                                                     \
     switch (c->n) {                                                             \
@@ -130,38 +37,6 @@ int launch_stage(dgx_ctx *c, const StageArgs &a) { DISPATCH_N(launch_stage_dn, c
 // Batched line kernel: one launch over the element groups of many solves of one
 // (dim, p, precision); es selects the elements-per-block variant.
 // ---------------------------------------------------------------------------
-template <int D, int N, typename T, bool RT, int ES>
-static int line_batch_es(const BatchBlock *blocks, int nblocks, const StageArgs *args, const PhysConst *pcs,
-                         const Ops1D &ops, hipStream_t st, int *epb)
-{
-    using LG = LineGeo<D, N, T, ES>;
-    if constexpr (LG::SBUF * sizeof(T) > 160 * 1024 - 4096) {
-        return DGX_ERR_UNSUPPORTED;  // variant would not fit the 160 KB of LDS a block can hold
-    } else {
-        if (epb) *epb = LG::EPB;
-        if (!blocks) return DGX_OK;  // query only
-        if constexpr (D == 2 && N < 7 && std::is_same<T, float>::value)
-            hipLaunchKernelGGL((line_kernel_batch_w5<D, N, RT, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
-                               args, pcs, ops);
-        else
-            hipLaunchKernelGGL((line_kernel_batch<D, N, RT, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
-                               args, pcs, ops);
-        return hipGetLastError() == hipSuccess ? DGX_OK : DGX_ERR_HIP;
-    }
-}
-
-template <int D, int N>
-static int line_batch_dn(int precision, int es, const BatchBlock *blocks, int nblocks, const StageArgs *args,
-                         const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb)
-{
-    if (precision == DGX_MIXED) {
-        if (es == 2) return line_batch_es<D, N, float, false, 2>(blocks, nblocks, args, pcs, ops, st, epb);
-        return line_batch_es<D, N, float, false, 1>(blocks, nblocks, args, pcs, ops, st, epb);
-    }
-    if (es == 2) return line_batch_es<D, N, double, true, 2>(blocks, nblocks, args, pcs, ops, st, epb);
-    return line_batch_es<D, N, double, true, 1>(blocks, nblocks, args, pcs, ops, st, epb);
-}
-
 int launch_line_batch(int dim, int n, int precision, int es, const BatchBlock *blocks, int nblocks,
                       const StageArgs *args, const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb)
 {
@@ -195,6 +70,18 @@ inverse_mass_kernel(const double *__restrict__ in, double *__restrict__ out, lon
         if (dim == 3) W *= ops.w[c];
         out[q] = in[q] / (W * detJ);
     }
+}
+
+__global__ void __launch_bounds__(256) fill_kernel(double *__restrict__ out, long n, double value)
+{
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) out[q] = value;
+}
+
+int launch_fill(dgx_ctx *c, double *out, int64_t n, double value)
+{
+    const long blocks = std::min<long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, out, (long)n, value);
+    return check_launch(c, "fill_kernel");
 }
 
 int launch_inverse_mass(dgx_ctx *c, const double *in, double *out)
@@ -526,26 +413,35 @@ int launch_entropy(dgx_ctx *c, const double *u, double *out)
 
 // QGauss(p+1+overint) interpolation operators (values, derivatives, weights) and the per-cell
 // partial buffer shared by the reduction kernels.
+// QGauss(nx) table of the integrated-quantity kernels: values B[nx][n] and derivatives G[nx][n]
+// of the GLL Lagrange basis at the Gauss points, then the Gauss weights w[nx] (2 nx n + nx
+// doubles written to h).  nx <= 32.
+static void iq_table(int n, int nx, double *h)
+{
+    double x[32], w[32], xs[MAXN1D], ws[MAXN1D];
+    gauss_legendre(nx, x, w);
+    gauss_lobatto(n, xs, ws);
+    for (int q = 0; q < nx; ++q) {
+        double val[MAXN1D], der[MAXN1D];
+        lagrange(n, xs, x[q], val, der);
+        for (int j = 0; j < n; ++j) {
+            h[q * n + j] = val[j];
+            h[nx * n + q * n + j] = der[j];
+        }
+    }
+    for (int q = 0; q < nx; ++q) h[2 * nx * n + q] = w[q];
+}
+
 static int iq_operators(dgx_ctx *c, int overint)
 {
     const int n = c->n, nx = c->p + 1 + overint;
-    if (nx > 32) {
+    if (overint < 0 || nx > 32) {
         c->err = "over-integration too large";
         return DGX_ERR_INVALID;
     }
     if (c->iq_overint != overint) {
-        std::vector<double> x(nx), w(nx), xs(n), ws(n), h((size_t)2 * nx * n + nx);
-        gauss_legendre(nx, x.data(), w.data());
-        gauss_lobatto(n, xs.data(), ws.data());
-        for (int q = 0; q < nx; ++q) {
-            double val[MAXN1D], der[MAXN1D];
-            lagrange(n, xs.data(), x[q], val, der);
-            for (int j = 0; j < n; ++j) {
-                h[q * n + j] = val[j];
-                h[nx * n + q * n + j] = der[j];
-            }
-        }
-        for (int q = 0; q < nx; ++q) h[2 * nx * n + q] = w[q];
+        std::vector<double> h((size_t)2 * nx * n + nx);
+        iq_table(n, nx, h.data());
         (void)hipFree(c->iq_ops);
         c->iq_ops = nullptr;
         if (hipMalloc(&c->iq_ops, sizeof(double) * h.size()) != hipSuccess ||

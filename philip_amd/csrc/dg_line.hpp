@@ -428,6 +428,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             NodeF<D> n;
             for (int s = 0; s < NS; ++s) n.u[s] = float(ud[s]);
             node_fill<D>(n, fc);
+            flag_nonphysical<float>(a, n.u[0], n.p);
             nodef_store<D>(q, NP, n);
         } else {
             double ut[NS];
@@ -437,7 +438,9 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             for (int s = 0; s < NS; ++s) q[s * NP] = ut[s];
             const double rho = ut[0];
             for (int k = 0; k < D; ++k) q[(NS + k) * NP] = ut[1 + k] / rho;
-            q[(NS + D) * NP] = pressure<double, D>(ut, pc.gamm1);
+            const double pr = pressure<double, D>(ut, pc.gamm1);
+            flag_nonphysical<double>(a, rho, pr);
+            q[(NS + D) * NP] = pr;
         }
     }
 #pragma unroll

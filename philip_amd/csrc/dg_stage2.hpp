@@ -114,11 +114,13 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
             roundtrip<T, D>(tmp, pc, ut);
         }
         for (int s = 0; s < NS; ++s) sU[le][s][i] = ut[s];
+        const T pr = pressure<T, D>(ut, gamm1);
+        flag_nonphysical<T>(a, ut[0], pr);
         if (SPLIT) {
             const T rho = ut[0];
 #pragma unroll
             for (int k = 0; k < D; ++k) sP[le][k][i] = ut[1 + k] / rho;
-            sP[le][D][i] = pressure<T, D>(ut, gamm1);
+            sP[le][D][i] = pr;
         }
         if (!SPLIT || NSV) {
             T g[NS * D];

@@ -1,13 +1,19 @@
 """Device path vs the CPU FP64 oracle on identical seeded inputs, through the C ABI.
 
-Bars (stated per check): integer connectivity bit-exact; FP64 residual
-||R_dev - R_oracle|| / ||R_oracle|| <= max(1e-12, 2 x floor) in the inf- and
-2-norms, where floor is how far the oracle's own residual moves when its input
-is perturbed by one ulp (measured per case: ~1e-14 for conservative forms,
-up to ~2.5e-12 for split forms at M=0.1, p>=3, whose FP64 evaluation is that
-ill-conditioned); time-stepped solutions <= max(1e-11, 4 x floor) with the floor
-measured the same way on the stepped oracle; the reference's golden
-viscous-TGV kinetic energy reproduced at its own 1e-10 tolerance.
+Bars (fixed, stated per check):
+* integer connectivity bit-exact;
+* FP64 residual: per DoF, |R_dev - R_oracle| <= 1e-12 x the sum of the absolute values of the
+  terms summed into that DoF (two-point or flux-divergence volume terms, numerical and interior
+  surface fluxes, viscous terms; ``Oracle.residual_magnitude``), the FP64 rounding scale of a
+  sum evaluated in another order.  The conventional max-norm relative error
+  ||R_dev - R_oracle||_inf / ||R_oracle||_inf is printed beside it: at M = 0.1 the fluxes are
+  ~70x the residual (the pressure p_inf = 1/(gamma M^2) cancels), so that figure reaches a few
+  1e-12 for split forms without any term being wrong;
+* lifted right-hand side (M^-1 R): the same per-DoF bar with the magnitude lifted alike;
+* time-stepped solutions: ||u_dev - u_oracle||_inf / ||u_oracle||_inf <= 1e-12;
+* the reference's golden viscous-TGV kinetic energy reproduced at its own 1e-10 tolerance.
+3D cases of every degree run on 4^3 (or 3^3) meshes, whose -x and +x neighbours of a cell are
+different cells.
 """
 import os
 import sys
@@ -20,7 +26,8 @@ import philip_amd as P
 
 pytestmark = pytest.mark.gpu
 
-TOL_RESIDUAL = 1e-12
+TOL_RESIDUAL = 1e-12   # per DoF, relative to the DoF's term magnitude
+TOL_STATE = 1e-12      # stepped solutions, max-norm relative
 
 
 def log(msg):
@@ -40,15 +47,31 @@ def rel(a, b):
     return max(np.abs(a - b).max() / np.abs(b).max(), np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def ulp_perturb(u, seed=0):
-    rng = np.random.default_rng(seed)
-    return u * (1.0 + rng.integers(-1, 2, u.size) * 2.0 ** -52)
+def dof_rel_err(o, a, b, mag):
+    """max over DoFs of |a - b| / magnitude; a DoF whose terms are all below 1e-8 of its state's
+    largest magnitude is measured against that floor."""
+    m = mag.reshape(o.ncells, o.ns, o.Np)
+    den = np.maximum(m, 1e-8 * m.max(axis=(0, 2), keepdims=True))
+    return float((np.abs(a - b).reshape(m.shape) / den).max())
 
 
-def residual_bar(o, u, r_ref):
-    """max(1e-12, 2 x the oracle's own 1-ulp input sensitivity) for this case."""
-    floor = rel(o.residual(ulp_perturb(u)), r_ref)
-    return max(TOL_RESIDUAL, 2.0 * floor), floor
+def check_residual(o, d, u, label, lifted=True):
+    """Device residual (and M^-1 R) against the oracle at the fixed per-DoF bar; returns the
+    per-DoF error and the max-norm relative error."""
+    r_ref = o.residual(u)
+    mag = o.residual_magnitude(u)
+    d.set_solution(u)
+    r = d.assemble_residual()
+    e = dof_rel_err(o, r, r_ref, mag)
+    e_max = np.abs(r - r_ref).max() / np.abs(r_ref).max()
+    msg = f"  {label}: per-DoF err {e:.2e} (bar {TOL_RESIDUAL:.0e}), max-norm rel err {e_max:.2e}"
+    if lifted:
+        el = dof_rel_err(o, d.apply_inverse_mass_to_rhs(), o.inverse_mass(r_ref), o.inverse_mass(mag))
+        msg += f", lifted per-DoF err {el:.2e}"
+        assert el <= TOL_RESIDUAL, label
+    log(msg)
+    assert e <= TOL_RESIDUAL, label
+    return r, e, e_max
 
 
 SCHEMES = {
@@ -79,21 +102,13 @@ def test_connectivity_bit_exact():
 @pytest.mark.parametrize("scheme", list(SCHEMES))
 @pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
 def test_residual_3d(scheme, p):
-    kw = dict(dim=3, poly_degree=p, n_cells_1d=4 if p <= 3 else 2, **SCHEMES[scheme])
-    log(f"residual 3d scheme={scheme} p={p}")
+    kw = dict(dim=3, poly_degree=p, n_cells_1d=4, **SCHEMES[scheme])
+    log(f"residual 3d scheme={scheme} p={p} 4^3")
     o, d = make_pair(**kw)
     u = perturbed(o.ic_tgv(), o, seed=10 + p)
-    r_ref = o.residual(u)
-    d.set_solution(u)
-    r = d.assemble_residual()
+    r, _, _ = check_residual(o, d, u, f"{scheme} p={p}")
     # run-to-run determinism: no reductions or races, so two launches agree bitwise
     assert np.array_equal(r, d.assemble_residual())
-    e = rel(r, r_ref)
-    bar, floor = residual_bar(o, u, r_ref)
-    log(f"  rel err {e:.3e} (bar {bar:.1e}, oracle 1-ulp floor {floor:.1e})")
-    assert e <= bar
-    # lifted right-hand side (apply_inverse_global_mass_matrix)
-    assert rel(d.apply_inverse_mass_to_rhs(), o.inverse_mass(r_ref)) <= bar
 
 
 @pytest.mark.parametrize("scheme", ["lxf", "roe", "ch_roe", "ir_ec", "ra_ec"])
@@ -104,46 +119,36 @@ def test_residual_2d_vortex(scheme, p):
     log(f"residual 2d scheme={scheme} p={p}")
     o, d = make_pair(**kw)
     u = perturbed(o.ic_vortex(strength=3.0, u_inf=0.7, v_inf=0.9), o, seed=p)
-    r_ref = o.residual(u)
-    d.set_solution(u)
-    e = rel(d.assemble_residual(), r_ref)
-    bar, floor = residual_bar(o, u, r_ref)
-    log(f"  rel err {e:.3e} (bar {bar:.1e}, oracle 1-ulp floor {floor:.1e})")
-    assert e <= bar
+    check_residual(o, d, u, f"2d {scheme} p={p}", lifted=False)
+
+
+def ns_kw(p, diss, split, n_cells=4, **extra):
+    return dict(dim=3, poly_degree=p, n_cells_1d=n_cells, pde=O.NAVIER_STOKES, diss_flux=diss,
+                reynolds_inf=1600.0, prandtl=0.71, mach_inf=0.1, **extra,
+                **(dict(use_split_form=1, two_point_flux=O.RA, conv_flux=O.TWO_POINT_ROE) if split else
+                   dict(conv_flux=O.ROE)))
 
 
 @pytest.mark.parametrize("diss", [O.SIPG, O.BR2])
 @pytest.mark.parametrize("split", [0, 1])
-@pytest.mark.parametrize("p", [2, 3, 5])
+@pytest.mark.parametrize("p", [2, 3, 4, 5, 6])
 def test_residual_navier_stokes(diss, split, p):
-    kw = dict(dim=3, poly_degree=p, n_cells_1d=4 if p <= 3 else 2, pde=O.NAVIER_STOKES, diss_flux=diss,
-              reynolds_inf=1600.0, prandtl=0.71, mach_inf=0.1,
-              **(dict(use_split_form=1, two_point_flux=O.RA, conv_flux=O.TWO_POINT_ROE) if split else
-                 dict(conv_flux=O.ROE)))
-    log(f"residual NS diss={diss} split={split} p={p}")
+    kw = ns_kw(p, diss, split)
+    log(f"residual NS diss={diss} split={split} p={p} 4^3")
     o, d = make_pair(**kw)
     u = perturbed(o.ic_tgv(), o, seed=100 + p)
-    r_ref = o.residual(u)
-    d.set_solution(u)
-    r = d.assemble_residual()
+    check_residual(o, d, u, f"NS diss={diss} split={split} p={p}", lifted=False)
     for k in range(3):
-        assert rel(d.auxiliary_solution(k), o.auxiliary(k)) <= TOL_RESIDUAL
-    e = rel(r, r_ref)
-    bar, floor = residual_bar(o, u, r_ref)
-    log(f"  rel err {e:.3e} (bar {bar:.1e}, oracle 1-ulp floor {floor:.1e})")
-    assert e <= bar
+        q, q_ref = d.auxiliary_solution(k), o.auxiliary(k)
+        assert np.abs(q - q_ref).max() <= TOL_RESIDUAL * np.abs(q_ref).max()
 
 
-def test_residual_constant_viscosity_and_lexicographic_mesh():
-    kw = dict(dim=3, poly_degree=3, n_cells_1d=3, pde=O.NAVIER_STOKES, diss_flux=O.BR2,
-              use_constant_viscosity=1, constant_viscosity=1.0, reynolds_inf=1600.0, prandtl=0.71,
-              use_split_form=1, two_point_flux=O.RA, conv_flux=O.TWO_POINT_ROE)
+@pytest.mark.parametrize("p", [3, 5])
+def test_residual_constant_viscosity_and_lexicographic_mesh(p):
+    kw = ns_kw(p, O.BR2, 1, n_cells=3, use_constant_viscosity=1, constant_viscosity=1.0)
     o, d = make_pair(**kw)
     u = perturbed(o.ic_tgv(), o, seed=7)
-    d.set_solution(u)
-    r_ref = o.residual(u)
-    bar, _ = residual_bar(o, u, r_ref)
-    assert rel(d.assemble_residual(), r_ref) <= bar
+    check_residual(o, d, u, f"NS constant viscosity 3^3 p={p}", lifted=False)
 
 
 @pytest.mark.parametrize("method", ["euler", "heun", "ssprk3", "rk4", "rk4_3_5_3sstar", "rk3_2_5f_3sstarplus",
@@ -160,15 +165,11 @@ def test_time_stepping(method):
     dt = o.constant_time_step(0.01)
     d.allocate_ode_system(method)
     d.step_in_time(dt, 3)
-    up = ulp_perturb(u)
     for _ in range(3):
         o.step(method, dt, u)
-        o.step(method, dt, up)
-    floor = rel(up, u)
-    e = rel(d.solution(), u)
-    bar = max(1e-11, 4.0 * floor)
-    log(f"  rel err {e:.3e} (bar {bar:.1e}, oracle 1-ulp floor {floor:.1e})")
-    assert e <= bar
+    e = np.abs(d.solution() - u).max() / np.abs(u).max()
+    log(f"  rel err {e:.3e} (bar {TOL_STATE:.0e})")
+    assert e <= TOL_STATE
     assert d.current_iteration == 3 and abs(d.current_time - 3 * dt) < 1e-15
 
 
@@ -235,7 +236,7 @@ def test_adaptive_run_matches_oracle_steps():
         t += dt
         n += 1
     assert steps == n
-    assert rel(d.solution(), u) <= 1e-11
+    assert np.abs(d.solution() - u).max() / np.abs(u).max() <= TOL_STATE
 
 
 def test_restart_continues_bitwise(tmp_path):
