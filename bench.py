@@ -12,11 +12,14 @@ Time step: the reference's TGV rule dt = CFL*L/(n_dofs/nstate)^(1/3) with CFL 0.
 CFL 0.1 diverges within ten steps in the FP64 oracle as well).
 
 Metric: DOF-updates/s per RK stage = DOF x stages x steps / wall time.  The
-headline line is the mixed path (FP32 fluxes and contractions, FP64 state and
-RK registers); the FP64 path's throughput is reported beside it.
+headline `value` is the FP64 path (the reference's precision: every flux, the
+entropy round trip and the state in double).  The mixed path (FP32 fluxes and
+contractions, FP64 state, RK registers and reductions) is reported beside it in
+`mixed_path`, with its own roofline and its error against the FP64 path.
 Roofline: HBM-bound; algorithmic bytes per stage launch (FP64 registers of the
 fused 3S* stage) = read S1,S2,S3 + write S1',S2 = 40 B/DOF (32 B/DOF for the
-first stage, which writes S3 = S1 and starts S2 from zero).
+first stage, which writes S3 = S1 and starts S2 from zero).  Both paths move the
+same algorithmic bytes.
 """
 import argparse
 import json
@@ -28,14 +31,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "line_kernel_tgv_p3_32cubed_traffic.txt")
+KERNEL = {"fp64": "line_kernel<3,4,true,double,false>", "mixed": "line_kernel<3,4,false,float,false>"}
+TRAFFIC_FILE = {"fp64": os.path.join(ROOT, "profiles", "line_kernel_fp64_tgv_p3_32cubed_traffic.txt"),
+                "mixed": os.path.join(ROOT, "profiles", "line_kernel_tgv_p3_32cubed_traffic.txt")}
 
 
-def measured_traffic():
-    """HBM bytes per launch of the mixed stage kernel from the committed rocprofv3 counter
-    summary (read requests by size + WRITE_SIZE, calibrated on a plain copy), or None."""
+def measured_traffic(precision):
+    """HBM bytes per launch of the stage kernel from the committed rocprofv3 counter summary
+    (read requests by size + WRITE_SIZE, calibrated on a plain copy), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILE[precision]) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch"):
                     return float(line.split("=")[1])
@@ -50,21 +55,38 @@ def workload_params(precision):
                 conv_flux=P.TWO_POINT_ROE, pde=P.EULER, mach_inf=0.1, precision=precision)
 
 
-def cpu_baseline(threads):
+def host_cores():
+    """CPUs this process may run on (the box's share) and the host's total."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return usable, os.cpu_count() or usable
+
+
+def cpu_baseline(threads, min_seconds=10.0, max_steps=8):
     """Oracle (FP64 C restatement of the reference loop structure, OpenMP over cells) timed on
-    the host: one RK3(2)5F[3S*+] step of the full workload mesh (5 stages + the reference's FSAL
-    residual), credited with 5 stage updates."""
+    the host: RK3(2)5F[3S*+] steps of the full workload mesh, 5 stage residuals per step (the FSAL
+    residual of the embedded estimate is skipped, as the device path skips it at a fixed time
+    step), repeated until at least `min_seconds` of CPU time have passed."""
     import oracle as O
     O.set_threads(threads)
     o = O.Oracle(dim=3, poly_degree=3, n_cells_1d=32, use_split_form=1, two_point_flux=O.CH,
                  conv_flux=O.TWO_POINT_ROE, mach_inf=0.1)
+    o.set_fsal(False)
     u = o.ic_tgv()
     dt = o.constant_time_step(0.01)
     t0 = time.perf_counter()
-    o.step("rk3_2_5f_3sstarplus", dt, u)
+    n = 0
+    while n < max_steps and (n == 0 or time.perf_counter() - t0 < min_seconds):
+        o.step("rk3_2_5f_3sstarplus", dt, u)
+        n += 1
     t = time.perf_counter() - t0
-    return dict(value=o.ndofs * 5 / t, unit="DOF-updates/s", cores=threads, kind="port",
-                sample=f"1 RK3(2)5F[3S*+] step (5 stages + FSAL residual) of the 32^3 p=3 mesh, {t:.1f} s")
+    usable, total = host_cores()
+    return dict(value=o.ndofs * 5 * n / t, unit="DOF-updates/s", cores=threads, kind="port",
+                host_cpus_usable=usable, host_cpu_count=total,
+                sample=f"{n} RK3(2)5F[3S*+] step(s) (5 stage residuals each, no FSAL residual) of the 32^3 p=3 "
+                       f"mesh, {t:.1f} s on {threads} OpenMP threads")
 
 
 def run_gpu(precision, steps, warmup):
@@ -312,15 +334,26 @@ def copy_calibration(nbytes=1 << 30, reps=5):
                       "launches": reps}), flush=True)
 
 
+def roofline(precision, r, bytes_per_launch):
+    achieved = bytes_per_launch / (r["avg_ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(precision),
+            "kernel": KERNEL[precision], "avg_launch_ms": r["avg_ms"],
+            "algorithmic_bytes_per_launch": bytes_per_launch}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="precision of the headline line (the other is reported in its own block)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp64-line", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--no-second-precision", "--no-fp64-line", dest="no_second", action="store_true",
+                    help="skip the block of the other precision")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline "
+                    "(default: every CPU this process may run on)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances line")
@@ -340,7 +373,7 @@ def main():
     ndofs = r["ndofs"]
     # first stage of a 3S* step moves 32 B/DOF, the other four 40 B/DOF
     bytes_per_launch = ndofs * (32 + 4 * 40) / 5.0
-    achieved = bytes_per_launch / (r["avg_ms"] * 1e-3) / 1e9
+    other = "mixed" if args.precision == "fp64" else "fp64"
     line = {
         "metric": "DOF-updates/s per RK stage (p=3 Euler TGV)",
         "value": r["value"],
@@ -358,28 +391,32 @@ def main():
                    "dof": ndofs, "stages_per_step": r["stages"], "precision": args.precision,
                    "kinetic_energy_after_run": r["ke"],
                    "host_buffer_rate": r["pcie"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic() if args.precision == "mixed" else None,
-                     "kernel": "line_kernel<3,4,mixed>", "avg_launch_ms": r["avg_ms"],
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "roofline": roofline(args.precision, r, bytes_per_launch),
     }
-    if args.precision == "mixed" and not args.no_fp64_line:
-        r64 = run_gpu("fp64", max(5, args.steps // 4), 2)
-        line["fp64_path"] = {"value": r64["value"], "avg_launch_ms": r64["avg_ms"],
-                             "roofline_frac": bytes_per_launch / (r64["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if not args.no_second:
+        r2 = run_gpu(other, args.steps, args.warmup)
+        line[other + "_path"] = {"value": r2["value"], "ms_per_step": r2["ms_per_step"],
+                                 "dtype": "f64" if other == "fp64" else "f32-flux/f64-state",
+                                 "roofline": roofline(other, r2, bytes_per_launch)}
     if not args.no_batch:
         line["batched"] = batched_vortex()
-        line["navier_stokes"] = [navier_stokes_tgv("mixed"), navier_stokes_tgv("fp64")]
+        line["navier_stokes"] = [navier_stokes_tgv("fp64"), navier_stokes_tgv("mixed")]
     if not args.no_ns_history:
         line["tgv_1000_steps"] = tgv_history()
         line["navier_stokes_t2"] = navier_stokes_history()
+        if "mixed_path" in line:
+            h = line["tgv_1000_steps"]
+            line["mixed_path"]["error_vs_fp64"] = {
+                "max_rel_diff_kinetic_energy_1000_steps": h["max_rel_diff_kinetic_energy"],
+                "entropy_drift_mixed": h["mixed"]["entropy_drift"],
+                "max_entropy_diff_rel_S0": h["max_entropy_diff_rel_S0"]}
     if not args.no_serving:
         line["serving"] = serving()
         # arrival rate swept at both precisions (same request stream per rate)
         line["serving_sweep"] = [serving(rate=r, precision=pr) for pr in ("mixed", "fp64") for r in (20.0, 40.0, 80.0)]
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_threads)
+        threads = args.cpu_threads or host_cores()[0]
+        line["cpu_baseline"] = cpu_baseline(threads)
     print(json.dumps(line), flush=True)
 
 

@@ -143,6 +143,22 @@ int dgx_set_time(dgx_ctx *ctx, double time, int64_t iteration);
    not change the solution at a fixed time step. */
 int dgx_set_track_error_estimate(dgx_ctx *ctx, int enable);
 
+/* Error-adaptive low-storage RK (flow_solver_param.error_adaptive_time_step, flow_solver.cpp:505-507,
+   603-607; LowStorageRungeKuttaODESolver, low_storage_runge_kutta_ode_solver.cpp:89-178).
+   dgx_set_error_adaptive_parameters: the ODE parameters atol, rtol, beta1..3 (defaults 1e-3, 1e-3,
+   0.70, -0.23, 0.0, parameters_ode_solver.cpp:159-178); resets the controller history
+   epsilon = {1,1,1} and makes 3S*+ steps evaluate their FSAL residual (the embedded register).
+   dgx_get_automatic_initial_step_size: get_automatic_initial_step_size (:119-178) from two
+   right-hand sides of the current solution (needs a low-storage ODE system).
+   dgx_get_error_adaptive_step_size: get_automatic_error_adaptive_step_size (:89-118) after a
+   step of size dt: w = RMS of (S1 - E) / (atol + rtol max(|S1|, |E|)) with E the embedded
+   solution (S4, or the S2 of sum_stages for RK4(3)5[3S*]), epsilon shifted in, and
+   dt_next = eps0^(beta1/k) eps1^(beta2/k) eps2^(beta3/k) dt (k the method's order). */
+int dgx_set_error_adaptive_parameters(dgx_ctx *ctx, double atol, double rtol, double beta1, double beta2,
+                                      double beta3);
+int dgx_get_automatic_initial_step_size(dgx_ctx *ctx, double *dt);
+int dgx_get_error_adaptive_step_size(dgx_ctx *ctx, double dt, double *dt_next);
+
 /* PeriodicTurbulence::get_constant_time_step (periodic_turbulence.cpp:115-125). */
 double dgx_constant_time_step(const dgx_ctx *ctx, double cfl);
 /* PeriodicTurbulence::compute_and_update_integrated_quantities (periodic_turbulence.cpp:439-671)

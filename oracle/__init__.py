@@ -22,6 +22,8 @@ ORDER_DEALII, ORDER_LEXICOGRAPHIC = 0, 1
 METHODS = {"euler": 0, "heun": 1, "ssprk3": 2, "rk4": 3,
            "rk4_3_5_3sstar": 10, "rk3_2_5f_3sstarplus": 11,
            "rk4_3_9f_3sstarplus": 12, "rk5_4_10f_3sstarplus": 13}
+# rk_order of the low-storage methods (parameters_ode_solver.cpp:270-296)
+RK_ORDER = {"rk4_3_5_3sstar": 4, "rk3_2_5f_3sstarplus": 3, "rk4_3_9f_3sstarplus": 4, "rk5_4_10f_3sstarplus": 5}
 
 
 class Params(ctypes.Structure):
@@ -104,6 +106,10 @@ def _load():
     lib.dgo_run.argtypes = [vp, i, d, d, dp, dp, i]
     lib.dgo_set_threads.argtypes = [i]
     lib.dgo_set_mutation.argtypes = [i]
+    lib.dgo_step_error_norm.restype = d
+    lib.dgo_step_error_norm.argtypes = [vp, i, d, dp, d, d]
+    lib.dgo_initial_step_size.restype = d
+    lib.dgo_initial_step_size.argtypes = [vp, dp]
     lib.dgo_set_fsal.argtypes = [vp, i]
     lib.dgo_residual_magnitude.argtypes = [vp, dp, dp]
     return lib
@@ -318,6 +324,35 @@ class Oracle:
         m = METHODS[method] if isinstance(method, str) else method
         lib().dgo_step(self.h, m, dt, _dp(u))
         return u
+
+    def step_error_norm(self, method, dt, u, atol=1e-3, rtol=1e-3):
+        """One low-storage RK step of u in place; returns the error norm w of its embedded pair."""
+        m = METHODS[method] if isinstance(method, str) else method
+        return lib().dgo_step_error_norm(self.h, m, dt, _dp(u), atol, rtol)
+
+    def initial_step_size(self, u):
+        return lib().dgo_initial_step_size(self.h, _dp(np.ascontiguousarray(u, dtype=np.float64)))
+
+    def run_error_adaptive(self, method, final_time, u, atol=1e-3, rtol=1e-3, beta=(0.70, -0.23, 0.0)):
+        """FlowSolver::run with error_adaptive_time_step (flow_solver.cpp:505-507, 554-607): the first
+        step from get_automatic_initial_step_size, every later one from the PID update of
+        get_automatic_error_adaptive_step_size (low_storage_runge_kutta_ode_solver.cpp:89-118)
+        with epsilon = {1, 1, 1} initially; the last step clipped to the final time.  Returns the
+        steps taken."""
+        k = RK_ORDER[method]
+        eps = [1.0, 1.0, 1.0]
+        dt_next = self.initial_step_size(u)
+        t, dts = 0.0, []
+        while t < final_time:
+            dt = dt_next
+            if t + dt > final_time:
+                dt = final_time - t
+            w = self.step_error_norm(method, dt, u, atol, rtol)
+            t += dt
+            dts.append(dt)
+            eps = [1.0 / w, eps[0], eps[1]]
+            dt_next = pow(eps[0], 1.0 * beta[0] / k) * pow(eps[1], 1.0 * beta[1] / k) * pow(eps[2], 1.0 * beta[2] / k) * dt
+        return dts
 
     def constant_time_step(self, cfl):
         return lib().dgo_constant_time_step(self.h, cfl)

@@ -1806,6 +1806,107 @@ static void rhs_lifted(dgo_ctx *c, const double *u, double *k, double *scratch)
     dgo_apply_inverse_mass(c, scratch, k, 0);
 }
 
+/* LowStorageRungeKuttaODESolver (low_storage_runge_kutta_ode_solver.cpp:24-79, :207-222) one
+   step; with w != NULL also the error norm of the embedded pair that
+   get_automatic_error_adaptive_step_size (:89-118) computes after the step: S1 against S4
+   (3S*+, FSAL residual evaluated) or against the embedded solution S2 of sum_stages (:55-62). */
+static void lsrk_step(dgo_ctx *c, int m, const lsrk *Lp, double dt, double *u, double *scratch, double atol,
+                      double rtol, double *w)
+{
+    const lsrk L = *Lp;
+    const long nd = dgo_n_dofs(c);
+    double *S1 = (double *)malloc(sizeof(double) * nd);
+    double *S2 = (double *)calloc((size_t)nd, sizeof(double));
+    double *S3 = (double *)malloc(sizeof(double) * nd);
+    double *S4 = (double *)malloc(sizeof(double) * nd);
+    double *r = (double *)malloc(sizeof(double) * nd);
+    memcpy(S1, u, sizeof(double) * nd);
+    memcpy(S3, S1, sizeof(double) * nd);
+    memcpy(r, S1, sizeof(double) * nd);
+    memcpy(S4, S1, sizeof(double) * nd);
+    for (int i = 0; i < L.s; ++i) {
+        for (long q = 0; q < nd; ++q) S2[q] += L.delta[i] * S1[q];
+        memcpy(u, r, sizeof(double) * nd);
+        rhs_lifted(c, u, r, scratch);
+        for (long q = 0; q < nd; ++q) {
+            S1[q] *= L.gamma[i + 1][0];
+            S1[q] += L.gamma[i + 1][1] * S2[q];
+            S1[q] += L.gamma[i + 1][2] * S3[q];
+            r[q] *= dt;
+            S1[q] += L.beta[i + 1] * r[q];
+            if (L.plus) S4[q] += L.bhat[i] * r[q];
+        }
+        memcpy(r, S1, sizeof(double) * nd);
+    }
+    if (L.plus && (c->fsal || w)) {
+        /* FSAL residual for the embedded estimate (:63-69); it only updates S4 */
+        memcpy(u, r, sizeof(double) * nd);
+        rhs_lifted(c, u, r, scratch);
+        for (long q = 0; q < nd; ++q) { r[q] *= dt; S4[q] += L.bhat[L.s] * r[q]; }
+    }
+    if (!L.plus) {
+        /* embedded solution (:55-62): S2 = (S2 + delta_s S1 + delta_{s+1} S3) / sum of the num_delta deltas */
+        const int num_delta = (m == 10) ? 7 : L.s;
+        double sum_delta = 0.0;
+        for (int i = 0; i < num_delta; ++i) sum_delta += L.delta[i];
+        for (long q = 0; q < nd; ++q) {
+            S2[q] += L.delta[L.s] * S1[q];
+            S2[q] += L.delta[L.s + 1] * S3[q];
+            S2[q] /= sum_delta;
+        }
+    }
+    if (w) {
+        const double *E = L.plus ? S4 : S2;
+        double acc = 0.0;
+        for (long q = 0; q < nd; ++q) {
+            const double err = S1[q] - E[q];
+            const double a = fabs(S1[q]), b = fabs(E[q]);
+            acc = acc + pow(err / (atol + rtol * (a > b ? a : b)), 2);
+        }
+        *w = pow(acc / (double)nd, 0.5);
+    }
+    memcpy(u, S1, sizeof(double) * nd);
+    free(S1); free(S2); free(S3); free(S4); free(r);
+}
+
+double dgo_step_error_norm(dgo_ctx *c, int m, double dt, double *u, double atol, double rtol)
+{
+    lsrk L;
+    if (!get_lsrk(m, &L)) return -1.0;
+    double *scratch = (double *)malloc(sizeof(double) * dgo_n_dofs(c));
+    double w = 0.0;
+    lsrk_step(c, m, &L, dt, u, scratch, atol, rtol, &w);
+    free(scratch);
+    return w;
+}
+
+/* LowStorageRungeKuttaODESolver::get_automatic_initial_step_size (low_storage_runge_kutta_ode_solver.cpp:119-178) */
+double dgo_initial_step_size(dgo_ctx *c, const double *u)
+{
+    const long nd = dgo_n_dofs(c);
+    double *un = (double *)malloc(sizeof(double) * nd);
+    double *f0 = (double *)malloc(sizeof(double) * nd);
+    double *f1 = (double *)malloc(sizeof(double) * nd);
+    double *scratch = (double *)malloc(sizeof(double) * nd);
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0, h0, h1;
+    for (long q = 0; q < nd; ++q) d0 = fabs(u[q]) > d0 ? fabs(u[q]) : d0;
+    rhs_lifted(c, u, f0, scratch);
+    for (long q = 0; q < nd; ++q) d1 = fabs(f0[q]) > d1 ? fabs(f0[q]) : d1;
+    h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+    for (long q = 0; q < nd; ++q) un[q] = u[q] + h0 * f0[q];
+    rhs_lifted(c, un, f1, scratch);
+    for (long q = 0; q < nd; ++q) {
+        const double x = fabs(f1[q] - f0[q]);
+        d2 = x > d2 ? x : d2;
+    }
+    d2 /= h0;
+    const double dm = d1 > d2 ? d1 : d2;
+    if (dm <= 1e-15) h1 = (1e-6 > h0 * 1e-3) ? 1e-6 : h0 * 1e-3;
+    else h1 = 0.01 / dm;
+    free(un); free(f0); free(f1); free(scratch);
+    return (100 * h0 < h1) ? 100 * h0 : h1;
+}
+
 void dgo_step(dgo_ctx *c, int m, double dt, double *u)
 {
     const long nd = dgo_n_dofs(c);
@@ -1834,38 +1935,7 @@ void dgo_step(dgo_ctx *c, int m, double dt, double *u)
         for (int i = 0; i < B.s; ++i) free(stage[i]);
         free(upd);
     } else if (get_lsrk(m, &L)) {
-        /* LowStorageRungeKuttaODESolver (low_storage_runge_kutta_ode_solver.cpp:24-79, :207-222) */
-        double *S1 = (double *)malloc(sizeof(double) * nd);
-        double *S2 = (double *)calloc((size_t)nd, sizeof(double));
-        double *S3 = (double *)malloc(sizeof(double) * nd);
-        double *S4 = (double *)malloc(sizeof(double) * nd);
-        double *r = (double *)malloc(sizeof(double) * nd);
-        memcpy(S1, u, sizeof(double) * nd);
-        memcpy(S3, S1, sizeof(double) * nd);
-        memcpy(r, S1, sizeof(double) * nd);
-        memcpy(S4, S1, sizeof(double) * nd);
-        for (int i = 0; i < L.s; ++i) {
-            for (long q = 0; q < nd; ++q) S2[q] += L.delta[i] * S1[q];
-            memcpy(u, r, sizeof(double) * nd);
-            rhs_lifted(c, u, r, scratch);
-            for (long q = 0; q < nd; ++q) {
-                S1[q] *= L.gamma[i + 1][0];
-                S1[q] += L.gamma[i + 1][1] * S2[q];
-                S1[q] += L.gamma[i + 1][2] * S3[q];
-                r[q] *= dt;
-                S1[q] += L.beta[i + 1] * r[q];
-                if (L.plus) S4[q] += L.bhat[i] * r[q];
-            }
-            memcpy(r, S1, sizeof(double) * nd);
-        }
-        if (L.plus && c->fsal) {
-            /* FSAL residual for the embedded estimate (:63-69); it only updates S4 */
-            memcpy(u, r, sizeof(double) * nd);
-            rhs_lifted(c, u, r, scratch);
-            for (long q = 0; q < nd; ++q) { r[q] *= dt; S4[q] += L.bhat[L.s] * r[q]; }
-        }
-        memcpy(u, S1, sizeof(double) * nd);
-        free(S1); free(S2); free(S3); free(S4); free(r);
+        lsrk_step(c, m, &L, dt, u, scratch, 0.0, 0.0, NULL);
     }
     free(scratch);
 }

@@ -140,6 +140,11 @@ void dgo_ic_vortex(const dgo_ctx *c, double strength, double u_inf, double v_inf
    10 RK4(3)5[3S*], 11 RK3(2)5F[3S*+], 12 RK4(3)9F[3S*+], 13 RK5(4)10F[3S*+] */
 int dgo_n_stages(int method);
 void dgo_step(dgo_ctx *c, int method, double dt, double *u);
+/* one low-storage step (methods 10..13) that also returns the error norm of its embedded pair
+   (low_storage_runge_kutta_ode_solver.cpp:89-118, before the epsilon/PID update) */
+double dgo_step_error_norm(dgo_ctx *c, int method, double dt, double *u, double atol, double rtol);
+/* get_automatic_initial_step_size (low_storage_runge_kutta_ode_solver.cpp:119-178) */
+double dgo_initial_step_size(dgo_ctx *c, const double *u);
 double dgo_constant_time_step(const dgo_ctx *c, double cfl);
 double dgo_vortex_time_step(const dgo_ctx *c, double cfl, double speed);
 

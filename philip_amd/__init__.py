@@ -32,7 +32,8 @@ EXPORTED = [
     "dgx_numerical_entropy", "dgx_max_wave_speed", "dgx_adaptive_time_step", "dgx_set_stream",
     "dgx_enable_timing", "dgx_kernel_timing", "dgx_set_initial_condition_tgv",
     "dgx_set_initial_condition_vortex", "dgx_synchronize", "dgx_set_non_physical_behavior", "dgx_check_state",
-    "dgx_nonphysical_events",
+    "dgx_nonphysical_events", "dgx_set_error_adaptive_parameters", "dgx_get_automatic_initial_step_size",
+    "dgx_get_error_adaptive_step_size",
     "dgx_batch_create", "dgx_batch_destroy", "dgx_batch_last_error", "dgx_batch_size", "dgx_batch_instance",
     "dgx_batch_allocate", "dgx_batch_step_in_time", "dgx_batch_synchronize", "dgx_batch_variant",
     "dgx_request_bytes", "dgx_request_n_dofs", "dgx_server_create", "dgx_server_destroy", "dgx_server_last_error",
@@ -133,6 +134,9 @@ def load():
     lib.dgx_check_state.argtypes = [vp]
     lib.dgx_nonphysical_events.restype = i64
     lib.dgx_nonphysical_events.argtypes = [vp]
+    lib.dgx_set_error_adaptive_parameters.argtypes = [vp, d, d, d, d, d]
+    lib.dgx_get_automatic_initial_step_size.argtypes = [vp, dp]
+    lib.dgx_get_error_adaptive_step_size.argtypes = [vp, d, dp]
     lib.dgx_apply_inverse_mass_device.argtypes = [vp, vp, vp]
     lib.dgx_batch_create.argtypes = [P(Params), i, P(vp)]
     lib.dgx_batch_destroy.argtypes = [vp]
@@ -342,25 +346,50 @@ class DG:
         row["incompressible_kinetic_energy"] = q["incompressible_kinetic_energy"]
         return row
 
-    def run(self, method, dt, final_time, adaptive_cfl=None, table=None, allocate=True):
-        """FlowSolver::run time loop (flow_solver.cpp:554-610): constant dt, or with adaptive_cfl the
+    def set_error_adaptive_parameters(self, atol=1e-3, rtol=1e-3, beta=(0.70, -0.23, 0.0)):
+        self._chk(load().dgx_set_error_adaptive_parameters(self.h, atol, rtol, *beta), "set_error_adaptive_parameters")
+
+    def automatic_initial_step_size(self):
+        o = ctypes.c_double()
+        self._chk(load().dgx_get_automatic_initial_step_size(self.h, ctypes.byref(o)), "automatic_initial_step_size")
+        return o.value
+
+    def error_adaptive_step_size(self, dt):
+        o = ctypes.c_double()
+        self._chk(load().dgx_get_error_adaptive_step_size(self.h, dt, ctypes.byref(o)), "error_adaptive_step_size")
+        return o.value
+
+    def run(self, method, dt, final_time, adaptive_cfl=None, table=None, allocate=True, error_adaptive=None):
+        """FlowSolver::run time loop (flow_solver.cpp:495-610): constant dt; or with adaptive_cfl the
         step from the device's maximum wave speed before every step (flow_solver.cpp:502-504,
-        603-604); the last step is clipped to the final time.  table: list that receives one
-        unsteady-data row per step.  allocate=False continues with the registers already allocated
-        (a restarted run)."""
+        603-604); or with error_adaptive = dict(atol=, rtol=, beta=) (a low-storage method) the first
+        step from get_automatic_initial_step_size and every later one from the embedded-error PID
+        controller (flow_solver.cpp:505-507, 605-606).  The last step is clipped to the final time.
+        table: list that receives one unsteady-data row per step.  allocate=False continues with the
+        registers already allocated (a restarted run).  The steps taken are kept in self.last_dts."""
         if allocate:
             self.allocate_ode_system(method)
+        if error_adaptive is not None:
+            self.set_error_adaptive_parameters(**error_adaptive)
+            next_dt = self.automatic_initial_step_size()
         t = self.current_time
         steps = 0
+        self.last_dts = []
         while t < final_time:
-            step = self.adaptive_time_step(adaptive_cfl) if adaptive_cfl is not None else dt
+            if error_adaptive is not None:
+                step = next_dt
+            else:
+                step = self.adaptive_time_step(adaptive_cfl) if adaptive_cfl is not None else dt
             if t + step > final_time:
                 step = final_time - t
             self.step_in_time(step, 1)
+            self.last_dts.append(step)
             t = self.current_time
             steps += 1
             if table is not None:
                 table.append(self.unsteady_row())
+            if error_adaptive is not None:
+                next_dt = self.error_adaptive_step_size(step)
         return steps
 
     def save_restart(self, path, table=None):

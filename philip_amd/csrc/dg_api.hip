@@ -137,51 +137,88 @@ Ops1D build_ops(int n)
     return o;
 }
 
-static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-
-void cell_ijk(int dim, int N, int order, std::vector<int> &ijk)
+int Arena::init(int64_t bytes)
 {
-    const int nc = (dim == 3) ? N * N * N : N * N;
-    ijk.assign((size_t)nc * 3, 0);
-    const bool morton = (order == DGX_ORDER_DEALII) && is_pow2(N);
-    int levels = 0;
-    while ((1 << levels) < N) ++levels;
-    for (int e = 0; e < nc; ++e) {
-        int c[3] = {0, 0, 0};
-        if (morton) {
-            // children of a refined cell are numbered x-fastest; coarsest level in the high bits
-            for (int bit = 0; bit < levels; ++bit)
-                for (int k = 0; k < dim; ++k) c[k] |= ((e >> (bit * dim + k)) & 1) << bit;
-        } else {
-            c[0] = e % N;
-            c[1] = (e / N) % N;
-            if (dim == 3) c[2] = e / (N * N);
-        }
-        for (int k = 0; k < 3; ++k) ijk[(size_t)e * 3 + k] = c[k];
-    }
+    bytes = (bytes + 255) & ~int64_t(255);
+    void *p = nullptr;
+    if (hipMalloc(&p, (size_t)bytes) != hipSuccess) return DGX_ERR_HIP;
+    base_ = static_cast<char *>(p);
+    size_ = bytes;
+    free_.clear();
+    used_.clear();
+    free_[0] = bytes;
+    used_bytes_ = 0;
+    return DGX_OK;
 }
 
-void neighbors(int dim, int N, const std::vector<int> &ijk, std::vector<int> &nbr)
+void Arena::release()
 {
-    const int nc = (int)(ijk.size() / 3);
-    std::vector<int> lookup((size_t)nc);
-    for (int e = 0; e < nc; ++e) {
-        const int lin = ijk[e * 3] + N * (ijk[e * 3 + 1] + (dim == 3 ? N * ijk[e * 3 + 2] : 0));
-        lookup[lin] = e;
+    if (base_) (void)hipFree(base_);
+    base_ = nullptr;
+    size_ = used_bytes_ = 0;
+    free_.clear();
+    used_.clear();
+}
+
+void *Arena::alloc(int64_t bytes)
+{
+    bytes = (bytes + 255) & ~int64_t(255);
+    if (bytes <= 0) bytes = 256;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        const int64_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > bytes) free_[off + bytes] = len - bytes;
+        used_[off] = bytes;
+        used_bytes_ += bytes;
+        return base_ + off;
     }
-    nbr.assign((size_t)nc * 2 * dim, 0);
-    for (int e = 0; e < nc; ++e)
-        for (int f = 0; f < 2 * dim; ++f) {
-            int c[3] = {ijk[e * 3], ijk[e * 3 + 1], ijk[e * 3 + 2]};
-            const int k = f / 2;
-            c[k] = (c[k] + (f % 2 ? 1 : N - 1)) % N;
-            nbr[(size_t)e * 2 * dim + f] = lookup[c[0] + N * (c[1] + (dim == 3 ? N * c[2] : 0))];
+    return nullptr;
+}
+
+void Arena::free(void *p)
+{
+    if (!p) return;
+    const int64_t off = static_cast<char *>(p) - base_;
+    auto u = used_.find(off);
+    if (u == used_.end()) return;
+    int64_t start = off, len = u->second;
+    used_bytes_ -= len;
+    used_.erase(u);
+    auto next = free_.lower_bound(start);
+    if (next != free_.end() && next->first == start + len) {  // merge with the following range
+        len += next->second;
+        next = free_.erase(next);
+    }
+    if (next != free_.begin()) {  // merge with the preceding range
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == start) {
+            start = prev->first;
+            len += prev->second;
+            free_.erase(prev);
         }
+    }
+    free_[start] = len;
 }
 
 } // namespace dgx
 
 using namespace dgx;
+
+// device buffers of a context: from its arena (serving) or hipMalloc
+static void *dev_alloc(dgx_ctx *c, size_t bytes)
+{
+    if (c->arena) return c->arena->alloc((int64_t)bytes);
+    void *p = nullptr;
+    return hipMalloc(&p, bytes) == hipSuccess ? p : nullptr;
+}
+
+static void dev_free(dgx_ctx *c, void *p)
+{
+    if (!p) return;
+    if (c->arena) c->arena->free(p);
+    else (void)hipFree(p);
+}
 
 #define HIPCHK(call)                                                                     \
     do {                                                                                 \
@@ -226,15 +263,14 @@ void dgx_default_params(dgx_params *p)
     p->precision = DGX_FP64;
 }
 
-int dgx_create(const dgx_params *p, dgx_ctx **out)
+// DGFactory::create_discontinuous_galerkin + allocate_system: operators, device buffers (from
+// `arena` when given), connectivity computed on the device; all work enqueued on `stream`.
+static int create_ctx(const dgx_params *p, dgx::Arena *arena, hipStream_t stream, dgx_ctx **out)
 {
     if (!p || !out) return DGX_ERR_INVALID;
     *out = nullptr;
-    dgx_ctx *c = new dgx_ctx();
-    c->P = *p;
     auto bad = [&](int code, const std::string &m) {
         std::fprintf(stderr, "dgx_create: %s\n", m.c_str());
-        delete c;
         return code;
     };
     if (p->dim != 2 && p->dim != 3) return bad(DGX_ERR_INVALID, "dim must be 2 or 3");
@@ -246,9 +282,15 @@ int dgx_create(const dgx_params *p, dgx_ctx **out)
         return bad(DGX_ERR_INVALID, "two point flux and not using split form are not compatible");
     if (p->pde == DGX_NAVIER_STOKES && p->dim != 3) return bad(DGX_ERR_UNSUPPORTED, "Navier-Stokes device path is 3D");
     if (p->precision != DGX_FP64 && p->precision != DGX_MIXED) return bad(DGX_ERR_INVALID, "unknown precision");
+    dgx_ctx *c = new dgx_ctx();
+    c->P = *p;
+    c->arena = arena;
+    c->stream = stream;
     if (hipGetDevice(&c->device) != hipSuccess ||
-        hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
+        hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) {
+        delete c;
         return bad(DGX_ERR_HIP, "no HIP device");
+    }
     c->dim = p->dim;
     c->p = p->poly_degree;
     c->n = c->p + 1;
@@ -271,26 +313,43 @@ int dgx_create(const dgx_params *p, dgx_ctx **out)
     pc.two_point = p->two_point_flux;
     pc.conv_flux = p->conv_flux;
     pc.diss_flux = p->diss_flux;
-    std::vector<int> ijk;
-    cell_ijk(c->dim, N, p->cell_order, ijk);
-    c->cell_lat = ijk;
-    neighbors(c->dim, N, ijk, c->nbr_host);
+    c->nbr_count = (int64_t)c->ncells * 2 * c->dim;
     const size_t bytes = sizeof(double) * (size_t)c->ndofs;
-    if (hipMalloc(&c->nbr_dev, sizeof(int) * c->nbr_host.size()) != hipSuccess ||
-        hipMalloc(&c->u_dev, bytes) != hipSuccess || hipMalloc(&c->r_dev, bytes) != hipSuccess ||
-        hipMalloc(&c->status_dev, 256) != hipSuccess || hipMemset(c->status_dev, 0, 256) != hipSuccess)
-        return bad(DGX_ERR_HIP, "hipMalloc failed");
-    if (p->pde == DGX_NAVIER_STOKES)
+    c->nbr_dev = static_cast<int *>(dev_alloc(c, sizeof(int) * (size_t)c->nbr_count));
+    c->u_dev = static_cast<double *>(dev_alloc(c, bytes));
+    c->r_dev = static_cast<double *>(dev_alloc(c, bytes));
+    c->status_dev = static_cast<int *>(dev_alloc(c, 256));
+    bool ok = c->nbr_dev && c->u_dev && c->r_dev && c->status_dev;
+    if (ok && p->pde == DGX_NAVIER_STOKES)
         for (int k = 0; k < c->dim; ++k) {
-            const hipError_t he = (p->precision == DGX_MIXED)
-                                      ? hipMalloc(&c->auxf_dev[k], sizeof(float) * (size_t)c->ndofs)
-                                      : hipMalloc(&c->aux_dev[k], bytes);
-            if (he != hipSuccess) return bad(DGX_ERR_HIP, "hipMalloc failed");
+            if (p->precision == DGX_MIXED) c->auxf_dev[k] = static_cast<float *>(dev_alloc(c, sizeof(float) * (size_t)c->ndofs));
+            else c->aux_dev[k] = static_cast<double *>(dev_alloc(c, bytes));
+            ok = ok && (c->auxf_dev[k] || c->aux_dev[k]);
         }
-    if (hipMemcpy(c->nbr_dev, c->nbr_host.data(), sizeof(int) * c->nbr_host.size(), hipMemcpyHostToDevice) != hipSuccess)
-        return bad(DGX_ERR_HIP, "hipMemcpy failed");
-    if (hipMemset(c->u_dev, 0, bytes) != hipSuccess || hipMemset(c->r_dev, 0, bytes) != hipSuccess)
-        return bad(DGX_ERR_HIP, "hipMemset failed");
+    if (!ok) {
+        c->stream = nullptr;
+        dgx_destroy(c);
+        return arena ? DGX_ERR_STATE : bad(DGX_ERR_HIP, "hipMalloc failed");  // arena full: try later
+    }
+    if (hipMemsetAsync(c->status_dev, 0, 256, stream) != hipSuccess ||
+        hipMemsetAsync(c->u_dev, 0, bytes, stream) != hipSuccess ||
+        hipMemsetAsync(c->r_dev, 0, bytes, stream) != hipSuccess || launch_mesh(c) != DGX_OK) {
+        dgx_destroy(c);
+        return bad(DGX_ERR_HIP, "device set-up failed");
+    }
+    *out = c;
+    return DGX_OK;
+}
+
+int dgx_create(const dgx_params *p, dgx_ctx **out)
+{
+    dgx_ctx *c = nullptr;
+    int rc = create_ctx(p, nullptr, nullptr, &c);
+    if (rc) return rc;
+    if (hipStreamSynchronize(nullptr) != hipSuccess) {
+        dgx_destroy(c);
+        return DGX_ERR_HIP;
+    }
     *out = c;
     return DGX_OK;
 }
@@ -299,18 +358,19 @@ int dgx_destroy(dgx_ctx *c)
 {
     if (!c) return DGX_OK;
     (void)hipStreamSynchronize(c->stream);  // teardown: errors are not actionable here
-    (void)hipFree(c->nbr_dev);
-    (void)hipFree(c->u_dev);
-    (void)hipFree(c->r_dev);
-    (void)hipFree(c->minv_dev);
-    (void)hipFree(c->status_dev);
+    dev_free(c, c->nbr_dev);
+    dev_free(c, c->u_dev);
+    dev_free(c, c->r_dev);
+    dev_free(c, c->status_dev);
     for (int k = 0; k < 3; ++k) {
-        (void)hipFree(c->aux_dev[k]);
-        (void)hipFree(c->auxf_dev[k]);
+        dev_free(c, c->aux_dev[k]);
+        dev_free(c, c->auxf_dev[k]);
     }
-    for (double *r : c->regs) (void)hipFree(r);
+    for (double *r : c->regs) dev_free(c, r);
+    (void)hipFree(c->minv_dev);
     (void)hipFree(c->iq_ops);
     (void)hipFree(c->iq_partial);
+    (void)hipFree(c->vr_partial);
     if (c->timing.a) (void)hipEventDestroy(c->timing.a);
     if (c->timing.b) (void)hipEventDestroy(c->timing.b);
     delete c;
@@ -370,7 +430,8 @@ int dgx_get_neighbors(const dgx_ctx *cc, int32_t *host_nbr)
     dgx_ctx *c = const_cast<dgx_ctx *>(cc);
     if (!c || !host_nbr) return DGX_ERR_INVALID;
     // copy back what the kernels read, so integer parity is checked on the device copy
-    HIPCHK(hipMemcpy(host_nbr, c->nbr_dev, sizeof(int) * c->nbr_host.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(host_nbr, c->nbr_dev, sizeof(int) * c->nbr_count, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return DGX_OK;
 }
 
@@ -589,7 +650,8 @@ int dgx_allocate_ode_system(dgx_ctx *c, int method)
     if (c->batch_locked) return fail(c, DGX_ERR_STATE, "instance registers belong to a batch graph");
     Butcher B;
     Lsrk L;
-    for (double *r : c->regs) (void)hipFree(r);
+    if (!c->regs.empty()) HIPCHK(hipStreamSynchronize(c->stream));  // the old registers may be in use
+    for (double *r : c->regs) dev_free(c, r);
     c->regs.clear();
     int nregs;
     if (get_butcher(method, B)) nregs = 3 + B.s;   // acc, stage x2, k_j (only if referenced)
@@ -597,13 +659,14 @@ int dgx_allocate_ode_system(dgx_ctx *c, int method)
     else return fail(c, DGX_ERR_INVALID, "unknown ODE method");
     const size_t bytes = sizeof(double) * (size_t)c->ndofs;
     for (int r = 0; r < nregs; ++r) {
-        double *p = nullptr;
-        HIPCHK(hipMalloc(&p, bytes));
-        HIPCHK(hipMemset(p, 0, bytes));
+        double *p = static_cast<double *>(dev_alloc(c, bytes));
+        if (!p) return fail(c, c->arena ? DGX_ERR_STATE : DGX_ERR_HIP, "RK register allocation failed");
         c->regs.push_back(p);
+        HIPCHK(hipMemsetAsync(p, 0, bytes, c->stream));
     }
     c->method = method;
     c->lsrk_cur = 0;
+    c->epsilon[0] = c->epsilon[1] = c->epsilon[2] = 1.0;
     return DGX_OK;
 }
 
@@ -612,6 +675,18 @@ int dgx_set_track_error_estimate(dgx_ctx *c, int enable)
     if (!c) return DGX_ERR_INVALID;
     c->track_error = enable != 0;
     return DGX_OK;
+}
+
+// rk_order of the low-storage methods (parameters_ode_solver.cpp:270-296)
+static int lsrk_order(int m)
+{
+    switch (m) {
+    case DGX_RK4_3_5_3SSTAR: return 4;
+    case DGX_RK3_2_5F_3SSTARPLUS: return 3;
+    case DGX_RK4_3_9F_3SSTARPLUS: return 4;
+    case DGX_RK5_4_10F_3SSTARPLUS: return 5;
+    default: return 0;
+    }
 }
 
 // Registers of explicit RK: 0 accumulator (solution_update), 1/2 stage
@@ -742,6 +817,72 @@ int dgx_step_in_time(dgx_ctx *c, double dt, int n_steps)
     return check_nonphysical(c, "step_in_time");
 }
 
+int dgx_set_error_adaptive_parameters(dgx_ctx *c, double atol, double rtol, double beta1, double beta2, double beta3)
+{
+    if (!c || !(atol >= 0.0) || !(rtol >= 0.0) || !(atol + rtol > 0.0)) return DGX_ERR_INVALID;
+    c->atol = atol;
+    c->rtol = rtol;
+    c->beta_pid[0] = beta1;
+    c->beta_pid[1] = beta2;
+    c->beta_pid[2] = beta3;
+    c->epsilon[0] = c->epsilon[1] = c->epsilon[2] = 1.0;  // as constructed (low_storage_..._solver.cpp:12)
+    c->track_error = true;  // 3S*+ methods: the FSAL residual feeds the embedded register
+    return DGX_OK;
+}
+
+int dgx_get_automatic_initial_step_size(dgx_ctx *c, double *dt)
+{
+    if (!c || !dt) return DGX_ERR_INVALID;
+    Lsrk L;
+    if (c->method < 0 || !get_lsrk(c->method, L)) return fail(c, DGX_ERR_STATE, "needs a low-storage RK system");
+    // the registers S2..S4 are rewritten by the next step before they are read: scratch here
+    double *un = c->regs[1], *f0 = c->regs[2], *f1 = c->regs[3];
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0, h0, h1;
+    int rc = launch_maxabs(c, c->u_dev, nullptr, c->ndofs, &d0);
+    if (!rc) rc = residual_into(c, c->u_dev, f0, MODE_RHS);
+    if (!rc) rc = launch_maxabs(c, f0, nullptr, c->ndofs, &d1);
+    if (rc) return rc;
+    h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+    rc = launch_axpy(c, c->u_dev, h0, f0, un, c->ndofs);
+    if (!rc) rc = residual_into(c, un, f1, MODE_RHS);
+    if (!rc) rc = launch_maxabs(c, f1, f0, c->ndofs, &d2);
+    if (rc) return rc;
+    d2 /= h0;
+    if (std::max(d1, d2) <= 1e-15) h1 = std::max(1e-6, h0 * 1e-3);
+    else h1 = 0.01 / std::max(d1, d2);
+    *dt = std::min(100 * h0, h1);
+    return DGX_OK;
+}
+
+int dgx_get_error_adaptive_step_size(dgx_ctx *c, double dt, double *dt_next)
+{
+    if (!c || !dt_next) return DGX_ERR_INVALID;
+    Lsrk L;
+    if (c->method < 0 || !get_lsrk(c->method, L)) return fail(c, DGX_ERR_STATE, "needs a low-storage RK system");
+    if (L.plus && !c->track_error)
+        return fail(c, DGX_ERR_STATE, "3S*+ error estimate needs dgx_set_error_adaptive_parameters before the step");
+    const int num_delta = (c->method == DGX_RK4_3_5_3SSTAR) ? 7 : L.s;  // parameters_ode_solver.cpp:270-296
+    double sum_delta = 0.0;
+    for (int i = 0; i < num_delta; ++i) sum_delta += L.delta[i];
+    double sum = 0.0;
+    // after the step the solution S1 is in u_dev; S2 = regs[1], S3 = regs[2], S4 = regs[3]
+    int rc = launch_lsrk_error(c, c->u_dev, L.plus ? c->regs[3] : c->regs[1], c->regs[2], L.plus ? 1 : 0,
+                               L.delta[L.s], L.delta[L.s + 1], sum_delta, c->atol, c->rtol, &sum);
+    if (rc) return rc;
+    const double w = std::pow(sum / (double)c->ndofs, 0.5);
+    const double k = lsrk_order(c->method);
+    c->epsilon[2] = c->epsilon[1];
+    c->epsilon[1] = c->epsilon[0];
+    c->epsilon[0] = 1.0 / w;
+    *dt_next = std::pow(c->epsilon[0], 1.0 * c->beta_pid[0] / k) * std::pow(c->epsilon[1], 1.0 * c->beta_pid[1] / k) *
+               std::pow(c->epsilon[2], 1.0 * c->beta_pid[2] / k) * dt;
+    if (!std::isfinite(*dt_next) || !(*dt_next > 0.0)) {
+        ++c->nonphys_events;
+        return fail(c, DGX_ERR_NONPHYSICAL, "error-adaptive time step is not a positive finite number");
+    }
+    return DGX_OK;
+}
+
 double dgx_current_time(const dgx_ctx *c) { return c ? c->current_time : 0.0; }
 int64_t dgx_current_iteration(const dgx_ctx *c) { return c ? c->iteration : 0; }
 
@@ -762,69 +903,29 @@ int dgx_set_time(dgx_ctx *c, double time, int64_t iteration)
 
 // Initial conditions interpolated at the GLL support points of each cell
 // (set_initial_condition.cpp:45-51): TGV (initial_condition_function.cpp:261-336),
-// isentropic vortex with parameterised strength/advection (:643-690).
-static void node_xyz(const dgx_ctx *c, std::vector<double> &xs, int e, int node, double *x)
-{
-    const int a = node % c->n, b = (node / c->n) % c->n, d = node / (c->n * c->n);
-    const int id[3] = {a, b, d};
-    for (int k = 0; k < 3; ++k)
-        x[k] = (k < c->dim) ? c->P.domain_left + (c->cell_lat[(size_t)e * 3 + k] + xs[id[k]]) * c->h : 0.0;
-}
-
-static int upload_ic(dgx_ctx *c, const std::vector<double> &u)
-{
-    HIPCHK(hipMemcpy(c->u_dev, u.data(), sizeof(double) * c->ndofs, hipMemcpyHostToDevice));
-    c->current_time = 0.0;
-    c->iteration = 0;
-    return DGX_OK;
-}
-
+// isentropic vortex with parameterised strength/advection (:643-690), evaluated on the device.
 int dgx_set_initial_condition_tgv(dgx_ctx *c, int isothermal)
 {
     if (!c) return DGX_ERR_INVALID;
     if (c->dim != 3) return fail(c, DGX_ERR_INVALID, "Taylor-Green vortex is 3D");
-    std::vector<double> xs(c->n), ws(c->n), u((size_t)c->ndofs);
-    gauss_lobatto(c->n, xs.data(), ws.data());
-    const double gam = c->P.gamma_gas, m2 = c->P.mach_inf * c->P.mach_inf;
-    for (int e = 0; e < c->ncells; ++e)
-        for (int i = 0; i < c->Np; ++i) {
-            double x[3];
-            node_xyz(c, xs, e, i, x);
-            const double p = 1.0 / (gam * m2) + (1.0 / 16.0) * (std::cos(2.0 * x[0]) + std::cos(2.0 * x[1])) *
-                                                    (std::cos(2.0 * x[2]) + 2.0);
-            const double rho = isothermal ? p * gam * m2 : 1.0;
-            const double vx = std::sin(x[0]) * std::cos(x[1]) * std::cos(x[2]);
-            const double vy = -std::cos(x[0]) * std::sin(x[1]) * std::cos(x[2]);
-            const double vz = 0.0;
-            const double cons[5] = {rho, rho * vx, rho * vy, rho * vz,
-                                    p / (gam - 1.0) + 0.5 * rho * (vx * vx + vy * vy + vz * vz)};
-            for (int s = 0; s < 5; ++s) u[((size_t)e * 5 + s) * c->Np + i] = cons[s];
-        }
-    return upload_ic(c, u);
+    int rc = launch_ic(c, 0, isothermal, 0.0, 0.0, 0.0);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->current_time = 0.0;
+    c->iteration = 0;
+    return DGX_OK;
 }
 
 int dgx_set_initial_condition_vortex(dgx_ctx *c, double strength, double u_inf, double v_inf)
 {
     if (!c) return DGX_ERR_INVALID;
     if (c->dim != 2) return fail(c, DGX_ERR_INVALID, "isentropic vortex is 2D");
-    std::vector<double> xs(c->n), ws(c->n), u((size_t)c->ndofs);
-    gauss_lobatto(c->n, xs.data(), ws.data());
-    const double gam = 1.4;
-    const double M = std::sqrt(2 / gam);
-    const double beta = M * strength * std::sqrt(2.0) / 4.0 / M_PI * std::exp(1.0 / 2.0);
-    for (int e = 0; e < c->ncells; ++e)
-        for (int i = 0; i < c->Np; ++i) {
-            double x[3];
-            node_xyz(c, xs, e, i, x);
-            const double Om = beta * std::exp(-0.5 * (x[0] * x[0] + x[1] * x[1]));
-            const double dT = -(gam - 1.0) / 2.0 * Om * Om;
-            const double rho = std::pow(1 + dT, 1.0 / (gam - 1.0));
-            const double vx = u_inf - x[1] * Om, vy = v_inf + x[0] * Om;
-            const double p = 1.0 / gam * std::pow(1 + dT, gam / (gam - 1.0));
-            const double cons[4] = {rho, rho * vx, rho * vy, p / (gam - 1.0) + 0.5 * rho * (vx * vx + vy * vy)};
-            for (int s = 0; s < 4; ++s) u[((size_t)e * 4 + s) * c->Np + i] = cons[s];
-        }
-    return upload_ic(c, u);
+    int rc = launch_ic(c, 1, 0, strength, u_inf, v_inf);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->current_time = 0.0;
+    c->iteration = 0;
+    return DGX_OK;
 }
 
 int dgx_integrated_quantities(dgx_ctx *c, int overint, double *out4)
@@ -1263,6 +1364,7 @@ int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
 struct dgx_server {
     int64_t budget = 0, used = 0;
+    dgx::Arena arena;               // device memory of the active requests (sized to the budget)
     int max_active = 0, method = -1;
     Butcher B{};
     hipStream_t stream = nullptr;
@@ -1292,28 +1394,36 @@ static int64_t request_ndofs(const dgx_params &p)
         }                                                                                \
     } while (0)
 
+// Admits queued requests first come first served while a slot is free and their footprint fits
+// the budget.  Their buffers come from the server's arena and their connectivity and initial
+// data are computed by kernels on the server stream, so admission neither allocates device
+// memory nor waits for the requests already on the device.
 static int server_admit(dgx_server *s)
 {
     while (!s->queue.empty() && (int)s->active.size() < s->max_active && s->used + s->queue.front().bytes <= s->budget) {
         Request q = s->queue.front();
-        s->queue.erase(s->queue.begin());
         dgx_ctx *c = nullptr;
-        int rc = dgx_create(&q.r.params, &c);
+        int rc = create_ctx(&q.r.params, &s->arena, s->stream, &c);
+        if (rc == DGX_ERR_STATE) break;  // no contiguous range free in the arena yet: wait for retirements
         if (rc) {
-            s->err = "dgx_create failed for an admitted request";
+            s->err = "context set-up failed for an admitted request";
             return rc;
         }
-        c->stream = s->stream;
         rc = (q.r.initial_condition == DGX_IC_VORTEX)
-                 ? dgx_set_initial_condition_vortex(c, q.r.vortex_strength, q.r.vortex_u, q.r.vortex_v)
-                 : dgx_set_initial_condition_tgv(c, q.r.tgv_isothermal);
+                 ? launch_ic(c, 1, 0, q.r.vortex_strength, q.r.vortex_u, q.r.vortex_v)
+                 : ((c->dim == 3) ? launch_ic(c, 0, q.r.tgv_isothermal, 0.0, 0.0, 0.0)
+                                  : fail(c, DGX_ERR_INVALID, "Taylor-Green vortex is 3D"));
         if (!rc) rc = dgx_allocate_ode_system(c, s->method);
+        if (rc == DGX_ERR_STATE && c->arena) {  // registers did not fit: undo, wait for retirements
+            dgx_destroy(c);
+            break;
+        }
         if (rc) {
             s->err = c->err;
-            c->stream = nullptr;
             dgx_destroy(c);
             return rc;
         }
+        s->queue.erase(s->queue.begin());
         const double L = q.r.params.domain_right - q.r.params.domain_left;
         const double dx = L / std::pow((double)(unsigned)(c->ndofs / c->ns), 1.0 / c->dim);
         q.dt = q.r.cfl * dx / q.r.wave_speed;
@@ -1473,8 +1583,7 @@ static int server_retire(dgx_server *s)
         res.steps = q.steps;
         res.dof_updates = (double)q.c->ndofs * s->B.s * q.steps;
         s->done[q.id] = std::move(res);
-        q.c->stream = nullptr;
-        dgx_destroy(q.c);
+        dgx_destroy(q.c);  // buffers back to the arena
         s->used -= q.bytes;
         s->active.erase(s->active.begin() + k);
     }
@@ -1512,6 +1621,12 @@ int dgx_server_create(int64_t hbm_budget_bytes, int max_active, int method, dgx_
         delete s;
         return DGX_ERR_HIP;
     }
+    // the budget plus room for the per-request status words and 256-byte alignment of each buffer
+    if (s->arena.init(hbm_budget_bytes + (16ll << 20) + 8192ll * max_active) != DGX_OK) {
+        (void)hipStreamDestroy(s->stream);
+        delete s;
+        return DGX_ERR_HIP;
+    }
     *out = s;
     return DGX_OK;
 }
@@ -1520,10 +1635,8 @@ int dgx_server_destroy(dgx_server *s)
 {
     if (!s) return DGX_OK;
     (void)hipStreamSynchronize(s->stream);
-    for (Request &q : s->active) {
-        q.c->stream = nullptr;
-        dgx_destroy(q.c);
-    }
+    for (Request &q : s->active) dgx_destroy(q.c);
+    s->arena.release();
     (void)hipHostFree(s->stage_host);
     (void)hipFree(s->stage_dev);
     (void)hipStreamDestroy(s->stream);

@@ -51,14 +51,14 @@ void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
             hipLaunchKernelGGL((stage2_kernel<D, N, false, true, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
         else if (split) {
             if constexpr (N == 7 && std::is_same<T, float>::value)
-                hipLaunchKernelGGL((line_kernel_w4<D, N, RT, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
+                hipLaunchKernelGGL((line_kernel_w4<D, N, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
             else
-                hipLaunchKernelGGL((line_kernel<D, N, RT, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
+                hipLaunchKernelGGL((line_kernel<D, N, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
         } else
             hipLaunchKernelGGL((stage2_kernel<D, N, false, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
     } else {
         if (split)
-            hipLaunchKernelGGL((line_kernel<D, N, RT, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
+            hipLaunchKernelGGL((line_kernel<D, N, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
         else
             hipLaunchKernelGGL((stage2_kernel<D, N, false, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
     }
@@ -123,10 +123,10 @@ int line_batch_es(const BatchBlock *blocks, int nblocks, const StageArgs *args, 
         if (epb) *epb = LG::EPB;
         if (!blocks) return DGX_OK;  // query only
         if constexpr (D == 2 && N < 7 && std::is_same<T, float>::value)
-            hipLaunchKernelGGL((line_kernel_batch_w5<D, N, RT, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
+            hipLaunchKernelGGL((line_kernel_batch_w5<D, N, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
                                args, pcs, ops);
         else
-            hipLaunchKernelGGL((line_kernel_batch<D, N, RT, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
+            hipLaunchKernelGGL((line_kernel_batch<D, N, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
                                args, pcs, ops);
         return hipGetLastError() == hipSuccess ? DGX_OK : DGX_ERR_HIP;
     }

@@ -1,10 +1,13 @@
-// Float point physics of the mixed-precision path, direction K fixed at
-// compile time.  Each function is an algebraic restatement of the FP64 point
-// physics in dg_physics.hpp (same formula in exact arithmetic) arranged to
-// use few divisions: per-node quantities (1/rho, pressure, beta, sqrt(rho),
-// sound speed, enthalpy) are computed once per node and reused by every pair
-// and face that touches the node, and the reciprocals go through the
-// hardware v_rcp_f32 / v_sqrt_f32 / v_log_f32 instructions.
+// Cached-node point physics of the line kernels, direction K fixed at compile
+// time, in the arithmetic type T: float on the mixed path, double on the FP64
+// path.  Each function is an algebraic restatement of the point physics in
+// dg_physics.hpp (same formula in exact arithmetic) arranged to use few
+// divisions: per-node quantities (1/rho, pressure, beta, sqrt(rho), sound
+// speed, enthalpy) are computed once per node and reused by every pair and
+// face that touches the node.  In float the reciprocals, square roots and
+// logarithms are the hardware v_rcp_f32 / v_sqrt_f32 / v_log_f32 instructions;
+// in double they are the IEEE operations (correctly rounded division and
+// square root).
 //
 //   ec_flux_fast   Chandrashekar / Ranocha two-point flux (euler.cpp:709-807):
 //                  log means from f = (a-b)/(a+b), and 1/(2 beta_ln (gamma-1))
@@ -19,111 +22,131 @@
 
 namespace dgx {
 
+template <typename T> struct Fast;
 #ifndef DGX_HOST_TEST
-__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ float flog(float x) { return __logf(x); }
+template <> struct Fast<float> {
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float log_(float x) { return __logf(x); }
+};
+template <> struct Fast<double> {
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+    static __device__ __forceinline__ double log_(double x) { return log(x); }
+};
 #else
 // host build of the same formulas (CPU checks of the restatement)
-inline float frcp(float x) { return 1.0f / x; }
-inline float fsqrt(float x) { return std::sqrt(x); }
-inline float flog(float x) { return std::log(x); }
+template <typename T> struct FastHost {
+    static inline T rcp(T x) { return T(1) / x; }
+    static inline T sqrt_(T x) { return std::sqrt(x); }
+    static inline T log_(T x) { return std::log(x); }
+};
+template <> struct Fast<float> : FastHost<float> {};
+template <> struct Fast<double> : FastHost<double> {};
 #endif
+template <typename T> __device__ __forceinline__ T frcp(T x) { return Fast<T>::rcp(x); }
+template <typename T> __device__ __forceinline__ T fsqrt(T x) { return Fast<T>::sqrt_(x); }
+template <typename T> __device__ __forceinline__ T flog(T x) { return Fast<T>::log_(x); }
+template <typename T> __device__ __forceinline__ T fmx(T a, T b) { return a > b ? a : b; }
+template <typename T> __device__ __forceinline__ T fab(T a) { return a < T(0) ? -a : a; }
 
 // Per-node cache: conservative state and derived quantities.
-template <int D> struct NodeF {
-    float u[D + 2];
-    float vel[D];
-    float p, b, v2, sr, c, h;  // pressure, beta, |v|^2, sqrt(rho), sound speed, total enthalpy
+template <typename T, int D> struct NodeC {
+    T u[D + 2];
+    T vel[D];
+    T p, b, v2, sr, c, h;  // pressure, beta, |v|^2, sqrt(rho), sound speed, total enthalpy
 };
+template <int D> using NodeF = NodeC<float, D>;
 template <int D> constexpr int node_f_count() { return 2 * D + 8; }
 
-struct FastConst {
-    float gam, gamm1, inv_gamm1;
+template <typename T> struct FastConst {
+    T gam, gamm1, inv_gamm1;
     int two_point, conv_flux;
 };
 
-__device__ __forceinline__ FastConst fast_const(const PhysConst &pc)
+template <typename T> __device__ __forceinline__ FastConst<T> fast_const(const PhysConst &pc)
 {
-    FastConst f;
-    f.gam = float(pc.gam);
-    f.gamm1 = float(pc.gamm1);
-    f.inv_gamm1 = float(1.0 / pc.gamm1);
+    FastConst<T> f;
+    f.gam = T(pc.gam);
+    f.gamm1 = T(pc.gamm1);
+    f.inv_gamm1 = T(1.0 / pc.gamm1);
     f.two_point = pc.two_point;
     f.conv_flux = pc.conv_flux;
     return f;
 }
 
 // Fills every field of q from q.u; beta is rho/(2p) (Chandrashekar) or rho/p (Ranocha).
-template <int D>
-__device__ __forceinline__ void node_fill(NodeF<D> &q, const FastConst &fc)
+template <typename T, int D>
+__device__ __forceinline__ void node_fill(NodeC<T, D> &q, const FastConst<T> &fc)
 {
-    const float ir = frcp(q.u[0]);
-    float v2 = 0.f;
+    const T ir = frcp<T>(q.u[0]);
+    T v2 = T(0);
     for (int k = 0; k < D; ++k) {
         q.vel[k] = q.u[1 + k] * ir;
         v2 += q.vel[k] * q.vel[k];
     }
     q.v2 = v2;
-    q.p = fc.gamm1 * (q.u[D + 1] - 0.5f * q.u[0] * v2);
-    const float rp = q.u[0] * frcp(q.p);
-    q.b = (fc.two_point == 3) ? rp : 0.5f * rp;
-    q.sr = fsqrt(q.u[0]);
-    q.c = fsqrt(fc.gam * q.p * ir);
+    q.p = fc.gamm1 * (q.u[D + 1] - T(0.5) * q.u[0] * v2);
+    const T rp = q.u[0] * frcp<T>(q.p);
+    q.b = (fc.two_point == 3) ? rp : T(0.5) * rp;
+    q.sr = fsqrt<T>(q.u[0]);
+    q.c = fsqrt<T>(fc.gam * q.p * ir);
     q.h = (q.u[D + 1] + q.p) * ir;
 }
 
-// F of the Ismail-Roe log mean  mean_ln(a, b) = (a + b) / (2 F),  f = (a - b) / (a + b).
-__device__ __forceinline__ float log_mean_F(float a, float b, float f)
+// F of the Ismail-Roe log mean  mean_ln(a, b) = (a + b) / (2 F),  f = (a - b) / (a + b)
+// (series branch below 1e-2 as euler.cpp:629-647, Horner form with constant reciprocals).
+template <typename T>
+__device__ __forceinline__ T log_mean_F(T a, T b, T f)
 {
-    const float uu = f * f;
-    if (uu < 1.0e-2f) return 1.f + uu * (1.f / 3.f) + uu * uu * (1.f / 5.f) + uu * uu * uu * (1.f / 7.f);
-    return 0.5f * flog(a * frcp(b)) * frcp(f);
+    const T uu = f * f;
+    if (uu < T(1.0e-2)) return T(1) + uu * (T(1.0 / 3.0) + uu * (T(1.0 / 5.0) + uu * T(1.0 / 7.0)));
+    return T(0.5) * flog<T>(a * frcp<T>(b)) * frcp<T>(f);
 }
 
-template <int D, int K>
-__device__ __forceinline__ void ec_flux_fast(const float rho1, const float *v1, const float p1, const float b1,
-                                             const float q1v2, const float rho2, const float *v2, const float p2,
-                                             const float b2, const float q2v2, const FastConst &fc, float *f)
+template <typename T, int D, int K>
+__device__ __forceinline__ void ec_flux_fast(const T rho1, const T *v1, const T p1, const T b1, const T q1v2,
+                                             const T rho2, const T *v2, const T p2, const T b2, const T q2v2,
+                                             const FastConst<T> &fc, T *f)
 {
     const bool ra = (fc.two_point == 3);
-    const float sr = rho1 + rho2, isr = frcp(sr);
-    const float Fr = log_mean_F(rho1, rho2, (rho1 - rho2) * isr);
-    const float sb = b1 + b2, isb = frcp(sb);
-    const float Fb = log_mean_F(b1, b2, (b1 - b2) * isb);
-    const float rho_ln = 0.5f * sr * frcp(Fr);
-    float va[D], vsq = 0.f;
+    const T sr = rho1 + rho2, isr = frcp<T>(sr);
+    const T Fr = log_mean_F<T>(rho1, rho2, (rho1 - rho2) * isr);
+    const T sb = b1 + b2, isb = frcp<T>(sb);
+    const T Fb = log_mean_F<T>(b1, b2, (b1 - b2) * isb);
+    const T rho_ln = T(0.5) * sr * frcp<T>(Fr);
+    T va[D], vsq = T(0);
     for (int k = 0; k < D; ++k) {
-        va[k] = 0.5f * (v1[k] + v2[k]);
+        va[k] = T(0.5) * (v1[k] + v2[k]);
         vsq += va[k] * va[k];
     }
-    float p_hat, H;
+    T p_hat, H;
     if (ra) {
-        p_hat = 0.5f * (p1 + p2);
-        H = 2.f * Fb * isb * fc.inv_gamm1 + 4.f * p_hat * Fr * isr;
+        p_hat = T(0.5) * (p1 + p2);
+        H = T(2) * Fb * isb * fc.inv_gamm1 + T(4) * p_hat * Fr * isr;
     } else {
-        p_hat = 0.5f * sr * isb;
+        p_hat = T(0.5) * sr * isb;
         H = isb * (Fb * fc.inv_gamm1 + Fr);
     }
-    H += vsq - 0.25f * (q1v2 + q2v2);
-    const float m = rho_ln * va[K];
+    H += vsq - T(0.25) * (q1v2 + q2v2);
+    const T m = rho_ln * va[K];
     f[0] = m;
     for (int vd = 0; vd < D; ++vd) f[1 + vd] = m * va[vd];
     f[1 + K] += p_hat;
     f[D + 1] = m * H;
-    if (ra) f[D + 1] -= 0.5f * (p1 * v1[K] + p2 * v2[K]);
+    if (ra) f[D + 1] -= T(0.5) * (p1 * v1[K] + p2 * v2[K]);
 }
 
 // Two-point flux of two cached nodes; Kennedy-Gruber and Ismail-Roe go through
-// the generic float evaluation.
-template <int D, int K>
-__device__ __forceinline__ void pair_flux_fast(const NodeF<D> &a, const NodeF<D> &b, const FastConst &fc,
-                                               const PhysConst &pc, float *f)
+// the generic evaluation of dg_physics.hpp.
+template <typename T, int D, int K>
+__device__ __forceinline__ void pair_flux_fast(const NodeC<T, D> &a, const NodeC<T, D> &b, const FastConst<T> &fc,
+                                               const PhysConst &pc, T *f)
 {
     if (fc.two_point >= 2) {
-        ec_flux_fast<D, K>(a.u[0], a.vel, a.p, a.b, a.v2, b.u[0], b.vel, b.p, b.b, b.v2, fc, f);
+        ec_flux_fast<T, D, K>(a.u[0], a.vel, a.p, a.b, a.v2, b.u[0], b.vel, b.p, b.b, b.v2, fc, f);
     } else {
-        Prim<float, D> qa, qb;
+        Prim<T, D> qa, qb;
         qa.rho = a.u[0];
         qb.rho = b.u[0];
         for (int k = 0; k < D; ++k) {
@@ -132,79 +155,79 @@ __device__ __forceinline__ void pair_flux_fast(const NodeF<D> &a, const NodeF<D>
         }
         qa.p = a.p;
         qb.p = b.p;
-        two_point_flux_dir<float, D>(a.u, qa, b.u, qb, K, pc, f);
+        two_point_flux_dir<T, D>(a.u, qa, b.u, qb, K, pc, f);
     }
 }
 
-template <int D, int K>
-__device__ __forceinline__ void roe_fast(const NodeF<D> &L, const NodeF<D> &R, float sgn, const FastConst &fc,
-                                         float *out)
+template <typename T, int D, int K>
+__device__ __forceinline__ void roe_fast(const NodeC<T, D> &L, const NodeC<T, D> &R, T sgn, const FastConst<T> &fc,
+                                         T *out)
 {
-    const float vnL = L.vel[K] * sgn, vnR = R.vel[K] * sgn;
-    const float isum = frcp(L.sr + R.sr);
-    const float rho_ravg = L.sr * R.sr;
-    float v_ravg[D], vel2 = 0.f;
+    const T vnL = L.vel[K] * sgn, vnR = R.vel[K] * sgn;
+    const T isum = frcp<T>(L.sr + R.sr);
+    const T rho_ravg = L.sr * R.sr;
+    T v_ravg[D], vel2 = T(0);
     for (int k = 0; k < D; ++k) {
         v_ravg[k] = (R.sr * R.vel[k] + L.sr * L.vel[k]) * isum;
         vel2 += v_ravg[k] * v_ravg[k];
     }
-    const float h_ravg = (R.sr * R.h + L.sr * L.h) * isum;
-    const float vn = v_ravg[K] * sgn;
-    const float c2 = fc.gamm1 * (h_ravg - 0.5f * vel2);
-    const float c = (c2 > 0.f) ? fsqrt(c2) : 1e10f;
-    float eig[3] = {fabsf(vn - c), fabsf(vn), fabsf(vn + c)};
-    const float eL[3] = {fabsf(vnL - L.c), fabsf(vnL), fabsf(vnL + L.c)};
-    const float eR[3] = {fabsf(vnR - R.c), fabsf(vnR), fabsf(vnR + R.c)};
+    const T h_ravg = (R.sr * R.h + L.sr * L.h) * isum;
+    const T vn = v_ravg[K] * sgn;
+    const T c2 = fc.gamm1 * (h_ravg - T(0.5) * vel2);
+    const T c = (c2 > T(0)) ? fsqrt<T>(c2) : T(1e10);
+    T eig[3] = {fab<T>(vn - c), fab<T>(vn), fab<T>(vn + c)};
+    const T eL[3] = {fab<T>(vnL - L.c), fab<T>(vnL), fab<T>(vnL + L.c)};
+    const T eR[3] = {fab<T>(vnR - R.c), fab<T>(vnR), fab<T>(vnR + R.c)};
     for (int e = 0; e < 3; ++e) {
-        const float eps = fmaxf(fabsf(eig[e] - eL[e]), fabsf(eR[e] - eig[e]));
-        if (eig[e] < eps) eig[e] = 0.5f * (eig[e] * eig[e] * frcp(eps) + eps);
+        const T eps = fmx<T>(fab<T>(eig[e] - eL[e]), fab<T>(eR[e] - eig[e]));
+        if (eig[e] < eps) eig[e] = T(0.5) * (eig[e] * eig[e] * frcp<T>(eps) + eps);
     }
-    const float dp = R.p - L.p, drho = R.u[0] - L.u[0];
-    const float dVn = vnR - vnL;
-    const float hic2 = 0.5f * frcp(c2);
-    const float rc = rho_ravg * c * dVn;
-    const float co0 = eig[0] * (dp - rc) * hic2;
-    const float co1 = eig[1] * (drho - 2.f * dp * hic2);
-    const float co2 = eig[1] * rho_ravg;
-    const float co3 = eig[2] * (dp + rc) * hic2;
+    const T dp = R.p - L.p, drho = R.u[0] - L.u[0];
+    const T dVn = vnR - vnL;
+    const T hic2 = T(0.5) * frcp<T>(c2);
+    const T rc = rho_ravg * c * dVn;
+    const T co0 = eig[0] * (dp - rc) * hic2;
+    const T co1 = eig[1] * (drho - T(2) * dp * hic2);
+    const T co2 = eig[1] * rho_ravg;
+    const T co3 = eig[2] * (dp + rc) * hic2;
     // tangential velocity jump: the normal component cancels exactly
-    float dvt_dot = 0.f;
-    float A[D + 2];
+    T dvt_dot = T(0);
+    T A[D + 2];
     A[0] = co0 + co1 + co3;
     for (int k = 0; k < D; ++k) {
-        const float dvt = (k == K) ? 0.f : (R.vel[k] - L.vel[k]);
+        const T dvt = (k == K) ? T(0) : (R.vel[k] - L.vel[k]);
         dvt_dot += v_ravg[k] * dvt;
-        const float nk = (k == K) ? sgn : 0.f;
+        const T nk = (k == K) ? sgn : T(0);
         A[1 + k] = co0 * (v_ravg[k] - c * nk) + co1 * v_ravg[k] + co2 * dvt + co3 * (v_ravg[k] + c * nk);
     }
-    A[D + 1] = co0 * (h_ravg - c * vn) + co1 * vel2 * 0.5f + co2 * dvt_dot + co3 * (h_ravg + c * vn);
-    for (int s = 0; s < D + 2; ++s) out[s] = -0.5f * A[s];
+    A[D + 1] = co0 * (h_ravg - c * vn) + co1 * vel2 * T(0.5) + co2 * dvt_dot + co3 * (h_ravg + c * vn);
+    for (int s = 0; s < D + 2; ++s) out[s] = T(-0.5) * A[s];
 }
 
 // Numerical flux of a face node along the normal sgn * e_K, L the interior
 // (owner) side.
-template <int D, int K>
-__device__ __forceinline__ void num_flux_fast(const NodeF<D> &L, const NodeF<D> &R, float sgn, const FastConst &fc,
-                                              const PhysConst &pc, float *out)
+template <typename T, int D, int K>
+__device__ __forceinline__ void num_flux_fast(const NodeC<T, D> &L, const NodeC<T, D> &R, T sgn,
+                                              const FastConst<T> &fc, const PhysConst &pc, T *out)
 {
     const int cf = fc.conv_flux;
-    float base[D + 2];
+    T base[D + 2];
     if (cf >= 3) {
-        pair_flux_fast<D, K>(L, R, fc, pc, base);
+        pair_flux_fast<T, D, K>(L, R, fc, pc, base);
         for (int s = 0; s < D + 2; ++s) base[s] *= sgn;
     } else {
-        float f1[D + 2], f2[D + 2];
-        convective_flux_dir<float, D>(L.u, K, fc.gamm1, f1);
-        convective_flux_dir<float, D>(R.u, K, fc.gamm1, f2);
-        for (int s = 0; s < D + 2; ++s) base[s] = (0.5f * (f1[s] + f2[s])) * sgn;
+        T f1[D + 2], f2[D + 2];
+        convective_flux_dir<T, D>(L.u, K, fc.gamm1, f1);
+        convective_flux_dir<T, D>(R.u, K, fc.gamm1, f2);
+        for (int s = 0; s < D + 2; ++s) base[s] = (T(0.5) * (f1[s] + f2[s])) * sgn;
     }
-    float diss[D + 2];
-    for (int s = 0; s < D + 2; ++s) diss[s] = 0.f;
+    T diss[D + 2];
+    for (int s = 0; s < D + 2; ++s) diss[s] = T(0);
     if (cf == 0 || cf == 4) {
-        const float lam = fmaxf(fabsf(L.vel[K]) + L.c, fabsf(R.vel[K]) + R.c);
-        for (int s = 0; s < D + 2; ++s) diss[s] = -0.5f * lam * (R.u[s] - L.u[s]);
+        const T lam = fmx<T>(fab<T>(L.vel[K]) + L.c, fab<T>(R.vel[K]) + R.c);
+        for (int s = 0; s < D + 2; ++s) diss[s] = T(-0.5) * lam * (R.u[s] - L.u[s]);
     }
-    if (cf == 1 || cf == 5) roe_fast<D, K>(L, R, sgn, fc, diss);
+    if (cf == 1 || cf == 5) roe_fast<T, D, K>(L, R, sgn, fc, diss);
     for (int s = 0; s < D + 2; ++s) out[s] = base[s] + diss[s];
 }
 

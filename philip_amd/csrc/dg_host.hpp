@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -25,10 +26,25 @@ void lagrange(int n, const double *s, double x, double *val, double *der);
 // Operators of a collocated GLL element of n nodes.
 Ops1D build_ops(int n);
 
-// Cell order of straight_periodic_cube: Z order (refine_global) when n_cells_1d
-// is a power of two, lexicographic (subdivided_hyper_cube) otherwise.
-void cell_ijk(int dim, int n1d, int order, std::vector<int> &ijk);
-void neighbors(int dim, int n1d, const std::vector<int> &ijk, std::vector<int> &nbr);
+
+// First-fit sub-allocator over one device allocation (a server's HBM budget): request
+// buffers are carved from it and returned to it, so admitting and retiring requests calls
+// neither hipMalloc nor hipFree.  Free ranges are kept coalesced.  Not thread-safe.
+class Arena {
+public:
+    int init(int64_t bytes);            // one hipMalloc; DGX_OK or DGX_ERR_HIP
+    void release();                     // hipFree
+    void *alloc(int64_t bytes);         // 256-byte aligned; nullptr when no free range fits
+    void free(void *p);
+    int64_t in_use() const { return used_bytes_; }
+    int64_t capacity() const { return size_; }
+
+private:
+    char *base_ = nullptr;
+    int64_t size_ = 0, used_bytes_ = 0;
+    std::map<int64_t, int64_t> free_;   // offset -> length
+    std::map<int64_t, int64_t> used_;   // offset -> length
+};
 
 struct Timing {
     bool enabled = false;
@@ -46,8 +62,7 @@ struct dgx_ctx {
     double h = 0.0;
     dgx::Ops1D ops{};
     dgx::PhysConst pc{};
-    std::vector<int> nbr_host;
-    std::vector<int> cell_lat;     // [cell][3] lattice coordinates
+    int64_t nbr_count = 0;         // ncells * 2 * dim
     int *nbr_dev = nullptr;
     double *u_dev = nullptr;       // DGBase::solution
     double *r_dev = nullptr;       // DGBase::right_hand_side
@@ -69,12 +84,18 @@ struct dgx_ctx {
     dgx::Timing timing;
     int device = 0;
     int n_cu = 0;                  // compute units of the device
+    dgx::Arena *arena = nullptr;   // set: solution, residual, connectivity, gradient and RK
+                                   // registers come from this arena (serving), else hipMalloc
     bool batch_locked = false;     // RK registers captured in a batch graph: not re-allocatable
     // integrated-quantity operators (QGauss(p+1+overint))
     int iq_overint = -1;
     double *iq_ops = nullptr;      // B [nx][n], G [nx][n], w [nx]
     double *iq_partial = nullptr;  // per-block partial sums
     int iq_blocks = 0;
+    // error-adaptive low-storage RK (low_storage_runge_kutta_ode_solver.cpp:89-118)
+    double *vr_partial = nullptr;  // per-block partials of the vector reductions
+    double atol = 1e-3, rtol = 1e-3, beta_pid[3] = {0.70, -0.23, 0.0};
+    double epsilon[3] = {1.0, 1.0, 1.0};
 };
 
 // kernel launch helpers (dg_launch.hip)
@@ -82,6 +103,12 @@ namespace dgx {
 int launch_stage(dgx_ctx *c, const StageArgs &a);
 int launch_inverse_mass(dgx_ctx *c, const double *in, double *out);
 int launch_fill(dgx_ctx *c, double *out, int64_t n, double value);
+int launch_mesh(dgx_ctx *c);
+int launch_ic(dgx_ctx *c, int kind, int isothermal, double strength, double u_inf, double v_inf);
+int launch_maxabs(dgx_ctx *c, const double *a, const double *b, int64_t n, double *out);
+int launch_axpy(dgx_ctx *c, const double *x, double h, const double *f, double *out, int64_t n);
+int launch_lsrk_error(dgx_ctx *c, const double *s1, const double *e, const double *s3, int plus, double d5, double d6,
+                      double sum_delta, double atol, double rtol, double *sum_out);
 // Batched split-form line kernel over the element groups `blocks` of several solves of one
 // (dim, n, precision); es = elements-per-block variant (1 or 2).  With blocks == nullptr only
 // reports the variant's elements per block in *epb.  DGX_ERR_UNSUPPORTED if the variant does

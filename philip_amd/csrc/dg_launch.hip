@@ -84,6 +84,266 @@ int launch_fill(dgx_ctx *c, double *out, int64_t n, double value)
     return check_launch(c, "fill_kernel");
 }
 
+// ---------------------------------------------------------------------------
+// Vector reductions of the error-adaptive low-storage RK step
+// (low_storage_runge_kutta_ode_solver.cpp:89-178): per-block partials over a fixed grid, summed
+// (or maxed) in a fixed order on the host, so results are deterministic.
+// ---------------------------------------------------------------------------
+constexpr int VR_BLOCKS = 1024, VR_THREADS = 256;
+
+// partial[b] = max over block b's share of |a - b_| (b_ == nullptr: |a|)
+__global__ void __launch_bounds__(VR_THREADS)
+maxabs_kernel(const double *__restrict__ a, const double *__restrict__ b, long n, double *__restrict__ partial)
+{
+    __shared__ double red[VR_THREADS];
+    double m = 0.0;
+    for (long q = blockIdx.x * (long)VR_THREADS + threadIdx.x; q < n; q += (long)gridDim.x * VR_THREADS) {
+        const double x = b ? fabs(a[q] - b[q]) : fabs(a[q]);
+        m = x > m ? x : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int sh = VR_THREADS / 2; sh > 0; sh >>= 1) {
+        if ((int)threadIdx.x < sh) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + sh]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// out = x + h f  (deal.II Vector::add(h, f) on a copy of x)
+__global__ void __launch_bounds__(256) axpy_kernel(const double *__restrict__ x, double h, const double *__restrict__ f,
+                                                   double *__restrict__ out, long n)
+{
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        out[q] = x[q] + h * f[q];
+}
+
+// partial[b] = block b's share of sum (err / (atol + rtol max(|S1|, |E|)))^2, err = S1 - E, with E the
+// 3S*+ register S4, or (plus == 0) the embedded solution (S2 + d5 S1 + d6 S3) / sum_delta
+__global__ void __launch_bounds__(VR_THREADS)
+lsrk_error_kernel(const double *__restrict__ s1, const double *__restrict__ e, const double *__restrict__ s3, long n,
+                  int plus, double d5, double d6, double sum_delta, double atol, double rtol,
+                  double *__restrict__ partial)
+{
+    __shared__ double red[VR_THREADS];
+    double acc = 0.0;
+    for (long q = blockIdx.x * (long)VR_THREADS + threadIdx.x; q < n; q += (long)gridDim.x * VR_THREADS) {
+        const double a = s1[q];
+        double b = e[q];
+        if (!plus) {
+            b += d5 * a;
+            b += d6 * s3[q];
+            b /= sum_delta;
+        }
+        const double x = (a - b) / (atol + rtol * fmax(fabs(a), fabs(b)));
+        acc += x * x;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int sh = VR_THREADS / 2; sh > 0; sh >>= 1) {
+        if ((int)threadIdx.x < sh) red[threadIdx.x] += red[threadIdx.x + sh];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+static int vr_partials(dgx_ctx *c)
+{
+    if (!c->vr_partial && hipMalloc(&c->vr_partial, sizeof(double) * VR_BLOCKS) != hipSuccess) {
+        c->err = "hipMalloc failed";
+        return DGX_ERR_HIP;
+    }
+    return DGX_OK;
+}
+
+static int vr_fetch(dgx_ctx *c, double *host)
+{
+    if (hipMemcpyAsync(host, c->vr_partial, sizeof(double) * VR_BLOCKS, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        c->err = "reduction copy failed";
+        return DGX_ERR_HIP;
+    }
+    return DGX_OK;
+}
+
+int launch_maxabs(dgx_ctx *c, const double *a, const double *b, int64_t n, double *out)
+{
+    int rc = vr_partials(c);
+    if (rc) return rc;
+    hipLaunchKernelGGL(maxabs_kernel, dim3(VR_BLOCKS), dim3(VR_THREADS), 0, c->stream, a, b, (long)n, c->vr_partial);
+    rc = check_launch(c, "maxabs_kernel");
+    if (rc) return rc;
+    double h[VR_BLOCKS];
+    rc = vr_fetch(c, h);
+    if (rc) return rc;
+    double m = 0.0;
+    for (int k = 0; k < VR_BLOCKS; ++k) m = h[k] > m ? h[k] : m;
+    *out = m;
+    return DGX_OK;
+}
+
+int launch_axpy(dgx_ctx *c, const double *x, double h, const double *f, double *out, int64_t n)
+{
+    const long blocks = std::min<long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, x, h, f, out, (long)n);
+    return check_launch(c, "axpy_kernel");
+}
+
+int launch_lsrk_error(dgx_ctx *c, const double *s1, const double *e, const double *s3, int plus, double d5, double d6,
+                      double sum_delta, double atol, double rtol, double *sum_out)
+{
+    int rc = vr_partials(c);
+    if (rc) return rc;
+    hipLaunchKernelGGL(lsrk_error_kernel, dim3(VR_BLOCKS), dim3(VR_THREADS), 0, c->stream, s1, e, s3, (long)c->ndofs,
+                       plus, d5, d6, sum_delta, atol, rtol, c->vr_partial);
+    rc = check_launch(c, "lsrk_error_kernel");
+    if (rc) return rc;
+    double h[VR_BLOCKS];
+    rc = vr_fetch(c, h);
+    if (rc) return rc;
+    double acc = 0.0;
+    for (int k = 0; k < VR_BLOCKS; ++k) acc += h[k];
+    *sum_out = acc;
+    return DGX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Mesh connectivity and initial data on the device (straight_periodic_cube.cpp:14-67,
+// set_initial_condition.cpp:45-51, initial_condition_function.cpp:261-336, 643-690), so that a
+// context is set up without host-side loops or host-to-device copies of mesh-sized arrays.
+// ---------------------------------------------------------------------------
+struct MeshDesc {
+    int dim, N, morton, levels;
+};
+
+// lattice coordinates of cell e: Z order (refine_global, children x-fastest, coarsest level in
+// the high bits) for 2^k cells per direction in deal.II order, lexicographic otherwise
+__device__ __forceinline__ void cell_coords(const MeshDesc &m, int e, int *c)
+{
+    c[0] = c[1] = c[2] = 0;
+    if (m.morton) {
+        for (int bit = 0; bit < m.levels; ++bit)
+            for (int k = 0; k < m.dim; ++k) c[k] |= ((e >> (bit * m.dim + k)) & 1) << bit;
+    } else {
+        c[0] = e % m.N;
+        c[1] = (e / m.N) % m.N;
+        if (m.dim == 3) c[2] = e / (m.N * m.N);
+    }
+}
+
+__device__ __forceinline__ int cell_index(const MeshDesc &m, const int *c)
+{
+    if (!m.morton) return c[0] + m.N * (c[1] + (m.dim == 3 ? m.N * c[2] : 0));
+    int e = 0;
+    for (int bit = 0; bit < m.levels; ++bit)
+        for (int k = 0; k < m.dim; ++k) e |= ((c[k] >> bit) & 1) << (bit * m.dim + k);
+    return e;
+}
+
+// nbr[e][2k + side]: the periodic neighbour across face 2k (-e_k) / 2k+1 (+e_k)
+__global__ void __launch_bounds__(256) mesh_kernel(int *__restrict__ nbr, int ncells, MeshDesc m)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ncells) return;
+    int c[3];
+    cell_coords(m, e, c);
+    for (int f = 0; f < 2 * m.dim; ++f) {
+        int q[3] = {c[0], c[1], c[2]};
+        const int k = f / 2;
+        q[k] = (q[k] + ((f & 1) ? 1 : m.N - 1)) % m.N;
+        nbr[(long)e * 2 * m.dim + f] = cell_index(m, q);
+    }
+}
+
+struct IcDesc {
+    int kind;          // 0 TGV, 1 isentropic vortex
+    int isothermal;
+    double left, h, gam, mach2;
+    double beta, u_inf, v_inf;
+    double xs[MAXN1D]; // GLL support points on [0, 1]
+};
+
+// initial data interpolated at the GLL support points of each cell, one thread per node
+__global__ void __launch_bounds__(256) ic_kernel(double *__restrict__ u, int ncells, int n, MeshDesc m, IcDesc ic)
+{
+    const int Np = (m.dim == 3) ? n * n * n : n * n, ns = m.dim + 2;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)ncells * Np) return;
+    const int e = (int)(t / Np), i = (int)(t % Np);
+    int c[3];
+    cell_coords(m, e, c);
+    const int id[3] = {i % n, (i / n) % n, i / (n * n)};
+    double x[3];
+    for (int k = 0; k < 3; ++k) x[k] = (k < m.dim) ? ic.left + (c[k] + ic.xs[id[k]]) * ic.h : 0.0;
+    double cons[5];
+    if (ic.kind == 0) {
+        // InitialConditionFunction_TaylorGreenVortex[_Isothermal] (initial_condition_function.cpp:261-336)
+        const double p = 1.0 / (ic.gam * ic.mach2) + (1.0 / 16.0) * (cos(2.0 * x[0]) + cos(2.0 * x[1])) *
+                                                        (cos(2.0 * x[2]) + 2.0);
+        const double rho = ic.isothermal ? p * ic.gam * ic.mach2 : 1.0;
+        const double vx = sin(x[0]) * cos(x[1]) * cos(x[2]);
+        const double vy = -cos(x[0]) * sin(x[1]) * cos(x[2]);
+        const double vz = 0.0;
+        cons[0] = rho;
+        cons[1] = rho * vx;
+        cons[2] = rho * vy;
+        cons[3] = rho * vz;
+        cons[4] = p / (ic.gam - 1.0) + 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+    } else {
+        // isentropic vortex (initial_condition_function.cpp:643-690), strength and advection as parameters
+        const double gam = 1.4;
+        const double Om = ic.beta * exp(-0.5 * (x[0] * x[0] + x[1] * x[1]));
+        const double dT = -(gam - 1.0) / 2.0 * Om * Om;
+        const double rho = pow(1 + dT, 1.0 / (gam - 1.0));
+        const double vx = ic.u_inf - x[1] * Om, vy = ic.v_inf + x[0] * Om;
+        const double p = 1.0 / gam * pow(1 + dT, gam / (gam - 1.0));
+        cons[0] = rho;
+        cons[1] = rho * vx;
+        cons[2] = rho * vy;
+        cons[3] = p / (gam - 1.0) + 0.5 * rho * (vx * vx + vy * vy);
+    }
+    for (int s = 0; s < ns; ++s) u[((long)e * ns + s) * Np + i] = cons[s];
+}
+
+static MeshDesc mesh_desc(const dgx_ctx *c)
+{
+    MeshDesc m;
+    m.dim = c->dim;
+    m.N = c->P.n_cells_1d;
+    m.morton = (c->P.cell_order == DGX_ORDER_DEALII) && m.N > 0 && (m.N & (m.N - 1)) == 0;
+    m.levels = 0;
+    while ((1 << m.levels) < m.N) ++m.levels;
+    return m;
+}
+
+int launch_mesh(dgx_ctx *c)
+{
+    hipLaunchKernelGGL(mesh_kernel, dim3((c->ncells + 255) / 256), dim3(256), 0, c->stream, c->nbr_dev, c->ncells,
+                       mesh_desc(c));
+    return check_launch(c, "mesh_kernel");
+}
+
+int launch_ic(dgx_ctx *c, int kind, int isothermal, double strength, double u_inf, double v_inf)
+{
+    IcDesc ic{};
+    ic.kind = kind;
+    ic.isothermal = isothermal;
+    ic.left = c->P.domain_left;
+    ic.h = c->h;
+    ic.gam = c->P.gamma_gas;
+    ic.mach2 = c->P.mach_inf * c->P.mach_inf;
+    const double M = std::sqrt(2 / 1.4);
+    ic.beta = M * strength * std::sqrt(2.0) / 4.0 / M_PI * std::exp(1.0 / 2.0);
+    ic.u_inf = u_inf;
+    ic.v_inf = v_inf;
+    double ws[MAXN1D];
+    gauss_lobatto(c->n, ic.xs, ws);
+    const long n = (long)c->ncells * c->Np;
+    hipLaunchKernelGGL(ic_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->u_dev, c->ncells,
+                       c->n, mesh_desc(c), ic);
+    return check_launch(c, "ic_kernel");
+}
+
 int launch_inverse_mass(dgx_ctx *c, const double *in, double *out)
 {
     const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;

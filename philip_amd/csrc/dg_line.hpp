@@ -15,19 +15,19 @@
 // lane.
 //
 // Per block: EPB whole elements.
-//   A  node-parallel  load the stage state (FP64, coalesced), entropy round
-//                     trip (FP64 path), per-node cache into LDS; in the same
-//                     phase the neighbours' face traces (one item per face
-//                     node) are loaded into LDS, so the line phase makes no
-//                     global loads
+//   A  node-parallel  load the stage state (FP64, coalesced), per-node cache of
+//                     derived quantities into LDS; in the same phase the
+//                     neighbours' face traces (one item per face node) are
+//                     loaded into LDS, so the line phase makes no global loads
 //   L  line-parallel  pairs + faces of one line, sums kept in registers
 //                     (a long line's pairs shared by LS items); after a
 //                     barrier they go to LDS over the (now dead) node cache
 //                     and traces
 //   E  node-parallel  sum of the direction partials, FP64 epilogue
-// T = float: mixed path (per-node cache of derived quantities, float fluxes,
-// FP64 state and RK registers); T = double: FP64 path (the point physics of
-// dg_physics.hpp evaluated as in the reference).
+// T = float: mixed path (float fluxes, FP64 state and RK registers); T = double:
+// FP64 path.  Both evaluate the cached-node restatement of the point physics
+// (dg_fast.hpp).  The reference's entropy projection u(v(u)) is the identity
+// for collocated GLL nodes (the projection M^-1 chi^T W is I) and is not formed.
 #pragma once
 #include <type_traits>
 
@@ -49,8 +49,8 @@ template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
     static constexpr int EPB = ES > 0 ? G::EPB * ES : (G::EPB / -ES > 0 ? G::EPB / -ES : 1);
     static constexpr int BLOCK = ((EPB * NP + 63) / 64) * 64;
     static constexpr bool FAST = std::is_same<T, float>::value;
-    // node cache fields: FAST u[NS], vel[D], p, b, |v|^2, sqrt(rho), c, h; FP64 u[NS], vel[D], p
-    static constexpr int NQ = FAST ? node_f_count<D>() : NS + D + 1;
+    // node cache fields: u[NS], vel[D], p, b, |v|^2, sqrt(rho), c, h
+    static constexpr int NQ = node_f_count<D>();
     // element strides padded to 1 mod 32 words so the stride-N lines of
     // neighbouring elements fall in different LDS banks
     static constexpr int pad1(int x) { return x + (33 - x % 32) % 32; }
@@ -95,10 +95,10 @@ template <int D, int N, typename T, bool V> constexpr int line_es()
 }
 template <int D, int N, typename T, bool V> using LineGeoK = LineGeo<D, N, T, line_es<D, N, T, V>(), V>;
 
-// ---- mixed path: node cache in float ------------------------------------
+// ---- node cache in LDS ----------------------------------------------------
 
-template <int D>
-__device__ __forceinline__ void nodef_store(float *q, int st, const NodeF<D> &n)
+template <typename T, int D>
+__device__ __forceinline__ void nodef_store(T *q, int st, const NodeC<T, D> &n)
 {
     constexpr int NS = D + 2;
     for (int s = 0; s < NS; ++s) q[s * st] = n.u[s];
@@ -111,8 +111,8 @@ __device__ __forceinline__ void nodef_store(float *q, int st, const NodeF<D> &n)
     q[(NS + D + 5) * st] = n.h;
 }
 
-template <int D>
-__device__ __forceinline__ void nodef_load(const float *q, int st, NodeF<D> &n)
+template <typename T, int D>
+__device__ __forceinline__ void nodef_load(const T *q, int st, NodeC<T, D> &n)
 {
     constexpr int NS = D + 2;
     for (int s = 0; s < NS; ++s) n.u[s] = q[s * st];
@@ -125,8 +125,8 @@ __device__ __forceinline__ void nodef_load(const float *q, int st, NodeF<D> &n)
     n.h = q[(NS + D + 5) * st];
 }
 
-template <int D>
-__device__ __forceinline__ void nodef_select(bool first, const NodeF<D> &x, const NodeF<D> &y, NodeF<D> &o)
+template <typename T, int D>
+__device__ __forceinline__ void nodef_select(bool first, const NodeC<T, D> &x, const NodeC<T, D> &y, NodeC<T, D> &o)
 {
     for (int s = 0; s < D + 2; ++s) o.u[s] = first ? x.u[s] : y.u[s];
     for (int k = 0; k < D; ++k) o.vel[k] = first ? x.vel[k] : y.vel[k];
@@ -138,6 +138,23 @@ __device__ __forceinline__ void nodef_select(bool first, const NodeF<D> &x, cons
     o.h = first ? x.h : y.h;
 }
 
+// operator tables in the flux arithmetic type
+template <typename T> __device__ __forceinline__ T op_sk(const Ops1D &o, int a, int b)
+{
+    if constexpr (std::is_same<T, float>::value) return o.Skf[a][b];
+    else return o.Sk[a][b];
+}
+template <typename T> __device__ __forceinline__ T op_d(const Ops1D &o, int a, int b)
+{
+    if constexpr (std::is_same<T, float>::value) return o.Df[a][b];
+    else return o.D[a][b];
+}
+template <typename T> __device__ __forceinline__ T op_inv_wb(const Ops1D &o)
+{
+    if constexpr (std::is_same<T, float>::value) return o.inv_wbf;
+    else return o.inv_wb;
+}
+
 // Pairs of part H of LS: the N(N-1)/2 pairs a < b of a line, in the order (0,1), (0,2), ..,
 // (1,2), .., split into LS contiguous ranges (the line's work shared by LS items).
 template <int N, int LS, int H> __device__ __forceinline__ constexpr bool pair_in_part(int p0, int p1)
@@ -147,25 +164,25 @@ template <int N, int LS, int H> __device__ __forceinline__ constexpr bool pair_i
     return idx >= H * NPAIR / LS && idx < (H + 1) * NPAIR / LS;
 }
 
-template <int D, int N, int K, int LS = 1, int H = 0>
-__device__ __forceinline__ void pairs_f(const Ops1D &ops, const PhysConst &pc, const FastConst &fc, const float *sq,
-                                        int line, float (&acc)[N][D + 2])
+template <typename T, int D, int N, int K, int LS = 1, int H = 0>
+__device__ __forceinline__ void pairs_f(const Ops1D &ops, const PhysConst &pc, const FastConst<T> &fc, const T *sq,
+                                        int line, T (&acc)[N][D + 2])
 {
     constexpr int NP = Geo<D, N>::NP, NS = D + 2;
 #pragma unroll
     for (int p = 0; p < N; ++p)
-        for (int s = 0; s < NS; ++s) acc[p][s] = 0.f;
+        for (int s = 0; s < NS; ++s) acc[p][s] = T(0);
 #pragma unroll
     for (int p0 = 0; p0 < N; ++p0) {
 #pragma unroll
         for (int p1 = p0 + 1; p1 < N; ++p1) {
             if (!pair_in_part<N, LS, H>(p0, p1)) continue;
-            NodeF<D> A, B;
-            nodef_load<D>(sq + line_node<D, N>(K, line, p0), NP, A);
-            nodef_load<D>(sq + line_node<D, N>(K, line, p1), NP, B);
-            float F[NS];
-            pair_flux_fast<D, K>(A, B, fc, pc, F);
-            const float s01 = ops.Skf[p0][p1], s10 = ops.Skf[p1][p0];
+            NodeC<T, D> A, B;
+            nodef_load<T, D>(sq + line_node<D, N>(K, line, p0), NP, A);
+            nodef_load<T, D>(sq + line_node<D, N>(K, line, p1), NP, B);
+            T F[NS];
+            pair_flux_fast<T, D, K>(A, B, fc, pc, F);
+            const T s01 = op_sk<T>(ops, p0, p1), s10 = op_sk<T>(ops, p1, p0);
             for (int s = 0; s < NS; ++s) {
                 acc[p0][s] -= s01 * F[s];
                 acc[p1][s] -= s10 * F[s];
@@ -190,7 +207,7 @@ __device__ __forceinline__ void visc_line(const Ops1D &ops, const T *sfv, int li
         for (int s = 0; s < NS; ++s) fv[s] = sfv[(K * NS + s) * NP + nb];
 #pragma unroll
         for (int a2 = 0; a2 < N; ++a2) {
-            const T dab = std::is_same<T, float>::value ? T(ops.Df[a2][b]) : T(ops.D[a2][b]);
+            const T dab = op_d<T>(ops, a2, b);
             for (int s = 0; s < NS; ++s) acc[a2][s] -= dab * fv[s];
         }
     }
@@ -230,31 +247,32 @@ template <typename T> struct FaceIn {
 // side, so the neighbour's term is the negative.  nbl >= 0: the neighbour is
 // element nbl of this block (node cache in LDS); otherwise its trace is tr
 // (state-major, stride tst).
-template <int D, int N, int K, bool V>
-__device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, const FastConst &fc,
-                                       const FaceIn<float> &in, int le, int side, int fn, int e, int nb, int nbl,
-                                       float pen, float (&out)[D + 2], float (&out_nb)[D + 2])
+template <typename T, int D, int N, int K, bool V>
+__device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, const FastConst<T> &fc,
+                                       const FaceIn<T> &in, int le, int side, int fn, int e, int nb, int nbl,
+                                       T pen, T (&out)[D + 2], T (&out_nb)[D + 2])
 {
-    using LG = LineGeo<D, N, float, 1, V>;
+    using LG = LineGeo<D, N, T, 1, V>;
     constexpr int NP = LG::NP, NS = D + 2;
     const int fi = line_node<D, N>(K, fn, side ? N - 1 : 0);
     const int fj = line_node<D, N>(K, fn, side ? 0 : N - 1);
-    NodeF<D> qi, qn, L, R;
-    nodef_load<D>(in.sq + le * LG::ESQ + fi, NP, qi);
+    NodeC<T, D> qi, qn, L, R;
+    nodef_load<T, D>(in.sq + le * LG::ESQ + fi, NP, qi);
     if (nbl >= 0) {
-        nodef_load<D>(in.sq + nbl * LG::ESQ + fj, NP, qn);
+        nodef_load<T, D>(in.sq + nbl * LG::ESQ + fj, NP, qn);
     } else {
         for (int s = 0; s < NS; ++s) qn.u[s] = in.tr[s * in.tst];
-        node_fill<D>(qn, fc);
+        node_fill<T, D>(qn, fc);
     }
     const bool own = e < nb;
-    nodef_select<D>(own, qi, qn, L);
-    nodef_select<D>(own, qn, qi, R);
-    const float sgn = side ? 1.f : -1.f;
-    float fs[NS];
-    num_flux_fast<D, K>(L, R, own ? sgn : -sgn, fc, pc, fs);
+    nodef_select<T, D>(own, qi, qn, L);
+    nodef_select<T, D>(own, qn, qi, R);
+    const T sgn = side ? T(1) : T(-1);
+    T fs[NS];
+    num_flux_fast<T, D, K>(L, R, own ? sgn : -sgn, fc, pc, fs);
+    const T iwb = op_inv_wb<T>(ops);
     if constexpr (!V) {
-        const float w = own ? -ops.inv_wbf : ops.inv_wbf;
+        const T w = own ? -iwb : iwb;
         for (int s = 0; s < NS; ++s) {
             out[s] = w * fs[s];
             out_nb[s] = -out[s];
@@ -262,95 +280,25 @@ __device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, co
     } else {
         // the viscous surface term: the average is antisymmetric between the two sides, the
         // interior-flux part (strong form) is not, so both sides' values are returned
-        float fa[NS], fb[NS], fvs[NS];
+        T fa[NS], fb[NS], fvs[NS];
         for (int s = 0; s < NS; ++s) {
             fa[s] = in.sfv[le * LG::ESFV + (K * NS + s) * NP + fi];
             fb[s] = (nbl >= 0) ? in.sfv[nbl * LG::ESFV + (K * NS + s) * NP + fj] : in.trv[s * in.tst];
         }
-        face_visc<D, K, float>(pc, pen, qi.u, qn.u, fa, fb, sgn, fvs);
-        const float fsgn = own ? 1.f : -1.f;
+        face_visc<D, K, T>(pc, pen, qi.u, qn.u, fa, fb, sgn, fvs);
+        const T fsgn = own ? T(1) : T(-1);
         for (int s = 0; s < NS; ++s) {
-            const float A = -(fs[s] * fsgn) - fvs[s];
-            out[s] = ops.inv_wbf * (A + sgn * fa[s]);
-            out_nb[s] = ops.inv_wbf * ((-sgn * fb[s]) - A);
+            const T A = -(fs[s] * fsgn) - fvs[s];
+            out[s] = iwb * (A + sgn * fa[s]);
+            out_nb[s] = iwb * ((-sgn * fb[s]) - A);
         }
     }
-}
-
-// ---- FP64 path ------------------------------------------------------------
-
-template <int D, int N, int K, int LS = 1, int H = 0>
-__device__ __forceinline__ void pairs_d(const Ops1D &ops, const PhysConst &pc, const double *sq, int line,
-                                        double (&acc)[N][D + 2])
-{
-    constexpr int NP = Geo<D, N>::NP, NS = D + 2;
-#pragma unroll
-    for (int p = 0; p < N; ++p)
-        for (int s = 0; s < NS; ++s) acc[p][s] = 0.0;
-#pragma unroll
-    for (int p0 = 0; p0 < N; ++p0) {
-#pragma unroll
-        for (int p1 = p0 + 1; p1 < N; ++p1) {
-            if (!pair_in_part<N, LS, H>(p0, p1)) continue;
-            const int na = line_node<D, N>(K, line, p0), nbn = line_node<D, N>(K, line, p1);
-            double u1[NS], u2[NS];
-            Prim<double, D> q1, q2;
-            for (int s = 0; s < NS; ++s) {
-                u1[s] = sq[s * NP + na];
-                u2[s] = sq[s * NP + nbn];
-            }
-            q1.rho = u1[0];
-            q2.rho = u2[0];
-            for (int d = 0; d < D; ++d) {
-                q1.vel[d] = sq[(NS + d) * NP + na];
-                q2.vel[d] = sq[(NS + d) * NP + nbn];
-            }
-            q1.p = sq[(NS + D) * NP + na];
-            q2.p = sq[(NS + D) * NP + nbn];
-            double F[NS];
-            two_point_flux_dir<double, D>(u1, q1, u2, q2, K, pc, F);
-            const double s01 = ops.Sk[p0][p1], s10 = ops.Sk[p1][p0];
-            for (int s = 0; s < NS; ++s) {
-                acc[p0][s] -= s01 * F[s];
-                acc[p1][s] -= s10 * F[s];
-            }
-        }
-    }
-}
-
-template <int D, int N, int K>
-__device__ __forceinline__ void face_d(const Ops1D &ops, const PhysConst &pc, const FaceIn<double> &in, int le,
-                                       int side, int fn, int e, int nb, int nbl, double (&out)[D + 2])
-{
-    using LG = LineGeo<D, N, double>;
-    constexpr int NP = LG::NP, NS = D + 2;
-    const int fi = line_node<D, N>(K, fn, side ? N - 1 : 0);
-    const int fj = line_node<D, N>(K, fn, side ? 0 : N - 1);
-    double us[NS], unt[NS];
-    for (int s = 0; s < NS; ++s) {
-        us[s] = in.sq[le * LG::ESQ + s * NP + fi];
-        // in-block neighbour: its (round-tripped) state is in the node cache
-        unt[s] = (nbl >= 0) ? in.sq[nbl * LG::ESQ + s * NP + fj] : in.tr[s * in.tst];
-    }
-    const bool own = e < nb;
-    double ua[NS], ub[NS];
-    for (int s = 0; s < NS; ++s) {
-        ua[s] = own ? us[s] : unt[s];
-        ub[s] = own ? unt[s] : us[s];
-    }
-    const Prim<double, D> pa = to_prim<double, D>(ua, pc.gamm1);
-    const Prim<double, D> pb = to_prim<double, D>(ub, pc.gamm1);
-    const double sgn = side ? 1.0 : -1.0;
-    double fs[NS];
-    numerical_flux<double, D>(ua, pa, ub, pb, K, own ? sgn : -sgn, pc, fs);
-    const double fsgn = own ? 1.0 : -1.0;
-    for (int s = 0; s < NS; ++s) out[s] = ops.inv_wb * (-(fs[s] * fsgn));
 }
 
 // One block's work: elements blk0 .. blk0 + EPB - 1 of the solve described by a.
 // V: Navier-Stokes (split convective form plus the viscous volume and surface
 // terms, with the lifted gradient a.aux computed beforehand).
-template <int D, int N, bool RT, typename T, int ES, bool V = false>
+template <int D, int N, typename T, int ES, bool V = false>
 __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops, const PhysConst &pc, const int blk0)
 {
     using LG = LineGeo<D, N, T, ES, V>;
@@ -372,7 +320,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     __shared__ short sList[NWAVE][NFS];       // per-wave list of the faces evaluated in this block
 
     const int tid = threadIdx.x;
-    const FastConst fc = fast_const(pc);
+    const FastConst<T> fc = fast_const<T>(pc);
     // a neighbour is read from the block's own node cache when it is another element of the block
     auto in_block = [&](int nb, int fe) { return nb >= blk0 && nb < blk0 + EPB && nb != fe; };
 
@@ -423,39 +371,17 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
         }
     }
     if (active) {
-        T *q = sQ + le * LG::ESQ + i;
-        if constexpr (FAST) {
-            NodeF<D> n;
-            for (int s = 0; s < NS; ++s) n.u[s] = float(ud[s]);
-            node_fill<D>(n, fc);
-            flag_nonphysical<float>(a, n.u[0], n.p);
-            nodef_store<D>(q, NP, n);
-        } else {
-            double ut[NS];
-            if (RT) roundtrip<double, D>(ud, pc, ut);
-            else
-                for (int s = 0; s < NS; ++s) ut[s] = ud[s];
-            for (int s = 0; s < NS; ++s) q[s * NP] = ut[s];
-            const double rho = ut[0];
-            for (int k = 0; k < D; ++k) q[(NS + k) * NP] = ut[1 + k] / rho;
-            const double pr = pressure<double, D>(ut, pc.gamm1);
-            flag_nonphysical<double>(a, rho, pr);
-            q[(NS + D) * NP] = pr;
-        }
+        NodeC<T, D> n;
+        for (int s = 0; s < NS; ++s) n.u[s] = T(ud[s]);
+        node_fill<T, D>(n, fc);
+        flag_nonphysical<T>(a, n.u[0], n.p);
+        nodef_store<T, D>(sQ + le * LG::ESQ + i, NP, n);
     }
 #pragma unroll
     for (int j = 0; j < NFI; ++j) {
         if (fdst[j] < 0) continue;
         T *dst = sN + fdst[j];
-        if constexpr (FAST) {
-            for (int s = 0; s < NS; ++s) dst[s * LG::NFN] = float(un[j][s]);
-        } else {
-            double unt[NS];
-            if (RT) roundtrip<double, D>(un[j], pc, unt);
-            else
-                for (int s = 0; s < NS; ++s) unt[s] = un[j][s];
-            for (int s = 0; s < NS; ++s) dst[s * LG::NFN] = unt[s];
-        }
+        for (int s = 0; s < NS; ++s) dst[s * LG::NFN] = T(un[j][s]);
     }
     if constexpr (V) {
         // viscous fluxes from the lifted gradient: the own nodes' in every direction, the
@@ -539,8 +465,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             const T *q = sQ + lle * LG::ESQ;
             auto pitem = [&](auto kc, auto hc) {
                 constexpr int K = decltype(kc)::value, H = decltype(hc)::value;
-                if constexpr (FAST) pairs_f<D, N, K, LS, H>(ops, pc, fc, q, line, rp[j < JP ? j : 0]);
-                else pairs_d<D, N, K, LS, H>(ops, pc, q, line, rp[j < JP ? j : 0]);
+                pairs_f<T, D, N, K, LS, H>(ops, pc, fc, q, line, rp[j < JP ? j : 0]);
                 if constexpr (V) visc_line<D, N, K, T, LS, H>(ops, sFv + lle * LG::ESFV, line, rp[j < JP ? j : 0]);
             };
             auto pdir = [&](auto kc) {
@@ -562,11 +487,8 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             const FaceIn<T> in{sQ, sFv, sN + fle * LG::ESN + fo, sNv + fle * LG::ESNV + fo, LG::NFN};
             auto fitem = [&](auto kc) {
                 constexpr int K = decltype(kc)::value;
-                if constexpr (FAST)
-                    face_f<D, N, K, V>(ops, pc, fc, in, fle, side, fn, blk0 + fle, nb, nbl, pen, rf[j],
-                                       rfn[NBT ? j : 0]);
-                else
-                    face_d<D, N, K>(ops, pc, in, fle, side, fn, blk0 + fle, nb, nbl, rf[j]);
+                face_f<T, D, N, K, V>(ops, pc, fc, in, fle, side, fn, blk0 + fle, nb, nbl, pen, rf[j],
+                                      rfn[NBT ? j : 0]);
             };
             if (k == 0) fitem(std::integral_constant<int, 0>{});
             else if (k == 1) fitem(std::integral_constant<int, 1>{});
@@ -627,19 +549,19 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
 }
 
 // One solve: blocks in XCD-aware order over the element groups.
-template <int D, int N, bool RT, typename T>
+template <int D, int N, typename T>
 __global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK))
 line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
-    line_block<D, N, RT, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
+    line_block<D, N, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
 }
 // The same held to 4 waves/SIMD: used for the mixed path at 3D p=6, where it measured faster
 // despite spilling (12^3 TGV: 129 -> 120 us per stage; p=3..5 unchanged).
-template <int D, int N, bool RT, typename T>
+template <int D, int N, typename T>
 __global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK)) __attribute__((amdgpu_waves_per_eu(4)))
 line_kernel_w4(StageArgs a, Ops1D ops, PhysConst pc)
 {
-    line_block<D, N, RT, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
+    line_block<D, N, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
 }
 
 // Navier-Stokes, mixed path.  line_kernel_ns_w4 is held to 4 waves/SIMD (128 VGPRs; a few
@@ -649,15 +571,15 @@ template <int D, int N>
 __global__ void __launch_bounds__((LineGeoK<D, N, float, true>::BLOCK))
 line_kernel_ns(StageArgs a, Ops1D ops, PhysConst pc)
 {
-    line_block<D, N, false, float, line_es<D, N, float, true>(), true>(a, ops, pc,
-                                                                       xcd_block() * LineGeoK<D, N, float, true>::EPB);
+    line_block<D, N, float, line_es<D, N, float, true>(), true>(a, ops, pc,
+                                                                xcd_block() * LineGeoK<D, N, float, true>::EPB);
 }
 template <int D, int N>
 __global__ void __launch_bounds__((LineGeoK<D, N, float, true>::BLOCK)) __attribute__((amdgpu_waves_per_eu(4)))
 line_kernel_ns_w4(StageArgs a, Ops1D ops, PhysConst pc)
 {
-    line_block<D, N, false, float, line_es<D, N, float, true>(), true>(a, ops, pc,
-                                                                       xcd_block() * LineGeoK<D, N, float, true>::EPB);
+    line_block<D, N, float, line_es<D, N, float, true>(), true>(a, ops, pc,
+                                                                xcd_block() * LineGeoK<D, N, float, true>::EPB);
 }
 
 // Many independent solves of one (dim, p, precision) in one launch: block b
@@ -666,24 +588,24 @@ line_kernel_ns_w4(StageArgs a, Ops1D ops, PhysConst pc)
 struct BatchBlock {
     int inst, blk0;
 };
-template <int D, int N, bool RT, typename T, int ES>
+template <int D, int N, typename T, int ES>
 __global__ void __launch_bounds__((LineGeo<D, N, T, ES>::BLOCK))
 line_kernel_batch(const BatchBlock *__restrict__ blocks, const StageArgs *__restrict__ args,
                   const PhysConst *__restrict__ pcs, Ops1D ops)
 {
     const BatchBlock bb = blocks[xcd_block()];
-    line_block<D, N, RT, T, ES>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
+    line_block<D, N, T, ES>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
 }
 // The same held to 5 waves/SIMD (96 VGPRs, a few values spilled): used for the 2D mixed-path
 // groups up to p=5, where it measured faster (p=3: 64 -> 55 us, p=4: 141 -> 125 us per stage in
 // the 256-instance vortex mix); at p=6 it was slower (254 -> 263 us).
-template <int D, int N, bool RT, typename T, int ES>
+template <int D, int N, typename T, int ES>
 __global__ void __launch_bounds__((LineGeo<D, N, T, ES>::BLOCK)) __attribute__((amdgpu_waves_per_eu(5)))
 line_kernel_batch_w5(const BatchBlock *__restrict__ blocks, const StageArgs *__restrict__ args,
                      const PhysConst *__restrict__ pcs, Ops1D ops)
 {
     const BatchBlock bb = blocks[xcd_block()];
-    line_block<D, N, RT, T, ES>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
+    line_block<D, N, T, ES>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
 }
 
 } // namespace dgx

@@ -1,13 +1,13 @@
-"""CPU check of the mixed path's float point physics (dg_fast.hpp), built for
-the host with g++, against the oracle's FP64 point physics.
+"""CPU check of the line kernels' cached-node point physics (dg_fast.hpp), built for the host
+with g++ in float (mixed path) and in double (FP64 path), against the oracle's FP64 point
+physics.
 
-The float functions restate the reference formulas with fewer divisions
-(per-node reciprocals, log means from (a-b)/(a+b), Roe averages from sqrt(rho));
-a dropped term, a wrong sign or a swapped side shows up as an O(1) relative
-error, while float rounding of this restatement stays below 2e-5 of the flux
-magnitude on states near the TGV/vortex regimes (M = 0.1 and M ~ 1.2,
-density and pressure varying by up to 30% between the two sides so that both
-branches of the log mean are exercised)."""
+The functions restate the reference formulas with fewer divisions (per-node reciprocals, log
+means from (a-b)/(a+b), Roe averages from sqrt(rho)); a dropped term, a wrong sign or a swapped
+side shows up as an O(1) relative error, while rounding of this restatement stays below 2e-5 of
+the flux magnitude in float and below 1e-13 in double, on states near the TGV/vortex regimes
+(M = 0.1 and M ~ 1.2, density and pressure varying by up to 30% between the two sides so that
+both branches of the log mean are exercised)."""
 import os
 import subprocess
 
@@ -42,14 +42,15 @@ def states(rng, n, mach, spread):
 @pytest.mark.parametrize("tp", [O.KG, O.IR, O.CH, O.RA])
 @pytest.mark.parametrize("cf", [O.LXF, O.ROE, O.CENTRAL, O.TWO_POINT, O.TWO_POINT_LXF, O.TWO_POINT_ROE])
 @pytest.mark.parametrize("mach,spread", [(0.1, 0.01), (0.1, 0.3), (1.2, 0.3)])
-def test_float_fluxes_match_oracle(host_exe, tp, cf, mach, spread):
+@pytest.mark.parametrize("prec,bar", [(0, 2e-5), (1, 1e-13)], ids=["float", "double"])
+def test_cached_node_fluxes_match_oracle(host_exe, tp, cf, mach, spread, prec, bar):
     rng = np.random.default_rng(17 + tp + 7 * cf)
     n = 64
     ui = states(rng, n, mach, spread)
     # the second side: small perturbation of the first for half the samples
     ue = states(rng, n, mach, spread)
     ue[: n // 2] = ui[: n // 2] * (1 + 1e-3 * rng.standard_normal((n // 2, 5)))
-    inp = f"{n} {tp} {cf} 1.4 {mach}\n" + "\n".join(" ".join(f"{x:.17g}" for x in np.r_[a, b])
+    inp = f"{n} {tp} {cf} 1.4 {mach} {prec}\n" + "\n".join(" ".join(f"{x:.17g}" for x in np.r_[a, b])
                                                      for a, b in zip(ui, ue))
     out = subprocess.run([host_exe], input=inp, capture_output=True, text=True, check=True).stdout
     got = np.array([[float(x) for x in line.split()] for line in out.strip().splitlines()]).reshape(n, 3, 2, 5)
@@ -66,4 +67,4 @@ def test_float_fluxes_match_oracle(host_exe, tp, cf, mach, spread):
             scale_g = np.abs(G).max()
             worst = max(worst, np.abs(got[i, k, 0] - F[:, k]).max() / scale_f,
                         np.abs(got[i, k, 1] - G).max() / scale_g)
-    assert worst < 2e-5, worst
+    assert worst < bar, worst
