@@ -31,9 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNEL = {"fp64": "line_kernel<3,4,true,double,false>", "mixed": "line_kernel<3,4,false,float,false>"}
+KERNEL = {"fp64": "line_kernel<3,4,double>", "mixed": "line_kernel<3,4,float>"}
 TRAFFIC_FILE = {"fp64": os.path.join(ROOT, "profiles", "line_kernel_fp64_tgv_p3_32cubed_traffic.txt"),
-                "mixed": os.path.join(ROOT, "profiles", "line_kernel_tgv_p3_32cubed_traffic.txt")}
+                "mixed": os.path.join(ROOT, "profiles", "line_kernel_mixed_tgv_p3_32cubed_traffic.txt")}
 
 
 def measured_traffic(precision):
@@ -197,7 +197,7 @@ def batched_vortex(n_inst=256, steps=200, seed=2024, precision="mixed"):
                 workload="256 independent 2D vortex solves, N~U{8..64}, p~U{2..6}, RK4, one graph launch per step")
 
 
-def serving(n_req=48, rate=40.0, seed=77, precision="mixed", time_scale=0.1):
+def serving(n_req=400, rate=8.0, seed=77, precision="mixed", time_scale=1.0):
     """Serving stream: Poisson arrivals (rate req/s) of parametric solves, 70% 2D vortex
     (N~U{8..64}, p~U{2..6}) and 30% 3D Euler TGV (M~U[0.05,0.3], p~U{2..5}, 4^3..16^3), final
     time ~U[0.5,2] x time_scale; HBM budget 64 GiB.  Latency = arrival to result on the host."""
@@ -225,6 +225,7 @@ def serving(n_req=48, rate=40.0, seed=77, precision="mixed", time_scale=0.1):
     t0 = time.perf_counter()
     k = 0
     done = 0
+    max_active = max_queued = steps = 0
     while done < n_req:
         now = time.perf_counter() - t0
         while k < n_req and arrivals[k] <= now:
@@ -232,6 +233,8 @@ def serving(n_req=48, rate=40.0, seed=77, precision="mixed", time_scale=0.1):
             delay.append(now - arrivals[k])  # host polling delay, added to the latency
             k += 1
         a, q, done = s.step()
+        steps += 1
+        max_active, max_queued = max(max_active, a), max(max_queued, q)
         if a == 0 and q == 0 and k < n_req:
             time.sleep(max(0.0, arrivals[k] - (time.perf_counter() - t0)))
     wall = time.perf_counter() - t0
@@ -243,7 +246,8 @@ def serving(n_req=48, rate=40.0, seed=77, precision="mixed", time_scale=0.1):
     s.close()
     lat = np.array(lat)
     return dict(p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
-                throughput=dofu / wall, unit="DOF-updates/s", requests=n_req, rate_per_s=rate, wall_s=wall,
+                max_s=float(lat.max()), throughput=dofu / wall, unit="DOF-updates/s", requests=n_req,
+                rate_per_s=rate, wall_s=wall, scheduler_steps=steps, max_active=max_active, max_queued=max_queued,
                 precision=precision,
                 workload=f"70% 2D vortex / 30% 3D TGV, final time U[0.5,2]x{time_scale}, RK4, HBM budget 64 GiB")
 
@@ -358,6 +362,9 @@ def main():
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances line")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream line")
+    ap.add_argument("--serving-rate", type=float, default=8.0,
+                    help="Poisson arrival rate of the mixed-precision serving stream (FP64: half)")
+    ap.add_argument("--serving-sweep", action="store_true", help="also sweep the arrival rate")
     ap.add_argument("--no-ns-history", action="store_true",
                     help="skip the long runs at both precisions (Euler TGV 1000 steps, Navier-Stokes TGV to t=2)")
     args = ap.parse_args()
@@ -411,9 +418,12 @@ def main():
                 "entropy_drift_mixed": h["mixed"]["entropy_drift"],
                 "max_entropy_diff_rel_S0": h["max_entropy_diff_rel_S0"]}
     if not args.no_serving:
-        line["serving"] = serving()
-        # arrival rate swept at both precisions (same request stream per rate)
-        line["serving_sweep"] = [serving(rate=r, precision=pr) for pr in ("mixed", "fp64") for r in (20.0, 40.0, 80.0)]
+        line["serving"] = [serving(precision="mixed", rate=args.serving_rate),
+                           serving(precision="fp64", rate=args.serving_rate / 2)]
+        if args.serving_sweep:
+            # arrival rate swept at both precisions (same request stream per rate), shorter final times
+            line["serving_sweep"] = [serving(n_req=100, rate=r, precision=pr, time_scale=0.1)
+                                     for pr in ("mixed", "fp64") for r in (20.0, 40.0, 80.0)]
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or host_cores()[0]
         line["cpu_baseline"] = cpu_baseline(threads)

@@ -159,6 +159,17 @@ int dgx_set_error_adaptive_parameters(dgx_ctx *ctx, double atol, double rtol, do
 int dgx_get_automatic_initial_step_size(dgx_ctx *ctx, double *dt);
 int dgx_get_error_adaptive_step_size(dgx_ctx *ctx, double dt, double *dt_next);
 
+/* Relaxation Runge-Kutta for the explicit Butcher methods (ode_solver_param.use_relaxation_runge_kutta
+   and relaxation_runge_kutta_root_tolerance, default 5e-10, parameters_ode_solver.cpp:144-151;
+   RungeKuttaODESolver::adjust_time_step, runge_kutta_ode_solver.cpp:101-109; root-finding RRK,
+   root_finding_rrk_ode_solver.cpp:16-390): each step's update u_n + gamma dt sum b_i k_i and time
+   advance gamma dt, gamma the root of S(u_n + gamma d) - S(u_n) = gamma dt sum b_i v(u^(i)) . M k_i
+   (S the integrated numerical entropy, v the entropy variables).  Enable after
+   dgx_allocate_ode_system; DGX_ERR_NONPHYSICAL from a step whose root finding fails (the reference
+   aborts).  dgx_relaxation_parameter: gamma of the last step (1 before the first). */
+int dgx_set_relaxation_runge_kutta(dgx_ctx *ctx, int enable, double root_tolerance);
+double dgx_relaxation_parameter(const dgx_ctx *ctx);
+
 /* PeriodicTurbulence::get_constant_time_step (periodic_turbulence.cpp:115-125). */
 double dgx_constant_time_step(const dgx_ctx *ctx, double cfl);
 /* PeriodicTurbulence::compute_and_update_integrated_quantities (periodic_turbulence.cpp:439-671)

@@ -108,6 +108,8 @@ def _load():
     lib.dgo_set_mutation.argtypes = [i]
     lib.dgo_step_error_norm.restype = d
     lib.dgo_step_error_norm.argtypes = [vp, i, d, dp, d, d]
+    lib.dgo_step_rrk.restype = d
+    lib.dgo_step_rrk.argtypes = [vp, i, d, dp, dp, d]
     lib.dgo_initial_step_size.restype = d
     lib.dgo_initial_step_size.argtypes = [vp, dp]
     lib.dgo_set_fsal.argtypes = [vp, i]
@@ -329,6 +331,13 @@ class Oracle:
         """One low-storage RK step of u in place; returns the error norm w of its embedded pair."""
         m = METHODS[method] if isinstance(method, str) else method
         return lib().dgo_step_error_norm(self.h, m, dt, _dp(u), atol, rtol)
+
+    def step_rrk(self, method, dt, u, gamma=1.0, tol=5e-10):
+        """One relaxation-RK step of u in place; returns (time advanced gamma*dt, gamma)."""
+        m = METHODS[method] if isinstance(method, str) else method
+        g = ctypes.c_double(gamma)
+        adv = lib().dgo_step_rrk(self.h, m, dt, _dp(u), ctypes.byref(g), tol)
+        return adv, g.value
 
     def initial_step_size(self, u):
         return lib().dgo_initial_step_size(self.h, _dp(np.ascontiguousarray(u, dtype=np.float64)))

@@ -1511,10 +1511,15 @@ void dgo_set_fsal(dgo_ctx *c, int on) { c->fsal = on != 0; }
 /* Rounding scale of the residual, per DoF (test infrastructure, collocated GLL nodes): the sum
    of the absolute values of the terms the residual at that DoF is summed from -- two-point
    volume fluxes times the skew operator (split form) or flux values times the derivative
-   operator (conservative form and the viscous flux), and at face nodes the numerical fluxes and
-   the interior fluxes of the strong form -- each with the node's quadrature weight and metric
-   factor W_i h^(d-1) (surface terms also 1/w_0).  A residual computed in another summation
-   order differs from the oracle's by a few units of FP64 rounding of this scale. */
+   operator (conservative form and the viscous flux), and at face nodes the numerical fluxes,
+   the one-sided terms of their dissipation (the LxF/Roe dissipation -1/2 |A| (u_R - u_L) is the
+   sum of -1/2 |A| u_R and +1/2 |A| u_L; bounded here by 1/2 lambda (|u_L| + |u_R|), lambda the
+   larger spectral radius, which bounds every eigenvalue of the Roe matrix), the one-sided terms
+   of the SIPG penalty flux (F_v(u, (u_i - u_e) n) is linear in the jump) and the interior fluxes
+   of the strong form -- each with the node's quadrature weight and metric factor W_i h^(d-1)
+   (surface terms also 1/w_0).  A residual computed in another summation order, or from states
+   that differ by a few ulps (the entropy-variable round trip), differs from the oracle's by a
+   few units of FP64 rounding of this scale. */
 void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
 {
     const long nd = dgo_n_dofs(c);
@@ -1583,10 +1588,28 @@ void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
                 dgo_numerical_flux(c, c->P.conv_flux, ui, ue, nrm, fl);
                 dgo_convective_flux(c, ui, F);
                 for (int s = 0; s < ns; ++s) acc[s] += (fabs(fl[s]) + fabs(F[s * d + k])) / c->wq[0];
+                const int cf = c->P.conv_flux;
+                if (cf == DGO_LXF || cf == DGO_ROE || cf == DGO_TWO_POINT_LXF || cf == DGO_TWO_POINT_ROE) {
+                    const double li = max_normal_eig(c, ui, nrm), le = max_normal_eig(c, ue, nrm);
+                    const double lam = li > le ? li : le;
+                    for (int s = 0; s < ns; ++s) acc[s] += 0.5 * lam * (fabs(ui[s]) + fabs(ue[s])) / c->wq[0];
+                }
                 if (visc) {
                     viscous_num_flux(c, ui, ue, gi, ge, nrm, penalty, fl);
                     dgo_dissipative_flux(c, ui, gi, F);
                     for (int s = 0; s < ns; ++s) acc[s] += (fabs(fl[s]) + fabs(F[s * d + k])) / c->wq[0];
+                    if (c->P.diss_flux == DGO_SIPG) {
+                        /* penalty {{F_v(u, (u_i - u_e) n)}}: the one-sided terms in u_i and u_e */
+                        const double *side_u[2] = {ui, ue};
+                        for (int a = 0; a < 2; ++a)
+                            for (int b = 0; b < 2; ++b) {
+                                double g[NS_MAX * 3], Fv[NS_MAX * 3];
+                                for (int s = 0; s < ns; ++s)
+                                    for (int kk = 0; kk < d; ++kk) g[s * d + kk] = side_u[b][s] * nrm[kk];
+                                dgo_dissipative_flux(c, side_u[a], g, Fv);
+                                for (int s = 0; s < ns; ++s) acc[s] += 0.5 * penalty * fabs(Fv[s * d + k]) / c->wq[0];
+                            }
+                    }
                 }
             }
             for (int s = 0; s < ns; ++s) mag[IDX(c, e, s, i)] = c->wvol[i] * hd1 * acc[s];
@@ -2116,6 +2139,133 @@ double dgo_numerical_entropy(const dgo_ctx *c, const double *u)
     free(uq);
     (void)Np;
     return acc;
+}
+
+/* Relaxation Runge-Kutta, root-finding form (RRKODESolverBase::update_relaxation_parameter,
+   rrk_ode_solver_base.cpp:16-35; RootFindingRRKODESolver::compute_relaxation_parameter,
+   root_finding_rrk_ode_solver.cpp:16-176 with the secant iteration and its bisection fallback;
+   compute_entropy_change_estimate, :357-390; RungeKuttaBase::step_in_time, runge_kutta_base.cpp:21-38):
+   one Butcher step of size dt whose update u_n + gamma dt sum b_i k_i makes the integrated
+   numerical entropy change equal gamma times the estimate dt sum b_i v(u^(i)) . M k_i.
+   *gamma: in, the previous step's relaxation parameter (1 initially); out, this step's.
+   Returns the time advanced, gamma dt (-1 if the root finding failed). */
+static double rrk_root_function(dgo_ctx *c, double gamma, const double *un, const double *dir, double s_n,
+                                double est, double *tmp)
+{
+    const long nd = dgo_n_dofs(c);
+    for (long q = 0; q < nd; ++q) tmp[q] = un[q] + gamma * dir[q];
+    return dgo_numerical_entropy(c, tmp) - s_n - gamma * est;
+}
+
+double dgo_step_rrk(dgo_ctx *c, int m, double dt, double *u, double *gamma_io, double tol)
+{
+    butcher B;
+    if (!get_butcher(m, &B)) return -1.0;
+    const long nd = dgo_n_dofs(c);
+    const int d = c->dim, ns = c->ns, Np = c->Np;
+    double *un = (double *)malloc(sizeof(double) * nd);
+    double *scratch = (double *)malloc(sizeof(double) * nd);
+    double *tmp = (double *)malloc(sizeof(double) * nd);
+    double *dir = (double *)calloc((size_t)nd, sizeof(double));
+    double *stage[4], *stage_sol[4];
+    memcpy(un, u, sizeof(double) * nd);
+    double est = 0.0;
+    for (int i = 0; i < B.s; ++i) {
+        stage[i] = (double *)calloc((size_t)nd, sizeof(double));
+        stage_sol[i] = (double *)malloc(sizeof(double) * nd);
+        for (int j = 0; j < i; ++j)
+            if (B.a[i][j] != 0)
+                for (long q = 0; q < nd; ++q) stage[i][q] += B.a[i][j] * stage[j][q];
+        for (long q = 0; q < nd; ++q) stage[i][q] *= dt;
+        for (long q = 0; q < nd; ++q) stage[i][q] += 1.0 * un[q];
+        memcpy(stage_sol[i], stage[i], sizeof(double) * nd);
+        rhs_lifted(c, stage_sol[i], stage[i], scratch);
+        /* entropy variables of the stage solution . M k_i (collocated GLL: the projection is the
+           identity, M = W det J) */
+        double ent = 0.0;
+        for (int e = 0; e < c->ncells; ++e)
+            for (int q = 0; q < Np; ++q) {
+                double us[NS_MAX], vs[NS_MAX];
+                for (int st = 0; st < ns; ++st) us[st] = stage_sol[i][IDX(c, e, st, q)];
+                dgo_entropy_variables(c, us, vs);
+                for (int st = 0; st < ns; ++st) ent += vs[st] * (c->wvol[q] * c->detJ * stage[i][IDX(c, e, st, q)]);
+            }
+        est += B.b[i] * ent;
+    }
+    est *= dt;
+    for (int i = 0; i < B.s; ++i)
+        for (long q = 0; q < nd; ++q) dir[q] += B.b[i] * stage[i][q];
+    for (long q = 0; q < nd; ++q) dir[q] *= dt;
+    const double s_n = dgo_numerical_entropy(c, un);
+    /* secant from the previous relaxation parameter +- 1e-5, bisection if it fails */
+    double gamma_kp1 = 0.0;
+    int iter = 0, failed = 0;
+    const int limit = 100;
+    {
+        double residual = 1.0;
+        double gk = *gamma_io + 1e-5, gkm1 = *gamma_io - 1e-5;
+        double rk = rrk_root_function(c, gk, un, dir, s_n, est, tmp);
+        double rkm1 = rrk_root_function(c, gkm1, un, dir, s_n, est, tmp);
+        while (residual > tol && iter < limit) {
+            if (rkm1 == rk) {
+                gk *= 1.001;
+                rkm1 = rrk_root_function(c, gkm1, un, dir, s_n, est, tmp);
+                rk = rrk_root_function(c, gk, un, dir, s_n, est, tmp);
+            }
+            if (gk < 0.5 || gk > 1.5) {
+                gk = 1.0;
+                rk = rrk_root_function(c, gk, un, dir, s_n, est, tmp);
+            }
+            gamma_kp1 = gk - rk * (gk - gkm1) / (rk - rkm1);
+            residual = fabs(gamma_kp1 - gk);
+            iter++;
+            gkm1 = gk;
+            gk = gamma_kp1;
+            rkm1 = rk;
+            rk = rrk_root_function(c, gk, un, dir, s_n, est, tmp);
+            if (isnan(gk) || isnan(gkm1)) {
+                gk = 1.0 - 1e-5;
+                rk = rrk_root_function(c, gk, un, dir, s_n, est, tmp);
+                gkm1 = 1.0 + 1e-5;
+                rkm1 = rrk_root_function(c, gkm1, un, dir, s_n, est, tmp);
+                residual = 1.0;
+            }
+        }
+        if (iter == limit) failed = 1;
+    }
+    if (failed) {
+        iter = 0;
+        double lo = *gamma_io - 0.1, hi = *gamma_io + 0.1;
+        double rlo = rrk_root_function(c, lo, un, dir, s_n, est, tmp);
+        double rhi = rrk_root_function(c, hi, un, dir, s_n, est, tmp);
+        double residual = 1.0;
+        while (residual > tol && iter < limit) {
+            if (rlo * rhi > 0) {
+                lo -= 0.1;
+                hi += 0.1;
+            }
+            gamma_kp1 = 0.5 * (lo + hi);
+            const double r = rrk_root_function(c, gamma_kp1, un, dir, s_n, est, tmp);
+            if (r < 0) { lo = gamma_kp1; rlo = r; }
+            else { hi = gamma_kp1; rhi = r; }
+            residual = hi - lo;
+            iter++;
+        }
+    }
+    double advanced = -1.0;
+    if (iter < limit && gamma_kp1 >= 0.5) {
+        /* sum_stages with the relaxed step gamma dt (runge_kutta_ode_solver.cpp:86-98) */
+        const double dtr = dt * gamma_kp1;
+        memcpy(u, un, sizeof(double) * nd);
+        for (int i = 0; i < B.s; ++i)
+            for (long q = 0; q < nd; ++q) u[q] += dtr * B.b[i] * stage[i][q];
+        *gamma_io = gamma_kp1;
+        advanced = dtr;
+    }
+    for (int i = 0; i < B.s; ++i) { free(stage[i]); free(stage_sol[i]); }
+    free(un); free(scratch); free(tmp); free(dir);
+    (void)d;
+    return advanced;
 }
 
 /* FlowSolver::run time loop (flow_solver.cpp:554-610) with constant dt */

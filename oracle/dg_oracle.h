@@ -143,6 +143,9 @@ void dgo_step(dgo_ctx *c, int method, double dt, double *u);
 /* one low-storage step (methods 10..13) that also returns the error norm of its embedded pair
    (low_storage_runge_kutta_ode_solver.cpp:89-118, before the epsilon/PID update) */
 double dgo_step_error_norm(dgo_ctx *c, int method, double dt, double *u, double atol, double rtol);
+/* relaxation RK (root finding on the integrated numerical entropy) for a Butcher method:
+   *gamma in = previous relaxation parameter, out = this step's; returns gamma*dt (-1 on failure) */
+double dgo_step_rrk(dgo_ctx *c, int method, double dt, double *u, double *gamma, double tol);
 /* get_automatic_initial_step_size (low_storage_runge_kutta_ode_solver.cpp:119-178) */
 double dgo_initial_step_size(dgo_ctx *c, const double *u);
 double dgo_constant_time_step(const dgo_ctx *c, double cfl);

@@ -33,7 +33,7 @@ EXPORTED = [
     "dgx_enable_timing", "dgx_kernel_timing", "dgx_set_initial_condition_tgv",
     "dgx_set_initial_condition_vortex", "dgx_synchronize", "dgx_set_non_physical_behavior", "dgx_check_state",
     "dgx_nonphysical_events", "dgx_set_error_adaptive_parameters", "dgx_get_automatic_initial_step_size",
-    "dgx_get_error_adaptive_step_size",
+    "dgx_get_error_adaptive_step_size", "dgx_set_relaxation_runge_kutta", "dgx_relaxation_parameter",
     "dgx_batch_create", "dgx_batch_destroy", "dgx_batch_last_error", "dgx_batch_size", "dgx_batch_instance",
     "dgx_batch_allocate", "dgx_batch_step_in_time", "dgx_batch_synchronize", "dgx_batch_variant",
     "dgx_request_bytes", "dgx_request_n_dofs", "dgx_server_create", "dgx_server_destroy", "dgx_server_last_error",
@@ -137,6 +137,9 @@ def load():
     lib.dgx_set_error_adaptive_parameters.argtypes = [vp, d, d, d, d, d]
     lib.dgx_get_automatic_initial_step_size.argtypes = [vp, dp]
     lib.dgx_get_error_adaptive_step_size.argtypes = [vp, d, dp]
+    lib.dgx_set_relaxation_runge_kutta.argtypes = [vp, i, d]
+    lib.dgx_relaxation_parameter.restype = d
+    lib.dgx_relaxation_parameter.argtypes = [vp]
     lib.dgx_apply_inverse_mass_device.argtypes = [vp, vp, vp]
     lib.dgx_batch_create.argtypes = [P(Params), i, P(vp)]
     lib.dgx_batch_destroy.argtypes = [vp]
@@ -345,6 +348,14 @@ class DG:
             row["eps_vorticity"] = q["vorticity_dissipation"]
         row["incompressible_kinetic_energy"] = q["incompressible_kinetic_energy"]
         return row
+
+    def set_relaxation_runge_kutta(self, enable=True, root_tolerance=5e-10):
+        self._chk(load().dgx_set_relaxation_runge_kutta(self.h, int(enable), root_tolerance),
+                  "set_relaxation_runge_kutta")
+
+    @property
+    def relaxation_parameter(self):
+        return load().dgx_relaxation_parameter(self.h)
 
     def set_error_adaptive_parameters(self, atol=1e-3, rtol=1e-3, beta=(0.70, -0.23, 0.0)):
         self._chk(load().dgx_set_error_adaptive_parameters(self.h, atol, rtol, *beta), "set_error_adaptive_parameters")

@@ -371,6 +371,7 @@ int dgx_destroy(dgx_ctx *c)
     (void)hipFree(c->iq_ops);
     (void)hipFree(c->iq_partial);
     (void)hipFree(c->vr_partial);
+    (void)hipFree(c->rrk_partial);
     if (c->timing.a) (void)hipEventDestroy(c->timing.a);
     if (c->timing.b) (void)hipEventDestroy(c->timing.b);
     delete c;
@@ -667,6 +668,8 @@ int dgx_allocate_ode_system(dgx_ctx *c, int method)
     c->method = method;
     c->lsrk_cur = 0;
     c->epsilon[0] = c->epsilon[1] = c->epsilon[2] = 1.0;
+    c->rrk = false;  // (re-)enabled with dgx_set_relaxation_runge_kutta
+    c->rrk_gamma = 1.0;
     return DGX_OK;
 }
 
@@ -721,7 +724,12 @@ static int butcher_stage_args(dgx_ctx *c, const Butcher &B, double dt, int i, St
     a.last_stage = (i == B.s - 1);
     a.stage = i;
     a.b_dt = dt * B.b[i];
-    a.k_store = need_k[i] ? c->regs[3 + i] : nullptr;
+    a.k_store = (need_k[i] || c->rrk) ? c->regs[3 + i] : nullptr;
+    if (c->rrk) {
+        // relaxation RK: the accumulator gathers d = dt sum b_i k_i from zero and u_n is kept
+        a.acc_zero = 1;
+        a.acc_out = acc;
+    }
     if (i < B.s - 1) {
         a.u_next = stage_buf[i & 1];
         for (int j = 0; j <= i; ++j) {
@@ -734,8 +742,99 @@ static int butcher_stage_args(dgx_ctx *c, const Butcher &B, double dt, int i, St
     return DGX_OK;
 }
 
-static int step_butcher(dgx_ctx *c, const Butcher &B, double dt)
+// Relaxation parameter of a relaxation-RK step (RRKODESolverBase::update_relaxation_parameter,
+// rrk_ode_solver_base.cpp:16-35; RootFindingRRKODESolver::compute_relaxation_parameter,
+// root_finding_rrk_ode_solver.cpp:16-176): the root of r(g) = S(u_n + g d) - S(u_n) - g est, S the
+// integrated numerical entropy, d = dt sum b_i k_i, est = dt sum b_i v(u^(i)) . M k_i; secant
+// iteration from the previous parameter +- 1e-5 with the reference's safeguards, bisection if it
+// does not converge in 100 iterations.
+static int rrk_relaxation(dgx_ctx *c, const Butcher &B, double dt, double *gamma_out)
 {
+    const int nb = vr_blocks();
+    double est = 0.0;
+    for (int i = 0; i < B.s; ++i) {
+        double ent = 0.0;
+        int rc = fetch_partial_sum(c, c->rrk_partial + (size_t)i * nb, &ent);
+        if (rc) return rc;
+        est += B.b[i] * ent;
+    }
+    est *= dt;
+    const double *un = c->u_dev, *dir = c->regs[0];
+    double s_n = 0.0;
+    int rc = launch_entropy_combo(c, un, nullptr, 0.0, &s_n);
+    if (rc) return rc;
+    auto root = [&](double g, double *r) {
+        double sg = 0.0;
+        int e = launch_entropy_combo(c, un, dir, g, &sg);
+        *r = sg - s_n - g * est;
+        return e;
+    };
+    const int limit = 100;
+    int iter = 0;
+    double gamma_kp1 = 0.0;
+    bool failed = false;
+    {
+        double residual = 1.0, gk = c->rrk_gamma + 1e-5, gkm1 = c->rrk_gamma - 1e-5, rk = 0.0, rkm1 = 0.0;
+        if ((rc = root(gk, &rk)) || (rc = root(gkm1, &rkm1))) return rc;
+        while (residual > c->rrk_tol && iter < limit) {
+            if (rkm1 == rk) {
+                gk *= 1.001;
+                if ((rc = root(gkm1, &rkm1)) || (rc = root(gk, &rk))) return rc;
+            }
+            if (gk < 0.5 || gk > 1.5) {
+                gk = 1.0;
+                if ((rc = root(gk, &rk))) return rc;
+            }
+            gamma_kp1 = gk - rk * (gk - gkm1) / (rk - rkm1);
+            residual = std::fabs(gamma_kp1 - gk);
+            iter++;
+            gkm1 = gk;
+            gk = gamma_kp1;
+            rkm1 = rk;
+            if ((rc = root(gk, &rk))) return rc;
+            if (std::isnan(gk) || std::isnan(gkm1)) {
+                gk = 1.0 - 1e-5;
+                if ((rc = root(gk, &rk))) return rc;
+                gkm1 = 1.0 + 1e-5;
+                if ((rc = root(gkm1, &rkm1))) return rc;
+                residual = 1.0;
+            }
+        }
+        failed = (iter == limit);
+    }
+    if (failed) {
+        iter = 0;
+        double lo = c->rrk_gamma - 0.1, hi = c->rrk_gamma + 0.1, rlo = 0.0, rhi = 0.0, residual = 1.0;
+        if ((rc = root(lo, &rlo)) || (rc = root(hi, &rhi))) return rc;
+        while (residual > c->rrk_tol && iter < limit) {
+            if (rlo * rhi > 0) {
+                lo -= 0.1;
+                hi += 0.1;
+            }
+            gamma_kp1 = 0.5 * (lo + hi);
+            double r = 0.0;
+            if ((rc = root(gamma_kp1, &r))) return rc;
+            if (r < 0) {
+                lo = gamma_kp1;
+                rlo = r;
+            } else {
+                hi = gamma_kp1;
+                rhi = r;
+            }
+            residual = hi - lo;
+            iter++;
+        }
+    }
+    if (iter == limit || !(gamma_kp1 >= 0.5))
+        return fail(c, DGX_ERR_NONPHYSICAL, "relaxation RK failed to find a reasonable relaxation parameter");
+    *gamma_out = gamma_kp1;
+    return DGX_OK;
+}
+
+// One Butcher step; *dt_eff is the time advanced (gamma dt with relaxation RK, else dt).
+static int step_butcher(dgx_ctx *c, const Butcher &B, double dt, double *dt_eff)
+{
+    *dt_eff = dt;
     for (int i = 0; i < B.s; ++i) {
         StageArgs a;
         int rc = butcher_stage_args(c, B, dt, i, a);
@@ -746,6 +845,20 @@ static int step_butcher(dgx_ctx *c, const Butcher &B, double dt)
         }
         rc = launch_stage(c, a);
         if (rc) return rc;
+        if (c->rrk) {  // entropy-change estimate of this stage, before its state buffer is reused
+            rc = launch_rrk_estimate(c, a.u, a.k_store, c->rrk_partial + (size_t)i * vr_blocks());
+            if (rc) return rc;
+        }
+    }
+    if (c->rrk) {
+        double gamma = 1.0;
+        int rc = rrk_relaxation(c, B, dt, &gamma);
+        if (rc) return rc;
+        rc = launch_axpy(c, c->u_dev, gamma, c->regs[0], c->u_dev, c->ndofs);  // u_n + gamma d
+        if (rc) return rc;
+        c->rrk_gamma = gamma;
+        *dt_eff = gamma * dt;
+        return DGX_OK;
     }
     if (B.s == 1) std::swap(c->u_dev, c->regs[0]); // solution = solution_update
     return DGX_OK;
@@ -809,13 +922,35 @@ int dgx_step_in_time(dgx_ctx *c, double dt, int n_steps)
     const bool isb = get_butcher(c->method, B);
     if (!isb) get_lsrk(c->method, L);
     for (int s = 0; s < n_steps; ++s) {
-        int rc = isb ? step_butcher(c, B, dt) : step_lsrk(c, L, dt);
+        double dt_eff = dt;
+        int rc = isb ? step_butcher(c, B, dt, &dt_eff) : step_lsrk(c, L, dt);
         if (rc) return rc;
         ++c->iteration;
-        c->current_time += dt;
+        c->current_time += dt_eff;
     }
     return check_nonphysical(c, "step_in_time");
 }
+
+int dgx_set_relaxation_runge_kutta(dgx_ctx *c, int enable, double root_tolerance)
+{
+    if (!c || !(root_tolerance > 0.0)) return DGX_ERR_INVALID;
+    Butcher B;
+    if (enable && (c->method < 0 || !get_butcher(c->method, B)))
+        return fail(c, DGX_ERR_STATE, "relaxation RK needs an explicit Butcher method (allocate_ode_system first)");
+    if (enable && c->P.pde != DGX_EULER && c->P.pde != DGX_NAVIER_STOKES)
+        return fail(c, DGX_ERR_UNSUPPORTED, "relaxation RK is for Euler / Navier-Stokes");
+    if (enable && !c->rrk_partial) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, sizeof(double) * 4 * (size_t)vr_blocks()));
+        c->rrk_partial = static_cast<double *>(p);
+    }
+    c->rrk = enable != 0;
+    c->rrk_tol = root_tolerance;
+    c->rrk_gamma = 1.0;
+    return DGX_OK;
+}
+
+double dgx_relaxation_parameter(const dgx_ctx *c) { return c ? c->rrk_gamma : 0.0; }
 
 int dgx_set_error_adaptive_parameters(dgx_ctx *c, double atol, double rtol, double beta1, double beta2, double beta3)
 {

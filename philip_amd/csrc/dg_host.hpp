@@ -96,6 +96,11 @@ struct dgx_ctx {
     double *vr_partial = nullptr;  // per-block partials of the vector reductions
     double atol = 1e-3, rtol = 1e-3, beta_pid[3] = {0.70, -0.23, 0.0};
     double epsilon[3] = {1.0, 1.0, 1.0};
+    // relaxation RK (rrk_ode_solver_base.cpp, root_finding_rrk_ode_solver.cpp)
+    bool rrk = false;
+    double rrk_gamma = 1.0;        // relaxation parameter of the last step (1 before the first)
+    double rrk_tol = 5e-10;        // relaxation_runge_kutta_root_tolerance
+    double *rrk_partial = nullptr; // per-stage block partials of the entropy-change estimate
 };
 
 // kernel launch helpers (dg_launch.hip)
@@ -118,5 +123,10 @@ int launch_line_batch(int dim, int n, int precision, int es, const BatchBlock *b
 int launch_aux(dgx_ctx *c, const double *u);
 int launch_integrated(dgx_ctx *c, const double *u, int overint, double *host_out4);
 int launch_entropy(dgx_ctx *c, const double *u, double *host_out);
+// integrated numerical entropy of u + gamma dir (dir == nullptr: of u)
+int launch_entropy_combo(dgx_ctx *c, const double *u, const double *dir, double gamma, double *host_out);
+int launch_rrk_estimate(dgx_ctx *c, const double *u, const double *k, double *partial_dev);
+int fetch_partial_sum(dgx_ctx *c, const double *partial_dev, double *out);
+int vr_blocks();
 int launch_wave_speed(dgx_ctx *c, const double *u, int overint, double *host_out);
 }

@@ -59,6 +59,7 @@ struct StageArgs {
     double a_next[4];         // a[i+1][j], j <= i
     double b_dt;              // dt * b_i
     int first_stage, last_stage, stage;
+    int acc_zero;             // relaxation RK: the accumulator holds dt sum b_i k_i (starts at 0)
     // set to 1 by any thread that finds a non-physical stage state (density or pressure
     // negative or NaN; euler.cpp:155-173, 486-500); read and cleared by the host
     int *status;
@@ -188,7 +189,7 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs &a, const Ops1D &
         for (int s = 0; s < NS; ++s) {
             const long idx = base + s * NP;
             const double kk = dudt[s];
-            double accv = a.first_stage ? a.u0[idx] : a.acc[idx];
+            double accv = a.first_stage ? (a.acc_zero ? 0.0 : a.u0[idx]) : a.acc[idx];
             accv += a.b_dt * kk;
             a.acc_out[idx] = accv;
             if (a.k_store) a.k_store[idx] = kk;

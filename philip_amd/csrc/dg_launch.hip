@@ -501,6 +501,9 @@ integrated_kernel(const double *__restrict__ u, const double *__restrict__ ops, 
         for (int f = 0; f < 3; ++f) partial[(long)e * 3 + f] = red[f * IQ_THREADS];
 }
 
+// maximum that keeps a NaN (fmax drops it): a non-physical state reaches the host
+__device__ __forceinline__ double nan_max(double a, double b) { return (a != a || b != b) ? a + b : (a > b ? a : b); }
+
 // Maximum local wave speed |u| + c over the QGauss(nx)^dim points of every cell
 // (periodic_turbulence.cpp:371-422, euler.cpp:1033-1046): all states interpolated
 // sum-factorised (x pass in LDS, y and z passes in registers); per-cell maxima.
@@ -559,13 +562,13 @@ wave_speed_kernel(const double *__restrict__ u, const double *__restrict__ ops, 
             }
             const double p = (gam - 1.0) * (us[NS - 1] - 0.5 * rho * vel2);
             const double ev = sqrt(vel2) + sqrt(gam * p / rho);
-            vmax = ev > vmax ? ev : vmax;
+            vmax = (ev > vmax || ev != ev) ? ev : vmax;  // a NaN (non-physical state) propagates
         }
     }
     red[threadIdx.x] = vmax;
     __syncthreads();
     for (int sh = IQ_THREADS / 2; sh > 0; sh >>= 1) {
-        if ((int)threadIdx.x < sh) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + sh]);
+        if ((int)threadIdx.x < sh) red[threadIdx.x] = nan_max(red[threadIdx.x], red[threadIdx.x + sh]);
         __syncthreads();
     }
     if (threadIdx.x == 0) cellmax[e] = red[0];
@@ -576,11 +579,11 @@ max_reduce_kernel(const double *__restrict__ v, int n, double *__restrict__ out)
 {
     __shared__ double red[IQ_THREADS];
     double m = 0.0;
-    for (int i = threadIdx.x; i < n; i += IQ_THREADS) m = fmax(m, v[i]);
+    for (int i = threadIdx.x; i < n; i += IQ_THREADS) m = nan_max(m, v[i]);
     red[threadIdx.x] = m;
     __syncthreads();
     for (int sh = IQ_THREADS / 2; sh > 0; sh >>= 1) {
-        if ((int)threadIdx.x < sh) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + sh]);
+        if ((int)threadIdx.x < sh) red[threadIdx.x] = nan_max(red[threadIdx.x], red[threadIdx.x + sh]);
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = red[0];
@@ -590,8 +593,8 @@ max_reduce_kernel(const double *__restrict__ v, int n, double *__restrict__ out)
 // euler.cpp:437-447): per cell sum_i W_i det J (-rho_i ln(p_i rho_i^-gamma)); the three
 // partial slots of a cell hold (entropy, 0, 0) so the fixed-order reduction below applies.
 __global__ void __launch_bounds__(IQ_THREADS)
-entropy_kernel(const double *__restrict__ u, int n, int dim, int ncells, double detJ, double gam, Ops1D ops,
-               double *__restrict__ partial)
+entropy_kernel(const double *__restrict__ u, const double *__restrict__ dir, double gamma, int n, int dim,
+               int ncells, double detJ, double gam, Ops1D ops, double *__restrict__ partial)
 {
     __shared__ double red[IQ_THREADS];
     const int e = blockIdx.x;
@@ -599,7 +602,10 @@ entropy_kernel(const double *__restrict__ u, int n, int dim, int ncells, double 
     double acc = 0.0;
     for (int i = threadIdx.x; i < Np; i += blockDim.x) {
         double us[5];
-        for (int s = 0; s < ns; ++s) us[s] = u[((long)e * ns + s) * Np + i];
+        for (int s = 0; s < ns; ++s) {
+            const long idx = ((long)e * ns + s) * Np + i;
+            us[s] = dir ? u[idx] + gamma * dir[idx] : u[idx];  // u, or u + gamma dir (relaxation RK)
+        }
         const double rho = us[0];
         double vel2 = 0.0;
         for (int k = 0; k < dim; ++k) {
@@ -646,15 +652,78 @@ integrated_sum_kernel(const double *__restrict__ partial, int ncells, double *__
         for (int f = 0; f < 3; ++f) out3[f] = red[f][0];
 }
 
-int launch_entropy(dgx_ctx *c, const double *u, double *out)
+int launch_entropy(dgx_ctx *c, const double *u, double *out) { return launch_entropy_combo(c, u, nullptr, 0.0, out); }
+
+// per-block partials of sum over nodes of W det J v(u) . k, v the entropy variables of the stage
+// solution (compute_entropy_change_estimate, root_finding_rrk_ode_solver.cpp:357-390; collocated
+// GLL: the projection of v is v, M = W det J)
+template <int D>
+__global__ void __launch_bounds__(VR_THREADS)
+rrk_estimate_kernel(const double *__restrict__ u, const double *__restrict__ k, int n, int ncells, double detJ,
+                    Ops1D ops, PhysConst pc, double *__restrict__ partial)
+{
+    constexpr int NS = D + 2;
+    __shared__ double red[VR_THREADS];
+    const int Np = (D == 3) ? n * n * n : n * n;
+    const long nodes = (long)ncells * Np;
+    double acc = 0.0;
+    for (long t = blockIdx.x * (long)VR_THREADS + threadIdx.x; t < nodes; t += (long)gridDim.x * VR_THREADS) {
+        const long e = t / Np;
+        const int i = (int)(t % Np);
+        double us[NS], v[NS];
+        for (int s = 0; s < NS; ++s) us[s] = u[(e * NS + s) * Np + i];
+        entropy_variables<double, D>(us, pc, v);
+        double W = ops.w[i % n] * ops.w[(i / n) % n];
+        if (D == 3) W *= ops.w[i / (n * n)];
+        for (int s = 0; s < NS; ++s) acc += v[s] * (W * detJ * k[(e * NS + s) * Np + i]);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int sh = VR_THREADS / 2; sh > 0; sh >>= 1) {
+        if ((int)threadIdx.x < sh) red[threadIdx.x] += red[threadIdx.x + sh];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+int launch_rrk_estimate(dgx_ctx *c, const double *u, const double *k, double *partial_dev)
+{
+    const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
+    if (c->dim == 3)
+        hipLaunchKernelGGL(rrk_estimate_kernel<3>, dim3(VR_BLOCKS), dim3(VR_THREADS), 0, c->stream, u, k, c->n,
+                           c->ncells, detJ, c->ops, c->pc, partial_dev);
+    else
+        hipLaunchKernelGGL(rrk_estimate_kernel<2>, dim3(VR_BLOCKS), dim3(VR_THREADS), 0, c->stream, u, k, c->n,
+                           c->ncells, detJ, c->ops, c->pc, partial_dev);
+    return check_launch(c, "rrk_estimate_kernel");
+}
+
+// sum of the VR_BLOCKS partials at partial_dev, in block order, on the host
+int fetch_partial_sum(dgx_ctx *c, const double *partial_dev, double *out)
+{
+    double h[VR_BLOCKS];
+    if (hipMemcpyAsync(h, partial_dev, sizeof(double) * VR_BLOCKS, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        c->err = "reduction copy failed";
+        return DGX_ERR_HIP;
+    }
+    double acc = 0.0;
+    for (int b = 0; b < VR_BLOCKS; ++b) acc += h[b];
+    *out = acc;
+    return DGX_OK;
+}
+
+int vr_blocks() { return VR_BLOCKS; }
+
+int launch_entropy_combo(dgx_ctx *c, const double *u, const double *dir, double gamma, double *out)
 {
     if (!c->iq_partial && hipMalloc(&c->iq_partial, sizeof(double) * (3 * (size_t)c->ncells + 3)) != hipSuccess) {
         c->err = "hipMalloc failed";
         return DGX_ERR_HIP;
     }
     const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
-    hipLaunchKernelGGL(entropy_kernel, dim3(c->ncells), dim3(IQ_THREADS), 0, c->stream, u, c->n, c->dim, c->ncells,
-                       detJ, c->P.gamma_gas, c->ops, c->iq_partial);
+    hipLaunchKernelGGL(entropy_kernel, dim3(c->ncells), dim3(IQ_THREADS), 0, c->stream, u, dir, gamma, c->n, c->dim,
+                       c->ncells, detJ, c->P.gamma_gas, c->ops, c->iq_partial);
     int rc = check_launch(c, "entropy_kernel");
     if (rc) return rc;
     double *sums = c->iq_partial + 3 * (size_t)c->ncells;

@@ -64,7 +64,7 @@ template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
                                             : (N >= 7 ? (FAST ? 1 : 3) : (N >= 6 ? 2 : 1));
     static constexpr int ESV = pad1(LS * D * NS * NP);  // line partials [direction][part][state][node]
     static constexpr int NFN = 2 * D * NL;           // face nodes per element
-    static constexpr int ESN = pad1(NS * NFN);       // neighbour traces [state][face][face node]
+    static constexpr int ESN = pad1(NQ * NFN);       // neighbour traces' caches [field][face][face node]
     static constexpr int ITEMS = EPB * D * NL * LS;
     static constexpr int FITEMS = EPB * NFN;
     static constexpr int ESF = pad1(2 * D * NS * NL);   // face terms [face][state][face node]
@@ -73,7 +73,7 @@ template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
     // their face nodes [state][face][face node]
     static_assert(!V || FAST, "the Navier-Stokes line kernel is the mixed-precision path");
     static constexpr int ESFV = V ? pad1(D * NS * NP) : 0;
-    static constexpr int ESNV = V ? ESN : 0;
+    static constexpr int ESNV = V ? pad1(NS * NFN) : 0;
     static constexpr int EIN = ESQ + ESN + ESFV + ESNV;
     static constexpr int SBUF = (EIN > ESV + ESF) ? EPB * EIN : EPB * (ESV + ESF);
 };
@@ -236,7 +236,7 @@ __device__ __forceinline__ void face_visc(const PhysConst &pc, T pen, const T *u
 template <typename T> struct FaceIn {
     const T *sq;    // node caches of the block
     const T *sfv;   // viscous flux of the block's nodes, every direction (Navier-Stokes)
-    const T *tr;    // this face node's neighbour trace, stride tst
+    const T *tr;    // this face node's neighbour trace (node cache fields, stride tst)
     const T *trv;   // the neighbour's viscous flux along the face direction there (Navier-Stokes)
     int tst;
 };
@@ -246,7 +246,9 @@ template <typename T> struct FaceIn {
 // interior side), times -1/w_0 for the interior and +1/w_0 for the exterior
 // side, so the neighbour's term is the negative.  nbl >= 0: the neighbour is
 // element nbl of this block (node cache in LDS); otherwise its trace is tr
-// (state-major, stride tst).
+// (field-major, stride tst).  Both sides' caches were filled by node_fill in
+// phase A, so a face evaluated in two blocks (once from each side) sees the
+// same bits on both and the flux is the same whichever block holds the pair.
 template <typename T, int D, int N, int K, bool V>
 __device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, const FastConst<T> &fc,
                                        const FaceIn<T> &in, int le, int side, int fn, int e, int nb, int nbl,
@@ -258,12 +260,8 @@ __device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, co
     const int fj = line_node<D, N>(K, fn, side ? 0 : N - 1);
     NodeC<T, D> qi, qn, L, R;
     nodef_load<T, D>(in.sq + le * LG::ESQ + fi, NP, qi);
-    if (nbl >= 0) {
-        nodef_load<T, D>(in.sq + nbl * LG::ESQ + fj, NP, qn);
-    } else {
-        for (int s = 0; s < NS; ++s) qn.u[s] = in.tr[s * in.tst];
-        node_fill<T, D>(qn, fc);
-    }
+    if (nbl >= 0) nodef_load<T, D>(in.sq + nbl * LG::ESQ + fj, NP, qn);
+    else nodef_load<T, D>(in.tr, in.tst, qn);
     const bool own = e < nb;
     nodef_select<T, D>(own, qi, qn, L);
     nodef_select<T, D>(own, qn, qi, R);
@@ -311,7 +309,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     // result in registers, so the line and face partials reuse their storage.
     __shared__ T sBuf[LG::SBUF];
     T *const sQ = sBuf;                       // per-node cache [element][field][node]
-    T *const sN = sQ + EPB * LG::ESQ;         // neighbour face traces [element][state][face][face node]
+    T *const sN = sQ + EPB * LG::ESQ;         // neighbour face traces' caches [element][field][face][face node]
     T *const sFv = sN + EPB * LG::ESN;        // V: viscous flux [element][direction][state][node]
     T *const sNv = sFv + EPB * LG::ESFV;      // V: neighbours' viscous flux, as the traces
     T *const sV = sBuf;                       // line partials [element][direction][state][node]
@@ -365,7 +363,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
                 if (!in_block(nb, fe)) {
                     const int fj = line_node<D, N>(f >> 1, fn, (f & 1) ? 0 : N - 1);
                     for (int s = 0; s < NS; ++s) un[j][s] = a.u[((long)nb * NS + s) * NP + fj];
-                    fdst[j] = fle * LG::ESN + r;
+                    fdst[j] = t;  // = fle * NFN + r
                 }
             }
         }
@@ -380,8 +378,10 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
 #pragma unroll
     for (int j = 0; j < NFI; ++j) {
         if (fdst[j] < 0) continue;
-        T *dst = sN + fdst[j];
-        for (int s = 0; s < NS; ++s) dst[s * LG::NFN] = T(un[j][s]);
+        NodeC<T, D> n;
+        for (int s = 0; s < NS; ++s) n.u[s] = T(un[j][s]);
+        node_fill<T, D>(n, fc);
+        nodef_store<T, D>(sN + (fdst[j] / LG::NFN) * LG::ESN + fdst[j] % LG::NFN, LG::NFN, n);
     }
     if constexpr (V) {
         // viscous fluxes from the lifted gradient: the own nodes' in every direction, the
@@ -413,7 +413,8 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
 #pragma unroll
             for (int k = 1; k < D; ++k)
                 for (int s = 0; s < NS; ++s) fk[s] = (f >> 1) == k ? fv[k][s] : fk[s];
-            for (int s = 0; s < NS; ++s) sNv[fdst[j] + s * LG::NFN] = fk[s];
+            for (int s = 0; s < NS; ++s)
+                sNv[(fdst[j] / LG::NFN) * LG::ESNV + fdst[j] % LG::NFN + s * LG::NFN] = fk[s];
         }
     }
     __syncthreads();
