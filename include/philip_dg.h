@@ -176,6 +176,17 @@ double dgx_constant_time_step(const dgx_ctx *ctx, double cfl);
    with QGauss(p+1+overint): out[0]=kinetic energy, [1]=enstrophy, [2]=incompressible KE,
    [3]=vorticity-based dissipation rate; reductions in FP64 on the device. */
 int dgx_integrated_quantities(dgx_ctx *ctx, int overint, double *out4);
+/* Allocation-free variant of dgx_integrated_quantities.  No counterpart in the reference: it is
+   for callers that cannot allocate device memory (a time loop captured in a HIP graph that
+   records the kinetic-energy history on the device, a serving loop that must not hipMalloc).
+   Same kernels and same results, bit for bit; it allocates nothing (no hipMalloc/hipFree, no
+   host synchronisation) and only enqueues work on the context's stream: u_dev (n_dofs doubles
+   on the device; NULL = the context's solution) in, out4_dev (4 doubles on the device, in the
+   order of dgx_integrated_quantities) out.  Its scratch (per-cell partial sums, the three sums,
+   the QGauss table) is one static 6.3 MB device array per GPU: at most 262144 cells and
+   p + 1 + overint <= 24 (DGX_ERR_INVALID beyond).  Calls that overlap on one GPU (different
+   streams) race on that array. */
+int dgx_integrated_quantities_static(dgx_ctx *ctx, const double *u_dev, int overint, double *out4_dev);
 /* PeriodicTurbulence::compute_current_integrated_numerical_entropy (periodic_turbulence.cpp:765-868):
    sum over cells and GLL volume nodes of W det J (-rho ln(p rho^-gamma)), FP64 on the device. */
 int dgx_numerical_entropy(dgx_ctx *ctx, double *out);

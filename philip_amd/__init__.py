@@ -29,6 +29,7 @@ EXPORTED = [
     "dgx_assemble_residual_device", "dgx_rhs_device", "dgx_apply_inverse_mass_device",
     "dgx_allocate_ode_system", "dgx_step_in_time", "dgx_current_time", "dgx_current_iteration", "dgx_set_time",
     "dgx_set_track_error_estimate", "dgx_constant_time_step", "dgx_integrated_quantities",
+    "dgx_integrated_quantities_static",
     "dgx_numerical_entropy", "dgx_max_wave_speed", "dgx_adaptive_time_step", "dgx_set_stream",
     "dgx_enable_timing", "dgx_kernel_timing", "dgx_set_initial_condition_tgv",
     "dgx_set_initial_condition_vortex", "dgx_synchronize", "dgx_set_non_physical_behavior", "dgx_check_state",
@@ -124,6 +125,7 @@ def load():
     lib.dgx_constant_time_step.argtypes = [vp, d]
     lib.dgx_integrated_quantities.argtypes = [vp, i, dp]
     lib.dgx_numerical_entropy.argtypes = [vp, dp]
+    lib.dgx_integrated_quantities_static.argtypes = [vp, vp, i, vp]
     lib.dgx_max_wave_speed.argtypes = [vp, i, dp]
     lib.dgx_adaptive_time_step.argtypes = [vp, d, dp]
     lib.dgx_set_stream.argtypes = [vp, vp]

@@ -1069,6 +1069,12 @@ int dgx_integrated_quantities(dgx_ctx *c, int overint, double *out4)
     return launch_integrated(c, c->u_dev, overint, out4);
 }
 
+int dgx_integrated_quantities_static(dgx_ctx *c, const double *u_dev, int overint, double *out4_dev)
+{
+    if (!c || !out4_dev) return DGX_ERR_INVALID;
+    return launch_integrated_static(c, u_dev ? u_dev : c->u_dev, overint, out4_dev);
+}
+
 int dgx_numerical_entropy(dgx_ctx *c, double *out)
 {
     if (!c || !out) return DGX_ERR_INVALID;

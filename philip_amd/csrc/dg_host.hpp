@@ -122,6 +122,7 @@ int launch_line_batch(int dim, int n, int precision, int es, const BatchBlock *b
                       const StageArgs *args, const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb);
 int launch_aux(dgx_ctx *c, const double *u);
 int launch_integrated(dgx_ctx *c, const double *u, int overint, double *host_out4);
+int launch_integrated_static(dgx_ctx *c, const double *u, int overint, double *out4_dev);
 int launch_entropy(dgx_ctx *c, const double *u, double *host_out);
 // integrated numerical entropy of u + gamma dir (dir == nullptr: of u)
 int launch_entropy_combo(dgx_ctx *c, const double *u, const double *dir, double gamma, double *host_out);

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -858,6 +859,122 @@ int launch_integrated(dgx_ctx *c, const double *u, int overint, double *out4)
     out4[2] = s[2] / vol;
     out4[3] = 2.0 * out4[1] / c->P.reynolds_inf;
     return DGX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free integrated quantities, for callers that cannot allocate device memory (a
+// time loop captured in a HIP graph that records the kinetic-energy history on the device).
+// The per-cell partial sums, the three sums and the QGauss operator table live in `scratch`, a
+// file-scope device array: each GPU the library runs on holds its own copy until the process
+// ends.  Its device address is looked up once per device (hipGetSymbolAddress, with that
+// device current) and only that address is handed to kernels.  Every call on one GPU shares the
+// array: calls that overlap on one GPU (different streams) race.  Nothing in it has to start at
+// zero: every call writes the table, all the partials it reads and the sums before reading them.
+// ---------------------------------------------------------------------------
+constexpr int IQS_MAX_CELLS = 1 << 18;                           // 262,144 cells (64^3)
+constexpr int IQS_MAX_NX = 24;                                   // QGauss points per direction
+constexpr int IQS_OPS = 2 * IQS_MAX_NX * (MAXN1D - 1) + IQS_MAX_NX;  // B, G [nx][n], w[nx], n <= 7
+constexpr size_t IQS_SUMS = 3 * (size_t)IQS_MAX_CELLS;           // offset of the three sums
+constexpr size_t IQS_TABLE = IQS_SUMS + 8;                       // offset of the operator table
+constexpr size_t IQS_DOUBLES = IQS_TABLE + IQS_OPS;              // 6.3 MB in all
+static __device__ __align__(256) double scratch[IQS_DOUBLES];
+
+struct IqTable {
+    double v[IQS_OPS];  // passed by value as a kernel argument (2.9 KB)
+};
+
+__global__ void __launch_bounds__(256) iq_table_kernel(IqTable t, int count, double *__restrict__ dst)
+{
+    for (int i = threadIdx.x; i < count; i += blockDim.x) dst[i] = t.v[i];
+}
+
+// out4 as the host finishes dgx_integrated_quantities: sums / vol, dissipation 2 enstrophy / Re
+__global__ void __launch_bounds__(64) iq_finalize_kernel(const double *__restrict__ sums, double vol,
+                                                         double reynolds, double *__restrict__ out4)
+{
+    if (threadIdx.x == 0) {
+        const double ens = sums[1] / vol;
+        out4[0] = sums[0] / vol;
+        out4[1] = ens;
+        out4[2] = sums[2] / vol;
+        out4[3] = 2.0 * ens / reynolds;
+    }
+}
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[64];
+
+// Device address of `scratch` on the current device (looked up once per device).
+static int scratch_address(dgx_ctx *c, double **out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        c->err = "no current HIP device";
+        return DGX_ERR_HIP;
+    }
+    if (dev != c->device) {
+        c->err = "the context belongs to another device than the current one";
+        return DGX_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            c->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return DGX_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return DGX_OK;
+}
+
+int launch_integrated_static(dgx_ctx *c, const double *u, int overint, double *out4_dev)
+{
+    const int n = c->n, nx = c->p + 1 + overint;
+    if (overint < 0 || nx > IQS_MAX_NX) {
+        c->err = "allocation-free integrated quantities: p + 1 + overint must be at most 24";
+        return DGX_ERR_INVALID;
+    }
+    if (c->ncells > IQS_MAX_CELLS) {
+        c->err = "allocation-free integrated quantities: at most 262144 cells";
+        return DGX_ERR_INVALID;
+    }
+    double *sp = nullptr;
+    int rc = scratch_address(c, &sp);
+    if (rc) return rc;
+    double *partial = sp, *sums = sp + IQS_SUMS, *table = sp + IQS_TABLE;
+    IqTable t;
+    const int count = 2 * nx * n + nx;
+    iq_table(n, nx, t.v);
+    hipLaunchKernelGGL(iq_table_kernel, dim3(1), dim3(256), 0, c->stream, t, count, table);
+    rc = check_launch(c, "iq_table_kernel");
+    if (rc) return rc;
+    const int nr = (c->dim == 3) ? n * n : n;
+    const size_t shmem = sizeof(double) * ((size_t)(c->dim + 1) * c->Np + 2 * nx * n + nx +
+                                           (size_t)(c->dim + 1) * 2 * nx * nr + 3 * IQ_THREADS);
+    const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
+    switch (c->dim * 16 + n) {
+#define IQ_CASE(DD, NN)                                                                                          \
+    case DD * 16 + NN:                                                                                            \
+        hipLaunchKernelGGL((integrated_kernel<DD, NN>), dim3(c->ncells), dim3(IQ_THREADS), shmem, c->stream, u,   \
+                           table, nx, c->ncells, 1.0 / c->h, detJ, partial);                                      \
+        break;
+        IQ_CASE(3, 2) IQ_CASE(3, 3) IQ_CASE(3, 4) IQ_CASE(3, 5) IQ_CASE(3, 6) IQ_CASE(3, 7)
+        IQ_CASE(2, 2) IQ_CASE(2, 3) IQ_CASE(2, 4) IQ_CASE(2, 5) IQ_CASE(2, 6) IQ_CASE(2, 7)
+#undef IQ_CASE
+    default: c->err = "unsupported polynomial degree"; return DGX_ERR_UNSUPPORTED;
+    }
+    rc = check_launch(c, "integrated_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(integrated_sum_kernel, dim3(1), dim3(IQ_THREADS), 0, c->stream, partial, c->ncells, sums);
+    rc = check_launch(c, "integrated_sum_kernel");
+    if (rc) return rc;
+    double vol = 1.0;
+    for (int k = 0; k < c->dim; ++k) vol *= (c->P.domain_right - c->P.domain_left);
+    hipLaunchKernelGGL(iq_finalize_kernel, dim3(1), dim3(64), 0, c->stream, sums, vol, c->P.reynolds_inf, out4_dev);
+    return check_launch(c, "iq_finalize_kernel");
 }
 
 } // namespace dgx
