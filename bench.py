@@ -31,7 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNEL = {"fp64": "line_kernel<3,4,double>", "mixed": "line_kernel<3,4,float>"}
+KERNEL = {"fp64": "line_kernel_wb<3,4,double,3>", "mixed": "line_kernel<3,4,float>"}
 TRAFFIC_FILE = {"fp64": os.path.join(ROOT, "profiles", "line_kernel_fp64_tgv_p3_32cubed_traffic.txt"),
                 "mixed": os.path.join(ROOT, "profiles", "line_kernel_mixed_tgv_p3_32cubed_traffic.txt")}
 
@@ -184,6 +184,10 @@ def batched_vortex(n_inst=256, steps=200, seed=2024, precision="mixed"):
         d.set_initial_condition_vortex(*ic)
         dts.append(d.constant_time_step(0.05) / 2.0)
     b.allocate("rk4", dts)
+    tiles = {}
+    for k, kw in enumerate(kws):
+        epb, waves = b.tile(k)
+        tiles[f"p{kw['poly_degree']}"] = dict(elements_per_block=epb, waves_per_simd_bound=waves)
     b.step_in_time(2)
     b.synchronize()
     t0 = time.perf_counter()
@@ -194,6 +198,7 @@ def batched_vortex(n_inst=256, steps=200, seed=2024, precision="mixed"):
     b.close()
     return dict(value=total * 4 * steps / t, unit="DOF-updates/s", instances=n_inst, dof_total=total,
                 steps=steps, ms_per_step=1e3 * t / steps, precision=precision,
+                tiles=dict(sorted(tiles.items())),
                 workload="256 independent 2D vortex solves, N~U{8..64}, p~U{2..6}, RK4, one graph launch per step")
 
 

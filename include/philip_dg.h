@@ -251,17 +251,22 @@ int dgx_batch_size(const dgx_batch *b);
    owned by the batch. */
 dgx_ctx *dgx_batch_instance(dgx_batch *b, int i);
 /* Allocates every instance's RK registers for an explicit Butcher method with at least two
-   stages (Heun, SSPRK3, RK4) and the time step dt[i] of each instance, picks the
-   elements-per-block variant of each (dim, p, precision) group by timing the candidates
-   (cached per process), and captures one RK step of all instances in a HIP graph. */
+   stages (Heun, SSPRK3, RK4) and the time step dt[i] of each instance, picks the tile of each
+   (dim, p, precision) group (elements per block x waves/SIMD bound; 6 candidates in 2D, 2 in 3D)
+   by timing every candidate on the group's own instances (cached per process), and captures one
+   RK step of all instances in a HIP graph.  No tile changes an element's arithmetic: each
+   instance's result is its single solve's, bit for bit. */
 int dgx_batch_allocate(dgx_batch *b, int method, const double *dt);
 /* Advances every instance by n_steps steps (one graph launch per step); per-instance
    current_time / current_iteration advance as in dgx_step_in_time. */
 int dgx_batch_step_in_time(dgx_batch *b, int n_steps);
 int dgx_batch_synchronize(dgx_batch *b);
-/* Elements-per-block variant chosen for instance i's group (1 = 256-node blocks, 2 = twice),
-   0 if the instance is launched on its own. */
+/* Tile chosen for instance i's group, as 1 + its candidate index; 0 if the instance is launched
+   on its own (schemes other than split-form Euler). */
 int dgx_batch_variant(const dgx_batch *b, int i);
+/* The same tile as (elements per block, waves/SIMD bound; 0 = the compiler's allocation); both 0
+   for an instance launched on its own. */
+int dgx_batch_tile(const dgx_batch *b, int i, int *elements_per_block, int *waves_bound);
 
 /* ---------------------------------------------------------------------------
    Serving: a request loop over independent parametric solves.  No counterpart

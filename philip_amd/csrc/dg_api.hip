@@ -1162,7 +1162,7 @@ namespace {
 
 struct BatchGroup {
     int dim = 0, n = 0, precision = 0;
-    int es = 0;                          // elements-per-block variant; 0: per-instance launches
+    int tile = -1;                       // batched tile (batch_tile); -1: per-instance launches
     std::vector<int> inst;               // instance indices
     BatchBlock *blocks = nullptr;        // device block table
     int nblocks = 0;
@@ -1170,7 +1170,7 @@ struct BatchGroup {
     std::vector<StageArgs *> args;       // device, per stage: per slot
 };
 
-// elements-per-block variant chosen per (dim, n, precision) in this process
+// batched tile chosen per (dim, n, precision) in this process
 std::vector<std::pair<long, int>> g_variant_cache;
 
 } // namespace
@@ -1215,11 +1215,11 @@ static bool batch_line_eligible(const dgx_ctx *c)
     return c->P.use_split_form && c->P.pde == DGX_EULER;
 }
 
-// block table of a group for variant es: (slot, first element) per element group
-static int batch_blocks(dgx_batch *b, BatchGroup &g, int es)
+// block table of a group for tile `tile`: (slot, first element) per element group
+static int batch_blocks(dgx_batch *b, BatchGroup &g, int tile)
 {
     int epb = 0;
-    int rc = launch_line_batch(g.dim, g.n, g.precision, es, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr, &epb);
+    int rc = launch_line_batch(g.dim, g.n, g.precision, tile, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr, &epb);
     if (rc) return rc;
     std::vector<BatchBlock> tab;
     for (int slot = 0; slot < (int)g.inst.size(); ++slot) {
@@ -1231,12 +1231,13 @@ static int batch_blocks(dgx_batch *b, BatchGroup &g, int es)
     BHIPCHK(hipMalloc(&g.blocks, sizeof(BatchBlock) * tab.size()));
     BHIPCHK(hipMemcpy(g.blocks, tab.data(), sizeof(BatchBlock) * tab.size(), hipMemcpyHostToDevice));
     g.nblocks = (int)tab.size();
-    g.es = es;
+    g.tile = tile;
     return DGX_OK;
 }
 
-// Times the candidate variants on the group's stage-0 arguments in MODE_RHS (writing into
-// each instance's second stage buffer, which the step overwrites anyway) and keeps the fastest.
+// Times every candidate tile (elements per block x waves/SIMD bound, batch_tile) on the group's
+// stage-0 arguments in MODE_RHS (writing into each instance's second stage buffer, which the
+// step overwrites anyway; median of five launches after a warm-up) and keeps the fastest.
 static int batch_autotune(dgx_batch *b, BatchGroup &g, const Ops1D &ops)
 {
     const long key = ((long)g.dim * 16 + g.n) * 4 + g.precision;
@@ -1256,25 +1257,24 @@ static int batch_autotune(dgx_batch *b, BatchGroup &g, const Ops1D &ops)
     hipEvent_t e0, e1;
     BHIPCHK(hipEventCreate(&e0));
     BHIPCHK(hipEventCreate(&e1));
-    int best = 1;
+    int best = 0;
     float best_ms = 1e30f;
-    for (int es : {1, 2}) {
-        if (batch_blocks(b, g, es) != DGX_OK) continue;  // variant absent for this shape
-        float ms = 0.f;
-        for (int rep = 0; rep < 4; ++rep) {  // first launch is a warm-up
+    for (int tile = 0; tile < batch_tile_count(g.dim); ++tile) {
+        if (batch_blocks(b, g, tile) != DGX_OK) continue;  // tile does not fit this shape
+        float ts[6];
+        for (int rep = 0; rep < 6; ++rep) {  // first launch is a warm-up
             BHIPCHK(hipEventRecord(e0, b->stream));
-            if (launch_line_batch(g.dim, g.n, g.precision, es, g.blocks, g.nblocks, pa, g.pcs, ops, b->stream,
+            if (launch_line_batch(g.dim, g.n, g.precision, tile, g.blocks, g.nblocks, pa, g.pcs, ops, b->stream,
                                   nullptr))
                 return DGX_ERR_HIP;
             BHIPCHK(hipEventRecord(e1, b->stream));
             BHIPCHK(hipEventSynchronize(e1));
-            float t = 0.f;
-            BHIPCHK(hipEventElapsedTime(&t, e0, e1));
-            if (rep) ms += t;
+            BHIPCHK(hipEventElapsedTime(&ts[rep], e0, e1));
         }
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = es;
+        std::sort(ts + 1, ts + 6);
+        if (ts[3] < best_ms) {
+            best_ms = ts[3];
+            best = tile;
         }
     }
     (void)hipEventDestroy(e0);
@@ -1338,6 +1338,21 @@ int dgx_batch_variant(const dgx_batch *b, int i)
 {
     if (!b || i < 0 || i >= (int)b->inst.size()) return -1;
     return b->variant[i];
+}
+
+int dgx_batch_tile(const dgx_batch *b, int i, int *elements_per_block, int *waves_bound)
+{
+    if (!b || i < 0 || i >= (int)b->inst.size() || !elements_per_block || !waves_bound) return DGX_ERR_INVALID;
+    *elements_per_block = 0;
+    *waves_bound = 0;
+    const int v = b->variant[i];
+    if (!v) return DGX_OK;
+    const dgx_ctx *c = b->inst[i];
+    const bool fast = c->P.precision == DGX_MIXED;
+    const BatchTile t = batch_tile(c->dim, fast, v - 1);
+    *waves_bound = t.waves;
+    return launch_line_batch(c->dim, c->n, c->P.precision, v - 1, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr,
+                             elements_per_block);
 }
 
 int dgx_batch_allocate(dgx_batch *b, int method, const double *dt)
@@ -1404,7 +1419,7 @@ int dgx_batch_allocate(dgx_batch *b, int method, const double *dt)
         const Ops1D ops = b->inst[g.inst[0]]->ops;
         int rc = batch_autotune(b, g, ops);
         if (rc) return rc;
-        for (int i : g.inst) b->variant[i] = g.es;
+        for (int i : g.inst) b->variant[i] = g.tile + 1;
     }
     // capture one RK step of every instance
     BHIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1412,7 +1427,7 @@ int dgx_batch_allocate(dgx_batch *b, int method, const double *dt)
     for (int st = 0; st < B.s && rc == DGX_OK; ++st) {
         for (BatchGroup &g : b->groups) {
             const Ops1D &ops = b->inst[g.inst[0]]->ops;
-            rc = launch_line_batch(g.dim, g.n, g.precision, g.es, g.blocks, g.nblocks, g.args[st], g.pcs, ops,
+            rc = launch_line_batch(g.dim, g.n, g.precision, g.tile, g.blocks, g.nblocks, g.args[st], g.pcs, ops,
                                    b->stream, nullptr);
             if (rc) break;
         }
@@ -1598,7 +1613,7 @@ static int server_advance(dgx_server *s)
         }
     }
     struct G {
-        int dim, n, precision, es, epb;
+        int dim, n, precision, tile, epb;
         std::vector<int> members;
         int64_t off_blocks, off_pcs, off_args;
         int nblocks;
@@ -1617,13 +1632,13 @@ static int server_advance(dgx_server *s)
             x.n = c->n;
             x.precision = c->P.precision;
             const long key = ((long)x.dim * 16 + x.n) * 4 + x.precision;
-            x.es = 1;
+            x.tile = 0;
             for (auto &kv : g_variant_cache)
-                if (kv.first == key) x.es = kv.second;
-            if (launch_line_batch(x.dim, x.n, x.precision, x.es, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr,
+                if (kv.first == key) x.tile = kv.second;
+            if (launch_line_batch(x.dim, x.n, x.precision, x.tile, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr,
                                   &x.epb) != DGX_OK) {
-                x.es = 1;
-                launch_line_batch(x.dim, x.n, x.precision, 1, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr, &x.epb);
+                x.tile = 0;
+                launch_line_batch(x.dim, x.n, x.precision, 0, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr, &x.epb);
             }
             groups.push_back(x);
             g = &groups.back();
@@ -1671,7 +1686,7 @@ static int server_advance(dgx_server *s)
         for (G &g : groups) {
             const int m = (int)g.members.size();
             const Ops1D &ops = s->active[g.members[0]].c->ops;
-            int rc = launch_line_batch(g.dim, g.n, g.precision, g.es, (const BatchBlock *)(s->stage_dev + g.off_blocks),
+            int rc = launch_line_batch(g.dim, g.n, g.precision, g.tile, (const BatchBlock *)(s->stage_dev + g.off_blocks),
                                        g.nblocks, (const StageArgs *)(s->stage_dev + g.off_args) + st * m,
                                        (const PhysConst *)(s->stage_dev + g.off_pcs), ops, s->stream, nullptr);
             if (rc) {

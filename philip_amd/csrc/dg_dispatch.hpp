@@ -50,8 +50,9 @@ void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
         else if (!split && nsv)
             hipLaunchKernelGGL((stage2_kernel<D, N, false, true, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
         else if (split) {
-            if constexpr (N == 7 && std::is_same<T, float>::value)
-                hipLaunchKernelGGL((line_kernel_w4<D, N, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
+            if constexpr (line_waves<D, N, T>() > 0)
+                hipLaunchKernelGGL((line_kernel_wb<D, N, T, line_waves<D, N, T>()>), lgrid, lblock, 0, c->stream, a,
+                                   c->ops, c->pc);
             else
                 hipLaunchKernelGGL((line_kernel<D, N, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
         } else
@@ -112,18 +113,18 @@ int launch_aux_dn(dgx_ctx *c, const double *u)
     return check_launch(c, "aux_kernel");
 }
 
-template <int D, int N, typename T, bool RT, int ES>
+template <int D, int N, typename T, int ES, int W>
 int line_batch_es(const BatchBlock *blocks, int nblocks, const StageArgs *args, const PhysConst *pcs,
-                         const Ops1D &ops, hipStream_t st, int *epb)
+                  const Ops1D &ops, hipStream_t st, int *epb)
 {
     using LG = LineGeo<D, N, T, ES>;
     if constexpr (LG::SBUF * sizeof(T) > 160 * 1024 - 4096) {
-        return DGX_ERR_UNSUPPORTED;  // variant would not fit the 160 KB of LDS a block can hold
+        return DGX_ERR_UNSUPPORTED;  // would not fit the 160 KB of LDS a block can hold
     } else {
         if (epb) *epb = LG::EPB;
         if (!blocks) return DGX_OK;  // query only
-        if constexpr (D == 2 && N < 7 && std::is_same<T, float>::value)
-            hipLaunchKernelGGL((line_kernel_batch_w5<D, N, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
+        if constexpr (W > 0)
+            hipLaunchKernelGGL((line_kernel_batch_wb<D, N, T, ES, W>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
                                args, pcs, ops);
         else
             hipLaunchKernelGGL((line_kernel_batch<D, N, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
@@ -132,16 +133,37 @@ int line_batch_es(const BatchBlock *blocks, int nblocks, const StageArgs *args, 
     }
 }
 
-template <int D, int N>
-int line_batch_dn(int precision, int es, const BatchBlock *blocks, int nblocks, const StageArgs *args,
-                         const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb)
+template <int D, int N, typename T>
+int line_batch_t(int tile, const BatchBlock *blocks, int nblocks, const StageArgs *args, const PhysConst *pcs,
+                 const Ops1D &ops, hipStream_t st, int *epb)
 {
-    if (precision == DGX_MIXED) {
-        if (es == 2) return line_batch_es<D, N, float, false, 2>(blocks, nblocks, args, pcs, ops, st, epb);
-        return line_batch_es<D, N, float, false, 1>(blocks, nblocks, args, pcs, ops, st, epb);
+    constexpr bool F = std::is_same<T, float>::value;
+    constexpr int W = batch_waves(D, F);
+#define LBT_CASE(V)                                                                                              \
+    case V:                                                                                                      \
+        return line_batch_es<D, N, T, batch_tile(D, F, V).es, batch_tile(D, F, V).waves>(blocks, nblocks, args, pcs, \
+                                                                                         ops, st, epb);
+    if constexpr (W > 0) {
+        switch (tile) {
+            LBT_CASE(0) LBT_CASE(1) LBT_CASE(2) LBT_CASE(3) LBT_CASE(4) LBT_CASE(5)
+        default: return DGX_ERR_UNSUPPORTED;
+        }
+    } else {
+        switch (tile) {
+            LBT_CASE(0) LBT_CASE(1)
+        default: return DGX_ERR_UNSUPPORTED;
+        }
     }
-    if (es == 2) return line_batch_es<D, N, double, true, 2>(blocks, nblocks, args, pcs, ops, st, epb);
-    return line_batch_es<D, N, double, true, 1>(blocks, nblocks, args, pcs, ops, st, epb);
+#undef LBT_CASE
+}
+
+template <int D, int N>
+int line_batch_dn(int precision, int tile, const BatchBlock *blocks, int nblocks, const StageArgs *args,
+                  const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb)
+{
+    if (tile < 0 || tile >= batch_tile_count(D)) return DGX_ERR_UNSUPPORTED;
+    if (precision == DGX_MIXED) return line_batch_t<D, N, float>(tile, blocks, nblocks, args, pcs, ops, st, epb);
+    return line_batch_t<D, N, double>(tile, blocks, nblocks, args, pcs, ops, st, epb);
 }
 
 // explicit instantiation (in dg_inst_n<NN>.hip) and its extern declaration (in dg_launch.hip)

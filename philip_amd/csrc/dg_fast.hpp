@@ -56,8 +56,6 @@ template <typename T, int D> struct NodeC {
     T vel[D];
     T p, b, v2, sr, c, h;  // pressure, beta, |v|^2, sqrt(rho), sound speed, total enthalpy
 };
-template <int D> using NodeF = NodeC<float, D>;
-template <int D> constexpr int node_f_count() { return 2 * D + 8; }
 
 template <typename T> struct FastConst {
     T gam, gamm1, inv_gamm1;

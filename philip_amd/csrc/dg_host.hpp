@@ -115,10 +115,10 @@ int launch_axpy(dgx_ctx *c, const double *x, double h, const double *f, double *
 int launch_lsrk_error(dgx_ctx *c, const double *s1, const double *e, const double *s3, int plus, double d5, double d6,
                       double sum_delta, double atol, double rtol, double *sum_out);
 // Batched split-form line kernel over the element groups `blocks` of several solves of one
-// (dim, n, precision); es = elements-per-block variant (1 or 2).  With blocks == nullptr only
-// reports the variant's elements per block in *epb.  DGX_ERR_UNSUPPORTED if the variant does
-// not exist for this shape.
-int launch_line_batch(int dim, int n, int precision, int es, const BatchBlock *blocks, int nblocks,
+// (dim, n, precision); tile = index of the candidate tile (batch_tile: elements per block x
+// waves/SIMD bound).  With blocks == nullptr only reports the tile's elements per block in *epb.
+// DGX_ERR_UNSUPPORTED if the tile does not exist for this shape.
+int launch_line_batch(int dim, int n, int precision, int tile, const BatchBlock *blocks, int nblocks,
                       const StageArgs *args, const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb);
 int launch_aux(dgx_ctx *c, const double *u);
 int launch_integrated(dgx_ctx *c, const double *u, int overint, double *host_out4);

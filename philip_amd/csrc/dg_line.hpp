@@ -9,21 +9,24 @@
 // two-point flux F#(u_a, u_b), a < b, of its line once, keeps the N nodal
 // sums of the line in registers (node a gets -Sk[a][b] F, node b gets
 // -Sk[b][a] F, the flux being symmetric), adds the two surface numerical
-// fluxes at the line's ends (strong_dg.cpp:2368-3576, lift 1/w_0), and writes
-// one partial result per node and direction to LDS.  The direction is a
-// template parameter of the item's code, so no register array is indexed per
-// lane.
+// fluxes at the face nodes (strong_dg.cpp:2368-3576, lift 1/w_0) as items of
+// their own, and writes one partial result per node to LDS.  The direction is
+// a template parameter of the item's code, so no register array is indexed
+// per lane.
 //
 // Per block: EPB whole elements.
 //   A  node-parallel  load the stage state (FP64, coalesced), per-node cache of
 //                     derived quantities into LDS; in the same phase the
-//                     neighbours' face traces (one item per face node) are
-//                     loaded into LDS, so the line phase makes no global loads
-//   L  line-parallel  pairs + faces of one line, sums kept in registers
-//                     (a long line's pairs shared by LS items); after a
-//                     barrier they go to LDS over the (now dead) node cache
-//                     and traces
-//   E  node-parallel  sum of the direction partials, FP64 epilogue
+//                     neighbours' face-node states (one item per face node)
+//                     are loaded into LDS, so the line phase makes no global
+//                     loads
+//   L  item-parallel  pair-line items (a line's pairs, sums kept in registers;
+//                     a long line's pairs shared by LS items) and face-node
+//                     items (the surface term, written at once over the face
+//                     node's trace slot); after a barrier the line sums go to
+//                     LDS over the (now dead) node cache, one direction at a
+//                     time
+//   E  node-parallel  line partials plus surface terms, FP64 epilogue
 // T = float: mixed path (float fluxes, FP64 state and RK registers); T = double:
 // FP64 path.  Both evaluate the cached-node restatement of the point physics
 // (dg_fast.hpp).  The reference's entropy projection u(v(u)) is the identity
@@ -49,8 +52,8 @@ template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
     static constexpr int EPB = ES > 0 ? G::EPB * ES : (G::EPB / -ES > 0 ? G::EPB / -ES : 1);
     static constexpr int BLOCK = ((EPB * NP + 63) / 64) * 64;
     static constexpr bool FAST = std::is_same<T, float>::value;
-    // node cache fields: u[NS], vel[D], p, b, |v|^2, sqrt(rho), c, h
-    static constexpr int NQ = node_f_count<D>();
+    // node cache fields: rho, E, vel[D], p, b, |v|^2, sqrt(rho), c, h
+    static constexpr int NQ = D + 8;
     // element strides padded to 1 mod 32 words so the stride-N lines of
     // neighbouring elements fall in different LDS banks
     static constexpr int pad1(int x) { return x + (33 - x % 32) % 32; }
@@ -62,12 +65,13 @@ template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
     // kernel's item loop was no longer unrolled, spilling the line sums to scratch)
     static constexpr int LS = (V || D == 2) ? (N >= 7 ? 3 : (N >= 5 ? 2 : 1))
                                             : (N >= 7 ? (FAST ? 1 : 3) : (N >= 6 ? 2 : 1));
-    static constexpr int ESV = pad1(LS * D * NS * NP);  // line partials [direction][part][state][node]
+    static constexpr int ESV = pad1(LS * NS * NP);   // line partials summed over directions [part][state][node]
     static constexpr int NFN = 2 * D * NL;           // face nodes per element
-    static constexpr int ESN = pad1(NQ * NFN);       // neighbour traces' caches [field][face][face node]
+    // neighbour traces' states [state][face][face node]; each slot then takes that face node's
+    // surface term
+    static constexpr int ESN = pad1(NS * NFN);
     static constexpr int ITEMS = EPB * D * NL * LS;
     static constexpr int FITEMS = EPB * NFN;
-    static constexpr int ESF = pad1(2 * D * NS * NL);   // face terms [face][state][face node]
     // Navier-Stokes (V, mixed path only): own viscous flux in every direction
     // [direction][state][node] and the neighbours' viscous flux along the face direction at
     // their face nodes [state][face][face node]
@@ -75,7 +79,9 @@ template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
     static constexpr int ESFV = V ? pad1(D * NS * NP) : 0;
     static constexpr int ESNV = V ? pad1(NS * NFN) : 0;
     static constexpr int EIN = ESQ + ESN + ESFV + ESNV;
-    static constexpr int SBUF = (EIN > ESV + ESF) ? EPB * EIN : EPB * (ESV + ESF);
+    // [traces / surface terms | node cache | V: viscous fluxes]; the line partials take the node
+    // cache's place (and what follows it) once every item has run
+    static constexpr int SBUF = EPB * ESN + ((ESQ + ESFV + ESNV > ESV) ? EPB * (ESQ + ESFV + ESNV) : EPB * ESV);
 };
 
 // Elements-per-block variant of the single-solve line kernel: 1 for Euler; for
@@ -85,7 +91,7 @@ template <int D, int N, typename T, bool V> constexpr int line_es()
 {
     if constexpr (V) {
         using L1 = LineGeo<D, N, T, 1, true>;
-        constexpr int W = (L1::EIN > L1::ESV + L1::ESF ? L1::EIN : L1::ESV + L1::ESF) * (int)sizeof(T);
+        constexpr int W = (L1::SBUF / L1::EPB) * (int)sizeof(T);
         int div = 1;
         while (Geo<D, N>::EPB / div > 1 && (Geo<D, N>::EPB / div) * W > 65536) div *= 2;
         return div == 1 ? 1 : -div;
@@ -96,33 +102,52 @@ template <int D, int N, typename T, bool V> constexpr int line_es()
 template <int D, int N, typename T, bool V> using LineGeoK = LineGeo<D, N, T, line_es<D, N, T, V>(), V>;
 
 // ---- node cache in LDS ----------------------------------------------------
+// A node's cache holds rho, E, the velocity, p, beta, |v|^2, sqrt(rho), c and H (D + 8 values);
+// the momentum is not stored but rebuilt as rho v on load.  Every flux reads nodes in this
+// canonical form (node_canon), whether they come from the cache or from a neighbour trace whose
+// cache is built at the face, so a face evaluates to the same bits in every block layout.
 
 template <typename T, int D>
 __device__ __forceinline__ void nodef_store(T *q, int st, const NodeC<T, D> &n)
 {
-    constexpr int NS = D + 2;
-    for (int s = 0; s < NS; ++s) q[s * st] = n.u[s];
-    for (int k = 0; k < D; ++k) q[(NS + k) * st] = n.vel[k];
-    q[(NS + D) * st] = n.p;
-    q[(NS + D + 1) * st] = n.b;
-    q[(NS + D + 2) * st] = n.v2;
-    q[(NS + D + 3) * st] = n.sr;
-    q[(NS + D + 4) * st] = n.c;
-    q[(NS + D + 5) * st] = n.h;
+    q[0] = n.u[0];
+    q[st] = n.u[D + 1];
+    for (int k = 0; k < D; ++k) q[(2 + k) * st] = n.vel[k];
+    q[(D + 2) * st] = n.p;
+    q[(D + 3) * st] = n.b;
+    q[(D + 4) * st] = n.v2;
+    q[(D + 5) * st] = n.sr;
+    q[(D + 6) * st] = n.c;
+    q[(D + 7) * st] = n.h;
+}
+
+template <typename T, int D> __device__ __forceinline__ void node_canon(NodeC<T, D> &n)
+{
+    for (int k = 0; k < D; ++k) n.u[1 + k] = n.u[0] * n.vel[k];
 }
 
 template <typename T, int D>
 __device__ __forceinline__ void nodef_load(const T *q, int st, NodeC<T, D> &n)
 {
-    constexpr int NS = D + 2;
-    for (int s = 0; s < NS; ++s) n.u[s] = q[s * st];
-    for (int k = 0; k < D; ++k) n.vel[k] = q[(NS + k) * st];
-    n.p = q[(NS + D) * st];
-    n.b = q[(NS + D + 1) * st];
-    n.v2 = q[(NS + D + 2) * st];
-    n.sr = q[(NS + D + 3) * st];
-    n.c = q[(NS + D + 4) * st];
-    n.h = q[(NS + D + 5) * st];
+    n.u[0] = q[0];
+    n.u[D + 1] = q[st];
+    for (int k = 0; k < D; ++k) n.vel[k] = q[(2 + k) * st];
+    n.p = q[(D + 2) * st];
+    n.b = q[(D + 3) * st];
+    n.v2 = q[(D + 4) * st];
+    n.sr = q[(D + 5) * st];
+    n.c = q[(D + 6) * st];
+    n.h = q[(D + 7) * st];
+    node_canon<T, D>(n);
+}
+
+// A neighbour's face node from its conservative state (field-major, stride st).
+template <typename T, int D>
+__device__ __forceinline__ void trace_node(const T *q, int st, const FastConst<T> &fc, NodeC<T, D> &n)
+{
+    for (int s = 0; s < D + 2; ++s) n.u[s] = q[s * st];
+    node_fill<T, D>(n, fc);
+    node_canon<T, D>(n);
 }
 
 template <typename T, int D>
@@ -236,7 +261,7 @@ __device__ __forceinline__ void face_visc(const PhysConst &pc, T pen, const T *u
 template <typename T> struct FaceIn {
     const T *sq;    // node caches of the block
     const T *sfv;   // viscous flux of the block's nodes, every direction (Navier-Stokes)
-    const T *tr;    // this face node's neighbour trace (node cache fields, stride tst)
+    const T *tr;    // this face node's neighbour trace (conservative state, stride tst)
     const T *trv;   // the neighbour's viscous flux along the face direction there (Navier-Stokes)
     int tst;
 };
@@ -245,10 +270,11 @@ template <typename T> struct FaceIn {
 // numerical flux with the reference orientation (the lower cell index is the
 // interior side), times -1/w_0 for the interior and +1/w_0 for the exterior
 // side, so the neighbour's term is the negative.  nbl >= 0: the neighbour is
-// element nbl of this block (node cache in LDS); otherwise its trace is tr
-// (field-major, stride tst).  Both sides' caches were filled by node_fill in
-// phase A, so a face evaluated in two blocks (once from each side) sees the
-// same bits on both and the flux is the same whichever block holds the pair.
+// element nbl of this block (node cache in LDS); otherwise its trace state is
+// tr (field-major, stride tst) and its cache is built here.  Both ways give
+// the same canonical node, so a face evaluated in two blocks (once from each
+// side) sees the same bits on both and the flux is the same whichever block
+// holds the pair.
 template <typename T, int D, int N, int K, bool V>
 __device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, const FastConst<T> &fc,
                                        const FaceIn<T> &in, int le, int side, int fn, int e, int nb, int nbl,
@@ -261,7 +287,7 @@ __device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, co
     NodeC<T, D> qi, qn, L, R;
     nodef_load<T, D>(in.sq + le * LG::ESQ + fi, NP, qi);
     if (nbl >= 0) nodef_load<T, D>(in.sq + nbl * LG::ESQ + fj, NP, qn);
-    else nodef_load<T, D>(in.tr, in.tst, qn);
+    else trace_node<T, D>(in.tr, in.tst, fc, qn);
     const bool own = e < nb;
     nodef_select<T, D>(own, qi, qn, L);
     nodef_select<T, D>(own, qn, qi, R);
@@ -305,15 +331,15 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     constexpr int NFI = (LG::FITEMS + BLOCK - 1) / BLOCK;  // face-trace loads per thread
     constexpr int NFS = EPB * 2 * D;                       // face sides of the block
     constexpr int NWAVE = BLOCK / 64;
-    // The node cache and the neighbour traces are dead once every item has its
-    // result in registers, so the line and face partials reuse their storage.
+    // A face item writes its surface term over its own trace slot as soon as it has it; the
+    // node cache (and the viscous fluxes) are dead once every item has run, and the line
+    // partials take their storage.
     __shared__ T sBuf[LG::SBUF];
-    T *const sQ = sBuf;                       // per-node cache [element][field][node]
-    T *const sN = sQ + EPB * LG::ESQ;         // neighbour face traces' caches [element][field][face][face node]
-    T *const sFv = sN + EPB * LG::ESN;        // V: viscous flux [element][direction][state][node]
+    T *const sN = sBuf;                       // traces, then surface terms [element][state][face][face node]
+    T *const sQ = sN + EPB * LG::ESN;         // per-node cache [element][field][node]
+    T *const sFv = sQ + EPB * LG::ESQ;        // V: viscous flux [element][direction][state][node]
     T *const sNv = sFv + EPB * LG::ESFV;      // V: neighbours' viscous flux, as the traces
-    T *const sV = sBuf;                       // line partials [element][direction][state][node]
-    T *const sF = sBuf + EPB * LG::ESV;       // face terms [element][face][state][face node]
+    T *const sV = sQ;                         // line partials [element][part][state][node]
     __shared__ int sNb[EPB][2 * D];           // face neighbours
     __shared__ short sList[NWAVE][NFS];       // per-wave list of the faces evaluated in this block
 
@@ -378,10 +404,8 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
 #pragma unroll
     for (int j = 0; j < NFI; ++j) {
         if (fdst[j] < 0) continue;
-        NodeC<T, D> n;
-        for (int s = 0; s < NS; ++s) n.u[s] = T(un[j][s]);
-        node_fill<T, D>(n, fc);
-        nodef_store<T, D>(sN + (fdst[j] / LG::NFN) * LG::ESN + fdst[j] % LG::NFN, LG::NFN, n);
+        T *q = sN + (fdst[j] / LG::NFN) * LG::ESN + fdst[j] % LG::NFN;
+        for (int s = 0; s < NS; ++s) q[s * LG::NFN] = T(un[j][s]);
     }
     if constexpr (V) {
         // viscous fluxes from the lifted gradient: the own nodes' in every direction, the
@@ -390,7 +414,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             T g[NS * D], ut[NS], fv[D][NS];
 #pragma unroll
             for (int x = 0; x < NS * D; ++x) g[x] = aux_at<T>(a, x % D, ((long)e * NS + x / D) * NP + i);
-            for (int s = 0; s < NS; ++s) ut[s] = sQ[le * LG::ESQ + s * NP + i];
+            for (int s = 0; s < NS; ++s) ut[s] = T(ud[s]);
             viscous_flux_all<T, D>(ut, g, pc, fv);
 #pragma unroll
             for (int k = 0; k < D; ++k)
@@ -452,8 +476,6 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     const int items = PITEMS + nface * NL;
     const T pen = T(a.penalty);
     T rp[JP][N][NS];
-    T rf[JT][NS];
-    T rfn[NBT ? JT : 1][NS];
 #pragma unroll
     for (int j = 0; j < JT; ++j) {
         const int t = tid + j * BLOCK;
@@ -485,46 +507,56 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             const int nb = sNb[fle][2 * k + side];
             const int nbl = in_block(nb, blk0 + fle) ? nb - blk0 : -1;
             const int fo = (2 * k + side) * NL + fn;
-            const FaceIn<T> in{sQ, sFv, sN + fle * LG::ESN + fo, sNv + fle * LG::ESNV + fo, LG::NFN};
+            T *const slot = sN + fle * LG::ESN + fo;
+            const FaceIn<T> in{sQ, sFv, slot, sNv + fle * LG::ESNV + fo, LG::NFN};
+            T rf[NS], rfn[NS];
             auto fitem = [&](auto kc) {
                 constexpr int K = decltype(kc)::value;
-                face_f<T, D, N, K, V>(ops, pc, fc, in, fle, side, fn, blk0 + fle, nb, nbl, pen, rf[j],
-                                      rfn[NBT ? j : 0]);
+                face_f<T, D, N, K, V>(ops, pc, fc, in, fle, side, fn, blk0 + fle, nb, nbl, pen, rf, rfn);
             };
             if (k == 0) fitem(std::integral_constant<int, 0>{});
             else if (k == 1) fitem(std::integral_constant<int, 1>{});
             else if (D == 3) fitem(std::integral_constant<int, (D == 3 ? 2 : 1)>{});
+            // the trace slot was read only by this item; a neighbour in the block has no trace on
+            // the shared face (and no item of its own for it), so its slot takes its term
+            for (int s = 0; s < NS; ++s) slot[s * LG::NFN] = rf[s];
+            if (nbl >= 0) {
+                T *const w2 = sN + nbl * LG::ESN + ((2 * k + side) ^ 1) * NL + fn;
+                for (int s = 0; s < NS; ++s) w2[s * LG::NFN] = NBT ? rfn[s] : -rf[s];
+            }
         }
     }
     __syncthreads();  // node cache and traces consumed; their storage now takes the partials
+    // Line partials of one direction at a time into one [part][state][node] buffer: direction 0
+    // stores, each later direction adds after a barrier, so every part's sum is ((d0 + d1) + d2)
+    // in this order in every block layout.
+    auto line_out = [&](auto kc, bool add) {
+        constexpr int KD = decltype(kc)::value;
 #pragma unroll
-    for (int j = 0; j < JT; ++j) {
-        const int t = tid + j * BLOCK;
-        if (j < JP && t < PITEMS) {
+        for (int j = 0; j < JP; ++j) {
+            const int t = tid + j * BLOCK;
+            if (t >= PITEMS) continue;
             const int kh = t / (EPB * NL), r = t % (EPB * NL);
-            const int k = kh / LS;
+            if (kh / LS != KD) continue;
             const int lle = r / NL, line = r % NL;
             if (blk0 + lle >= a.ncells) continue;
-            T *v = sV + lle * LG::ESV + kh * NS * NP;
+            T *v = sV + lle * LG::ESV + (kh % LS) * NS * NP;
 #pragma unroll
             for (int p = 0; p < N; ++p) {
-                const int n = (k == 0) ? line_node<D, N>(0, line, p)
-                                       : (k == 1) ? line_node<D, N>(1, line, p) : line_node<D, N>(D == 3 ? 2 : 1, line, p);
-                for (int s = 0; s < NS; ++s) v[s * NP + n] = rp[j < JP ? j : 0][p][s];
-            }
-        } else if (t >= PITEMS && t < items) {
-            const int u = (t - PITEMS) / NL, fn = (t - PITEMS) % NL;
-            const int id = sList[wv][u];
-            const int k = id / (2 * EPB), r = id % (2 * EPB);
-            const int fle = r >> 1, f = 2 * k + (r & 1);
-            const int nb = sNb[fle][f];
-            T *w = sF + fle * LG::ESF + f * NS * NL + fn;
-            for (int s = 0; s < NS; ++s) w[s * NL] = rf[j][s];
-            if (in_block(nb, blk0 + fle)) {  // the shared face's term for the neighbour
-                T *w2 = sF + (nb - blk0) * LG::ESF + (f ^ 1) * NS * NL + fn;
-                for (int s = 0; s < NS; ++s) w2[s * NL] = NBT ? rfn[NBT ? j : 0][s] : -rf[j][s];
+                const int n = line_node<D, N>(KD, line, p);
+                for (int s = 0; s < NS; ++s) {
+                    if (add) v[s * NP + n] += rp[j][p][s];
+                    else v[s * NP + n] = rp[j][p][s];
+                }
             }
         }
+    };
+    line_out(std::integral_constant<int, 0>{}, false);
+    __syncthreads();
+    line_out(std::integral_constant<int, 1>{}, true);
+    if constexpr (D == 3) {
+        __syncthreads();
+        line_out(std::integral_constant<int, (D == 3 ? 2 : 1)>{}, true);
     }
     __syncthreads();
     if (!active) return;
@@ -536,15 +568,15 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     const T *v = sV + le * LG::ESV + i;
     for (int s = 0; s < NS; ++s) {
         T xs = v[s * NP];
-        for (int k = 1; k < D * LS; ++k) xs += v[(k * NS + s) * NP];
+        for (int h = 1; h < LS; ++h) xs += v[(h * NS + s) * NP];
         acc[s] = xs;
     }
-    const T *fb = sF + le * LG::ESF;
+    const T *fb = sN + le * LG::ESN;
     for (int k = 0; k < D; ++k) {
         if (c[k] != 0 && c[k] != N - 1) continue;
         const int f = 2 * k + (c[k] == N - 1 ? 1 : 0);
-        const T *w = fb + f * NS * NL + node_line<D, N>(k, i);
-        for (int s = 0; s < NS; ++s) acc[s] += w[s * NL];
+        const T *w = fb + f * NL + node_line<D, N>(k, i);
+        for (int s = 0; s < NS; ++s) acc[s] += w[s * LG::NFN];
     }
     stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc, pf, pf2, pf3);
 }
@@ -556,11 +588,21 @@ line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
     line_block<D, N, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
 }
-// The same held to 4 waves/SIMD: used for the mixed path at 3D p=6, where it measured faster
-// despite spilling (12^3 TGV: 129 -> 120 us per stage; p=3..5 unchanged).
-template <int D, int N, typename T>
-__global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK)) __attribute__((amdgpu_waves_per_eu(4)))
-line_kernel_w4(StageArgs a, Ops1D ops, PhysConst pc)
+// Waves per SIMD the single-solve 3D line kernel is held to (amdgpu_waves_per_eu; 0: the
+// compiler's choice).  Mixed path at p=6: 4, faster despite spilling (12^3 TGV: 129 -> 120 us
+// per stage; p=3..5 unchanged).  FP64 path at p=3..5: 3 (168 VGPRs, ~112 B/lane spilled at p=3,
+// two -> three blocks per CU): TGV p=3 32^3 258 -> 222 us per stage, p=4 24^3 327 -> 305 us,
+// p=5 20^3 398 -> 370 us (4 waves at p=3: 395 us); at p=2 no change and at p=6 much slower
+// (448 -> 724 us), so those keep the compiler's choice.
+template <int D, int N, typename T> constexpr int line_waves()
+{
+    if constexpr (D == 3 && N == 7 && std::is_same<T, float>::value) return 4;
+    if constexpr (D == 3 && N >= 4 && N <= 6 && std::is_same<T, double>::value) return 3;
+    return 0;
+}
+template <int D, int N, typename T, int W>
+__global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK)) __attribute__((amdgpu_waves_per_eu(W)))
+line_kernel_wb(StageArgs a, Ops1D ops, PhysConst pc)
 {
     line_block<D, N, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
 }
@@ -597,16 +639,32 @@ line_kernel_batch(const BatchBlock *__restrict__ blocks, const StageArgs *__rest
     const BatchBlock bb = blocks[xcd_block()];
     line_block<D, N, T, ES>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
 }
-// The same held to 5 waves/SIMD (96 VGPRs, a few values spilled): used for the 2D mixed-path
-// groups up to p=5, where it measured faster (p=3: 64 -> 55 us, p=4: 141 -> 125 us per stage in
-// the 256-instance vortex mix); at p=6 it was slower (254 -> 263 us).
-template <int D, int N, typename T, int ES>
-__global__ void __launch_bounds__((LineGeo<D, N, T, ES>::BLOCK)) __attribute__((amdgpu_waves_per_eu(5)))
-line_kernel_batch_w5(const BatchBlock *__restrict__ blocks, const StageArgs *__restrict__ args,
+// The same held to W waves/SIMD (amdgpu_waves_per_eu): a candidate tile of the batched
+// autotuner.
+template <int D, int N, typename T, int ES, int W>
+__global__ void __launch_bounds__((LineGeo<D, N, T, ES>::BLOCK)) __attribute__((amdgpu_waves_per_eu(W)))
+line_kernel_batch_wb(const BatchBlock *__restrict__ blocks, const StageArgs *__restrict__ args,
                      const PhysConst *__restrict__ pcs, Ops1D ops)
 {
     const BatchBlock bb = blocks[xcd_block()];
     line_block<D, N, T, ES>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
+}
+
+// Tiles the batched autotuner times per (dim, nodes per direction, precision): elements per
+// block (ES: 1 = the 256-node default, 2 = twice as many, -2 = half as many) x a waves/SIMD bound
+// (0 = the compiler's register allocation; else amdgpu_waves_per_eu(W), trading spills for
+// occupancy).  Neither changes the arithmetic of an element (the line split LS is fixed per
+// shape for that reason: it reorders the pair sums), so every tile gives the single solve's bits.
+struct BatchTile {
+    int es, waves;
+};
+constexpr int BATCH_TILES_2D = 6, BATCH_TILES_3D = 2;
+constexpr int batch_tile_count(int dim) { return dim == 2 ? BATCH_TILES_2D : BATCH_TILES_3D; }
+constexpr int batch_waves(int dim, bool fast) { return dim == 2 ? (fast ? 5 : 4) : 0; }
+constexpr BatchTile batch_tile(int dim, bool fast, int v)
+{
+    const int es = (v % 3 == 0) ? 1 : (v % 3 == 1 ? 2 : -2);
+    return BatchTile{es, v >= 3 ? batch_waves(dim, fast) : 0};
 }
 
 } // namespace dgx
