@@ -49,6 +49,11 @@ def measured_traffic(precision):
     return None
 
 
+def progress(msg):
+    """One line per finished part of the run, on stderr (the JSON line alone goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def workload_params(precision):
     import philip_amd as P
     return dict(dim=3, poly_degree=3, n_cells_1d=32, use_split_form=1, two_point_flux=P.CH,
@@ -237,7 +242,10 @@ def serving(n_req=400, rate=8.0, seed=77, precision="mixed", time_scale=1.0):
             ids.append(s.submit(reqs[k]))
             delay.append(now - arrivals[k])  # host polling delay, added to the latency
             k += 1
-        a, q, done = s.step()
+        a, q, done_now = s.step()
+        if done_now // 100 > done // 100:
+            progress(f"serving {precision}: {done_now} of {n_req} requests done")
+        done = done_now
         steps += 1
         max_active, max_queued = max(max_active, a), max(max_queued, q)
         if a == 0 and q == 0 and k < n_req:
@@ -365,6 +373,8 @@ def main():
                     "(default: every CPU this process may run on)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
+    ap.add_argument("--navier-stokes-only", choices=["fp64", "mixed"], default=None,
+                    help="only run the Navier-Stokes TGV stage rate at this precision (for profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances line")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream line")
     ap.add_argument("--serving-rate", type=float, default=8.0,
@@ -373,6 +383,9 @@ def main():
     ap.add_argument("--no-ns-history", action="store_true",
                     help="skip the long runs at both precisions (Euler TGV 1000 steps, Navier-Stokes TGV to t=2)")
     args = ap.parse_args()
+    if args.navier_stokes_only:
+        print(json.dumps(navier_stokes_tgv(args.navier_stokes_only)), flush=True)
+        return
     if args.copy_calibration:
         copy_calibration()
         return
@@ -405,17 +418,23 @@ def main():
                    "host_buffer_rate": r["pcie"]},
         "roofline": roofline(args.precision, r, bytes_per_launch),
     }
+    progress(f"{args.precision} stage: {r['avg_ms'] * 1e3:.1f} us per launch")
     if not args.no_second:
         r2 = run_gpu(other, args.steps, args.warmup)
+        progress(f"{other} stage: {r2['avg_ms'] * 1e3:.1f} us per launch")
         line[other + "_path"] = {"value": r2["value"], "ms_per_step": r2["ms_per_step"],
                                  "dtype": "f64" if other == "fp64" else "f32-flux/f64-state",
                                  "roofline": roofline(other, r2, bytes_per_launch)}
     if not args.no_batch:
         line["batched"] = batched_vortex()
+        progress(f"batched: {line['batched']['value']:.3e} DOF-updates/s")
         line["navier_stokes"] = [navier_stokes_tgv("fp64"), navier_stokes_tgv("mixed")]
+        progress("Navier-Stokes stage rates done")
     if not args.no_ns_history:
         line["tgv_1000_steps"] = tgv_history()
+        progress("TGV 1000-step histories done")
         line["navier_stokes_t2"] = navier_stokes_history()
+        progress("Navier-Stokes TGV to t=2 done")
         if "mixed_path" in line:
             h = line["tgv_1000_steps"]
             line["mixed_path"]["error_vs_fp64"] = {
@@ -423,8 +442,10 @@ def main():
                 "entropy_drift_mixed": h["mixed"]["entropy_drift"],
                 "max_entropy_diff_rel_S0": h["max_entropy_diff_rel_S0"]}
     if not args.no_serving:
-        line["serving"] = [serving(precision="mixed", rate=args.serving_rate),
-                           serving(precision="fp64", rate=args.serving_rate / 2)]
+        line["serving"] = []
+        for pr, rate in (("mixed", args.serving_rate), ("fp64", args.serving_rate / 2)):
+            line["serving"].append(serving(precision=pr, rate=rate))
+            progress(f"serving {pr}: p50 {line['serving'][-1]['p50_s']:.3f} s, p99 {line['serving'][-1]['p99_s']:.3f} s")
         if args.serving_sweep:
             # arrival rate swept at both precisions (same request stream per rate), shorter final times
             line["serving_sweep"] = [serving(n_req=100, rate=r, precision=pr, time_scale=0.1)
@@ -432,6 +453,7 @@ def main():
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or host_cores()[0]
         line["cpu_baseline"] = cpu_baseline(threads)
+        progress(f"CPU baseline: {line['cpu_baseline']['value']:.3e} DOF-updates/s on {threads} threads")
     print(json.dumps(line), flush=True)
 
 

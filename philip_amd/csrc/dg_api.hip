@@ -1182,6 +1182,9 @@ struct dgx_batch {
     std::vector<int> variant;            // per instance
     int method = -1;
     hipStream_t stream = nullptr;
+    std::vector<hipStream_t> side;       // one per group after the first: the groups' launches of a
+                                         // stage are independent graph branches
+    std::vector<hipEvent_t> events;      // fork/join events of the captured step
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string err;
@@ -1208,6 +1211,8 @@ static void batch_free_groups(dgx_batch *b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+    for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+    b->events.clear();
 }
 
 static bool batch_line_eligible(const dgx_ctx *c)
@@ -1236,24 +1241,15 @@ static int batch_blocks(dgx_batch *b, BatchGroup &g, int tile)
 }
 
 // Times every candidate tile (elements per block x waves/SIMD bound, batch_tile) on the group's
-// stage-0 arguments in MODE_RHS (writing into each instance's second stage buffer, which the
-// step overwrites anyway; median of five launches after a warm-up) and keeps the fastest.
+// first RK stage (median of five launches after a warm-up) and keeps the fastest.
 static int batch_autotune(dgx_batch *b, BatchGroup &g, const Ops1D &ops)
 {
     const long key = ((long)g.dim * 16 + g.n) * 4 + g.precision;
     for (auto &kv : g_variant_cache)
         if (kv.first == key) return batch_blocks(b, g, kv.second);
-    std::vector<StageArgs> probe(g.inst.size());
-    for (size_t slot = 0; slot < g.inst.size(); ++slot) {
-        dgx_ctx *c = b->inst[g.inst[slot]];
-        StageArgs a = base_args(c, c->u_dev);
-        a.mode = MODE_RHS;
-        a.out = c->regs[2];
-        probe[slot] = a;
-    }
-    StageArgs *pa = nullptr;
-    BHIPCHK(hipMalloc(&pa, sizeof(StageArgs) * probe.size()));
-    BHIPCHK(hipMemcpy(pa, probe.data(), sizeof(StageArgs) * probe.size(), hipMemcpyHostToDevice));
+    // the group's own first-stage launch: it reads u_n and writes only the accumulator and the
+    // next stage state, which the first step recomputes
+    const StageArgs *pa = g.args[0];
     hipEvent_t e0, e1;
     BHIPCHK(hipEventCreate(&e0));
     BHIPCHK(hipEventCreate(&e1));
@@ -1279,7 +1275,6 @@ static int batch_autotune(dgx_batch *b, BatchGroup &g, const Ops1D &ops)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(pa);
     g_variant_cache.push_back({key, best});
     return batch_blocks(b, g, best);
 }
@@ -1320,6 +1315,7 @@ int dgx_batch_destroy(dgx_batch *b)
         c->batch_locked = false;
         dgx_destroy(c);
     }
+    for (hipStream_t st : b->side) (void)hipStreamDestroy(st);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return DGX_OK;
@@ -1421,16 +1417,38 @@ int dgx_batch_allocate(dgx_batch *b, int method, const double *dt)
         if (rc) return rc;
         for (int i : g.inst) b->variant[i] = g.tile + 1;
     }
-    // capture one RK step of every instance
+    // capture one RK step of every instance; within a stage the groups' launches are branches of
+    // the graph (forked from and joined into the batch stream), so the GPU can run blocks of
+    // several groups at once
+    const int ng = (int)b->groups.size();
+    while ((int)b->side.size() < ng - 1) {
+        hipStream_t st = nullptr;
+        BHIPCHK(hipStreamCreate(&st));
+        b->side.push_back(st);
+    }
+    for (int k = 0; k < B.s * ng; ++k) {
+        hipEvent_t e = nullptr;
+        BHIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->events.push_back(e);
+    }
     BHIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int rc = DGX_OK;
     for (int st = 0; st < B.s && rc == DGX_OK; ++st) {
-        for (BatchGroup &g : b->groups) {
+        hipEvent_t *ev = b->events.data() + st * ng;  // [0]: fork, [k]: join of group k
+        if (ng > 1 && hipEventRecord(ev[0], b->stream) != hipSuccess) rc = DGX_ERR_HIP;
+        for (int k = 0; k < ng && rc == DGX_OK; ++k) {
+            BatchGroup &g = b->groups[k];
+            hipStream_t gs = k ? b->side[k - 1] : b->stream;
+            if (k && hipStreamWaitEvent(gs, ev[0], 0) != hipSuccess) rc = DGX_ERR_HIP;
             const Ops1D &ops = b->inst[g.inst[0]]->ops;
-            rc = launch_line_batch(g.dim, g.n, g.precision, g.tile, g.blocks, g.nblocks, g.args[st], g.pcs, ops,
-                                   b->stream, nullptr);
-            if (rc) break;
+            if (!rc)
+                rc = launch_line_batch(g.dim, g.n, g.precision, g.tile, g.blocks, g.nblocks, g.args[st], g.pcs, ops,
+                                       gs, nullptr);
+            if (!rc && k && hipEventRecord(ev[k], gs) != hipSuccess) rc = DGX_ERR_HIP;
         }
+        for (int k = 1; k < ng && rc == DGX_OK; ++k)
+            if (hipStreamWaitEvent(b->stream, ev[k], 0) != hipSuccess) rc = DGX_ERR_HIP;
+        if (rc && b->err.empty()) b->err = "capture of the batched step failed";
         for (int i = 0; i < n && rc == DGX_OK; ++i) {
             dgx_ctx *c = b->inst[i];
             if (b->variant[i]) continue;

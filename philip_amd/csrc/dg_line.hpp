@@ -41,9 +41,13 @@ namespace dgx {
 
 // Issue the loads of the low-storage RK registers S2, S3 at the start of the
 // kernel (held in registers through the line phase) instead of in the epilogue:
-// on for the mixed path (its register budget keeps 4 waves/SIMD), off for FP64.
+// on for the mixed path (its register budget keeps 4 waves/SIMD), off for FP64,
+// which issues them once the line phase is over instead, so that they are in
+// flight while the line sums go to LDS (TGV p=3 32^3: 220 -> 208 us per stage;
+// p=2 -3 %, p=6 -6 %, p=4 unchanged; at 3D p=5 slower, 366 -> 399 us, so not there).
 constexpr bool LINE_PREFETCH_RK_F = true;
 constexpr bool LINE_PREFETCH_RK_D = false;
+template <int D, int N> constexpr bool line_late_prefetch_d() { return !(D == 3 && N == 6); }
 // ES: elements per block relative to the 256-node default (ES > 0: ES times,
 // ES < 0: 1/|ES|); per-element strides do not depend on it.
 template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
@@ -57,7 +61,13 @@ template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
     // element strides padded to 1 mod 32 words so the stride-N lines of
     // neighbouring elements fall in different LDS banks
     static constexpr int pad1(int x) { return x + (33 - x % 32) % 32; }
-    static constexpr int ESQ = pad1(NQ * NP);
+    // 3D, N = 4: node cache in the skewed order of qpos (the 16 x-lines of an element would
+    // otherwise read 4 words apart, 4 lanes per bank pair); its float element stride is 16 mod 32
+    // words so the two elements a 32-lane access spans take complementary bank halves.  At p=3
+    // the stage kernel ran 3 % (mixed) and 1.5 % (FP64) faster; at N = 6 the skew was slower.
+    static constexpr bool QSKEW = (D == 3 && N == 4);
+    static constexpr int pad16(int x) { return x + (48 - x % 32) % 32; }
+    static constexpr int ESQ = (QSKEW && FAST) ? pad16(NQ * NP) : pad1(NQ * NP);
     // Long lines are shared by LS items (the pairs and the viscous columns split evenly), so
     // that the line phase keeps most of the block busy: from p = 4 in 2D and for Navier-Stokes;
     // for 3D Euler at p = 5, and at p = 6 on the FP64 path only (at p = 4 the extra partials
@@ -102,6 +112,21 @@ template <int D, int N, typename T, bool V> constexpr int line_es()
 template <int D, int N, typename T, bool V> using LineGeoK = LineGeo<D, N, T, line_es<D, N, T, V>(), V>;
 
 // ---- node cache in LDS ----------------------------------------------------
+// Position of node i (x fastest) in its element's cache.  3D, N = 4: the skewed order
+// ((x + y) mod N) + N ((y + z) mod N) + N^2 z, in which every grid line in every direction meets
+// each bank group once: with one coordinate fixed the other two map one-to-one onto
+// (x + y, y + z), so the N^2 lines of a direction read N^2 different positions mod N^2
+// (16 lanes of an element: the 16 distinct 8-byte bank pairs).  Otherwise the identity.
+template <int D, int N> __device__ __forceinline__ int qpos(int i)
+{
+    if constexpr (D == 3 && N == 4) {
+        const int x = i % N, y = (i / N) % N, z = i / (N * N);
+        return (x + y) % N + N * ((y + z) % N) + N * N * z;
+    } else {
+        return i;
+    }
+}
+
 // A node's cache holds rho, E, the velocity, p, beta, |v|^2, sqrt(rho), c and H (D + 8 values);
 // the momentum is not stored but rebuilt as rho v on load.  Every flux reads nodes in this
 // canonical form (node_canon), whether they come from the cache or from a neighbour trace whose
@@ -203,8 +228,8 @@ __device__ __forceinline__ void pairs_f(const Ops1D &ops, const PhysConst &pc, c
         for (int p1 = p0 + 1; p1 < N; ++p1) {
             if (!pair_in_part<N, LS, H>(p0, p1)) continue;
             NodeC<T, D> A, B;
-            nodef_load<T, D>(sq + line_node<D, N>(K, line, p0), NP, A);
-            nodef_load<T, D>(sq + line_node<D, N>(K, line, p1), NP, B);
+            nodef_load<T, D>(sq + qpos<D, N>(line_node<D, N>(K, line, p0)), NP, A);
+            nodef_load<T, D>(sq + qpos<D, N>(line_node<D, N>(K, line, p1)), NP, B);
             T F[NS];
             pair_flux_fast<T, D, K>(A, B, fc, pc, F);
             const T s01 = op_sk<T>(ops, p0, p1), s10 = op_sk<T>(ops, p1, p0);
@@ -285,8 +310,8 @@ __device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, co
     const int fi = line_node<D, N>(K, fn, side ? N - 1 : 0);
     const int fj = line_node<D, N>(K, fn, side ? 0 : N - 1);
     NodeC<T, D> qi, qn, L, R;
-    nodef_load<T, D>(in.sq + le * LG::ESQ + fi, NP, qi);
-    if (nbl >= 0) nodef_load<T, D>(in.sq + nbl * LG::ESQ + fj, NP, qn);
+    nodef_load<T, D>(in.sq + le * LG::ESQ + qpos<D, N>(fi), NP, qi);
+    if (nbl >= 0) nodef_load<T, D>(in.sq + nbl * LG::ESQ + qpos<D, N>(fj), NP, qn);
     else trace_node<T, D>(in.tr, in.tst, fc, qn);
     const bool own = e < nb;
     nodef_select<T, D>(own, qi, qn, L);
@@ -399,7 +424,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
         for (int s = 0; s < NS; ++s) n.u[s] = T(ud[s]);
         node_fill<T, D>(n, fc);
         flag_nonphysical<T>(a, n.u[0], n.p);
-        nodef_store<T, D>(sQ + le * LG::ESQ + i, NP, n);
+        nodef_store<T, D>(sQ + le * LG::ESQ + qpos<D, N>(i), NP, n);
     }
 #pragma unroll
     for (int j = 0; j < NFI; ++j) {
@@ -527,6 +552,14 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
         }
     }
     __syncthreads();  // node cache and traces consumed; their storage now takes the partials
+    constexpr bool LATE_PF = !PREFETCH && !FAST && line_late_prefetch_d<D, N>();
+    const bool late_pf = LATE_PF && active && a.mode == MODE_LSRK && !a.lsrk_first;
+    if (late_pf)
+        for (int s = 0; s < NS; ++s) {
+            const long idx = ((long)e * NS + s) * NP + i;
+            pf2[s] = ld_stream(a.s2 + idx);
+            pf3[s] = ld_stream(a.s3 + idx);
+        }
     // Line partials of one direction at a time into one [part][state][node] buffer: direction 0
     // stores, each later direction adds after a barrier, so every part's sum is ((d0 + d1) + d2)
     // in this order in every block layout.
@@ -578,7 +611,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
         const T *w = fb + f * NL + node_line<D, N>(k, i);
         for (int s = 0; s < NS; ++s) acc[s] += w[s * LG::NFN];
     }
-    stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc, pf, pf2, pf3);
+    stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc, pf || late_pf, pf2, pf3);
 }
 
 // One solve: blocks in XCD-aware order over the element groups.
@@ -589,14 +622,14 @@ line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
     line_block<D, N, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
 }
 // Waves per SIMD the single-solve 3D line kernel is held to (amdgpu_waves_per_eu; 0: the
-// compiler's choice).  Mixed path at p=6: 4, faster despite spilling (12^3 TGV: 129 -> 120 us
-// per stage; p=3..5 unchanged).  FP64 path at p=3..5: 3 (168 VGPRs, ~112 B/lane spilled at p=3,
+// compiler's choice).  The mixed path keeps the compiler's choice (a 4-wave bound at p=6 was
+// faster with an earlier, larger LDS layout; it now measures 296 against 278 us per stage at 16^3).
+// FP64 path at p=3..5: 3 (168 VGPRs, ~64 B/lane spilled at p=3,
 // two -> three blocks per CU): TGV p=3 32^3 258 -> 222 us per stage, p=4 24^3 327 -> 305 us,
 // p=5 20^3 398 -> 370 us (4 waves at p=3: 395 us); at p=2 no change and at p=6 much slower
 // (448 -> 724 us), so those keep the compiler's choice.
 template <int D, int N, typename T> constexpr int line_waves()
 {
-    if constexpr (D == 3 && N == 7 && std::is_same<T, float>::value) return 4;
     if constexpr (D == 3 && N >= 4 && N <= 6 && std::is_same<T, double>::value) return 3;
     return 0;
 }
