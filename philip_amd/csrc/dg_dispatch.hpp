@@ -24,7 +24,7 @@ inline int check_launch(dgx_ctx *c, const char *what)
     return DGX_OK;
 }
 
-template <int D, int N, typename T, bool RT>
+template <int D, int N, typename T>
 void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
 {
     const bool split = c->P.use_split_form != 0;
@@ -43,12 +43,12 @@ void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
                 else
                     hipLaunchKernelGGL((line_kernel_ns<D, N>), vgrid, vblock, 0, c->stream, a, c->ops, c->pc);
             } else {
-                hipLaunchKernelGGL((stage2_kernel<D, N, true, true, RT, T>), grid, block, 0, c->stream, a, c->ops,
+                hipLaunchKernelGGL((stage2_kernel<D, N, true, true, T>), grid, block, 0, c->stream, a, c->ops,
                                    c->pc);
             }
         }
         else if (!split && nsv)
-            hipLaunchKernelGGL((stage2_kernel<D, N, false, true, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
+            hipLaunchKernelGGL((stage2_kernel<D, N, false, true, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
         else if (split) {
             if constexpr (line_waves<D, N, T>() > 0)
                 hipLaunchKernelGGL((line_kernel_wb<D, N, T, line_waves<D, N, T>()>), lgrid, lblock, 0, c->stream, a,
@@ -56,22 +56,23 @@ void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
             else
                 hipLaunchKernelGGL((line_kernel<D, N, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
         } else
-            hipLaunchKernelGGL((stage2_kernel<D, N, false, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
+            hipLaunchKernelGGL((stage2_kernel<D, N, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
     } else {
         if (split)
             hipLaunchKernelGGL((line_kernel<D, N, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
         else
-            hipLaunchKernelGGL((stage2_kernel<D, N, false, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
+            hipLaunchKernelGGL((stage2_kernel<D, N, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
     }
 }
 
 template <int D, int N>
 void stage_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
 {
-    // FP64: fluxes in double with the reference's entropy-projection round trip;
-    // mixed: fluxes and contractions in float, state and RK registers in double
-    if (c->P.precision == DGX_MIXED) stage2_dn<D, N, float, false>(c, a, grid, block);
-    else stage2_dn<D, N, double, true>(c, a, grid, block);
+    // FP64: fluxes in double; mixed: fluxes and contractions in float, state and RK registers
+    // in double.  The reference's entropy projection u(v(u)) is the identity for collocated GLL
+    // nodes and is formed by neither.
+    if (c->P.precision == DGX_MIXED) stage2_dn<D, N, float>(c, a, grid, block);
+    else stage2_dn<D, N, double>(c, a, grid, block);
 }
 
 template <int D, int N>

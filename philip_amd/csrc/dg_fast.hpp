@@ -6,8 +6,8 @@
 // speed, enthalpy) are computed once per node and reused by every pair and
 // face that touches the node.  In float the reciprocals, square roots and
 // logarithms are the hardware v_rcp_f32 / v_sqrt_f32 / v_log_f32 instructions;
-// in double they are the IEEE operations (correctly rounded division and
-// square root).
+// in double the reciprocal is v_rcp_f64 with two Newton-Raphson steps, square
+// root and logarithm the library's double routines.
 //
 //   ec_flux_fast   Chandrashekar / Ranocha two-point flux (euler.cpp:709-807):
 //                  log means from f = (a-b)/(a+b), and 1/(2 beta_ln (gamma-1))
@@ -30,7 +30,17 @@ template <> struct Fast<float> {
     static __device__ __forceinline__ float log_(float x) { return __logf(x); }
 };
 template <> struct Fast<double> {
-    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    // v_rcp_f64 refined by two Newton-Raphson steps r <- r + r (1 - x r): within an ulp or two of
+    // the correctly rounded 1/x at half the instructions of the IEEE division sequence
+    // (FP64 stage kernel at p=3: 206 -> 197 us per stage)
+    static __device__ __forceinline__ double rcp(double x)
+    {
+        double r = __builtin_amdgcn_rcp(x);
+        double e = __builtin_fma(-x, r, 1.0);
+        r = __builtin_fma(r, e, r);
+        e = __builtin_fma(-x, r, 1.0);
+        return __builtin_fma(r, e, r);
+    }
     static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
     static __device__ __forceinline__ double log_(double x) { return log(x); }
 };

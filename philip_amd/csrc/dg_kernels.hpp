@@ -1,7 +1,7 @@
 // Shared pieces of the strong-form DG kernels for collocated Gauss-Lobatto
 // nodes on periodic Cartesian hexahedra/quadrilaterals: operator tables, stage
-// arguments, node indexing, the entropy round trip, and the auxiliary
-// (lifted-gradient) kernel.  The fused stage kernel is in dg_stage2.hpp.
+// arguments, node indexing, and the auxiliary (lifted-gradient) kernel.  The
+// fused stage kernel is in dg_stage2.hpp.
 // Reference path restated:
 //   volume   src/dg/strong_dg.cpp:1017-1579   (conservative D-divergence or
 //            Hadamard skew two-point form, Chan 2019 eq. 15)
@@ -119,14 +119,6 @@ __device__ __forceinline__ int node_with(int i, int k, int l)
     node_coords<D, N>(i, c);
     c[k] = l;
     return (D == 3) ? c[0] + N * (c[1] + N * c[2]) : c[0] + N * c[1];
-}
-
-template <typename T, int D>
-__device__ __forceinline__ void roundtrip(const T *u, const PhysConst &pc, T *ut)
-{
-    T v[D + 2];
-    entropy_variables<T, D>(u, pc, v);
-    conservative_from_entropy<T, D>(v, pc, ut);
 }
 
 // Streaming access to the RK registers: each value is read or written once per

@@ -22,7 +22,6 @@ struct PhysConst {
 
 template <typename T> __device__ __forceinline__ T dsqrt(T x) { return sqrt(x); }
 template <typename T> __device__ __forceinline__ T dlog(T x) { return log(x); }
-template <typename T> __device__ __forceinline__ T dexp(T x) { return exp(x); }
 template <typename T> __device__ __forceinline__ T dpow(T x, T y) { return pow(x, y); }
 template <typename T> __device__ __forceinline__ T dabs(T x) { return fabs(x); }
 template <> __device__ __forceinline__ float dpow<float>(float x, float y) { return powf(x, y); }
@@ -35,7 +34,6 @@ template <typename T> __device__ __forceinline__ T fdiv(T a, T b) { return a / b
 template <> inline float dsqrt<float>(float x) { return sqrtf(x); }
 #else
 template <> __device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
-template <> __device__ __forceinline__ float dexp<float>(float x) { return __expf(x); }
 template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 #endif
@@ -78,20 +76,6 @@ __device__ __forceinline__ void entropy_variables(const T *u, const PhysConst &p
     v[0] = (rho_theta * (gam + T(1) - s) - u[D + 1]) / rho_theta;
     for (int k = 0; k < D; ++k) v[k + 1] = u[k + 1] / rho_theta;
     v[D + 1] = -rho / rho_theta;
-}
-
-// compute_conservative_variables_from_entropy_variables (euler.cpp:832-853)
-template <typename T, int D>
-__device__ __forceinline__ void conservative_from_entropy(const T *v, const PhysConst &pc, T *u)
-{
-    const T gam = T(pc.gam), gamm1 = T(pc.gamm1);
-    T vv = T(0);
-    for (int k = 0; k < D; ++k) vv += v[k + 1] * v[k + 1];
-    const T s = gam - v[0] + T(0.5) * vv / v[D + 1];
-    const T rho_theta = dpow<T>((gamm1 / dpow<T>(-v[D + 1], gam)), T(1) / gamm1) * dexp<T>(-s / gamm1);
-    u[0] = -rho_theta * v[D + 1];
-    for (int k = 0; k < D; ++k) u[k + 1] = rho_theta * v[k + 1];
-    u[D + 1] = rho_theta * (T(1) - T(0.5) * vv / v[D + 1]);
 }
 
 // compute_ismail_roe_logarithmic_mean (euler.cpp:629-647)

@@ -2,9 +2,10 @@
 //
 // Per block: EPB whole elements.  Work items change shape between phases so
 // that every lane has work:
-//   A  node-parallel   load the stage state (FP64, coalesced), entropy
-//                      round trip for the split form, primitive cache and
-//                      (conservative/viscous) volume fluxes into LDS
+//   A  node-parallel   load the stage state (FP64, coalesced), primitive
+//                      cache and (conservative/viscous) volume fluxes into
+//                      LDS (the split form's entropy projection u(v(u)) is
+//                      the identity for collocated GLL nodes, not formed)
 //   C  face-parallel   one item per (face, face node): neighbour trace from
 //                      HBM, surface numerical flux (reference owner
 //                      orientation: the lower cell index is "interior"),
@@ -77,7 +78,7 @@ __device__ __forceinline__ int pair_index(int a, int b)
     return a * (2 * N - a - 1) / 2 + (b - a - 1);
 }
 
-template <int D, int N, bool SPLIT, bool NSV, bool RT, typename T>
+template <int D, int N, bool SPLIT, bool NSV, typename T>
 __global__ void __launch_bounds__((Geo<D, N>::BLOCK))
 stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
@@ -108,11 +109,6 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
         for (int s = 0; s < NS; ++s) ud[s] = a.u[((long)e * NS + s) * NP + i];
         T ut[NS];
         for (int s = 0; s < NS; ++s) ut[s] = T(ud[s]);
-        if (SPLIT && RT) {
-            T tmp[NS];
-            for (int s = 0; s < NS; ++s) tmp[s] = ut[s];
-            roundtrip<T, D>(tmp, pc, ut);
-        }
         for (int s = 0; s < NS; ++s) sU[le][s][i] = ut[s];
         const T pr = pressure<T, D>(ut, gamm1);
         flag_nonphysical<T>(a, ut[0], pr);
@@ -173,9 +169,7 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                 und[s] = a.u[((long)nb * NS + s) * NP + fj];
                 un[s] = T(und[s]);
             }
-            if (SPLIT && RT) roundtrip<T, D>(un, pc, unt);
-            else
-                for (int s = 0; s < NS; ++s) unt[s] = un[s];
+            for (int s = 0; s < NS; ++s) unt[s] = un[s];
             // reference orientation: the lower cell index is the interior side; selected by value so
             // the flux evaluation itself is not under a divergent branch
             const bool own = fe < nb;
