@@ -191,8 +191,9 @@ def batched_vortex(n_inst=256, steps=200, seed=2024, precision="mixed"):
     b.allocate("rk4", dts)
     tiles = {}
     for k, kw in enumerate(kws):
-        epb, waves = b.tile(k)
-        tiles[f"p{kw['poly_degree']}"] = dict(elements_per_block=epb, waves_per_simd_bound=waves)
+        epb, waves, threads = b.tile(k)
+        tiles[f"p{kw['poly_degree']}"] = dict(elements_per_block=epb, waves_per_simd_bound=waves,
+                                              threads_per_block=threads)
     b.step_in_time(2)
     b.synchronize()
     t0 = time.perf_counter()

@@ -252,7 +252,8 @@ int dgx_batch_size(const dgx_batch *b);
 dgx_ctx *dgx_batch_instance(dgx_batch *b, int i);
 /* Allocates every instance's RK registers for an explicit Butcher method with at least two
    stages (Heun, SSPRK3, RK4) and the time step dt[i] of each instance, picks the tile of each
-   (dim, p, precision) group (elements per block x waves/SIMD bound; 6 candidates in 2D, 2 in 3D)
+   (dim, p, precision) group (elements per block x waves/SIMD bound x threads per block; 12
+   candidates in 2D, 2 in 3D)
    by timing every candidate on the group's own instances (cached per process), and captures one
    RK step of all instances in a HIP graph.  No tile changes an element's arithmetic: each
    instance's result is its single solve's, bit for bit. */
@@ -261,12 +262,17 @@ int dgx_batch_allocate(dgx_batch *b, int method, const double *dt);
    current_time / current_iteration advance as in dgx_step_in_time. */
 int dgx_batch_step_in_time(dgx_batch *b, int n_steps);
 int dgx_batch_synchronize(dgx_batch *b);
+/* Forces candidate tile `tile` (0..11; 2D shapes take all twelve, 3D the first two) for every
+   group of the next dgx_batch_allocate whose shape has it, instead of timing the candidates;
+   -1 restores the autotuner.  For reproducible launch shapes and for tests: every tile gives
+   the same results bit for bit. */
+int dgx_batch_set_tile(dgx_batch *b, int tile);
 /* Tile chosen for instance i's group, as 1 + its candidate index; 0 if the instance is launched
    on its own (schemes other than split-form Euler). */
 int dgx_batch_variant(const dgx_batch *b, int i);
-/* The same tile as (elements per block, waves/SIMD bound; 0 = the compiler's allocation); both 0
-   for an instance launched on its own. */
-int dgx_batch_tile(const dgx_batch *b, int i, int *elements_per_block, int *waves_bound);
+/* The same tile as (elements per block, waves/SIMD bound with 0 = the compiler's allocation,
+   threads per block); all 0 for an instance launched on its own. */
+int dgx_batch_tile(const dgx_batch *b, int i, int *elements_per_block, int *waves_bound, int *threads_per_block);
 
 /* ---------------------------------------------------------------------------
    Serving: a request loop over independent parametric solves.  No counterpart

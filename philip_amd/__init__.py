@@ -37,7 +37,7 @@ EXPORTED = [
     "dgx_get_error_adaptive_step_size", "dgx_set_relaxation_runge_kutta", "dgx_relaxation_parameter",
     "dgx_batch_create", "dgx_batch_destroy", "dgx_batch_last_error", "dgx_batch_size", "dgx_batch_instance",
     "dgx_batch_allocate", "dgx_batch_step_in_time", "dgx_batch_synchronize", "dgx_batch_variant",
-    "dgx_batch_tile",
+    "dgx_batch_tile", "dgx_batch_set_tile",
     "dgx_request_bytes", "dgx_request_n_dofs", "dgx_server_create", "dgx_server_destroy", "dgx_server_last_error",
     "dgx_server_submit", "dgx_server_step", "dgx_server_result", "dgx_server_release", "dgx_server_memory",
 ]
@@ -155,7 +155,8 @@ def load():
     lib.dgx_batch_step_in_time.argtypes = [vp, i]
     lib.dgx_batch_synchronize.argtypes = [vp]
     lib.dgx_batch_variant.argtypes = [vp, i]
-    lib.dgx_batch_tile.argtypes = [vp, i, P(ctypes.c_int), P(ctypes.c_int)]
+    lib.dgx_batch_tile.argtypes = [vp, i, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]
+    lib.dgx_batch_set_tile.argtypes = [vp, i]
     lib.dgx_request_bytes.restype = i64
     lib.dgx_request_bytes.argtypes = [P(Request), i]
     lib.dgx_request_n_dofs.restype = i64
@@ -486,11 +487,16 @@ class Batch:
     def variant(self, i):
         return load().dgx_batch_variant(self.h, i)
 
+    def set_tile(self, tile):
+        """Force candidate tile `tile` (-1: autotune) for the next allocate."""
+        self._chk(load().dgx_batch_set_tile(self.h, tile), "batch_set_tile")
+
     def tile(self, i):
-        """(elements per block, waves/SIMD bound) of instance i's group; (0, 0) if launched alone."""
-        e, w = ctypes.c_int(), ctypes.c_int()
-        self._chk(load().dgx_batch_tile(self.h, i, ctypes.byref(e), ctypes.byref(w)), "batch_tile")
-        return e.value, w.value
+        """(elements per block, waves/SIMD bound, threads per block) of instance i's group; zeros if
+        the instance is launched on its own."""
+        e, w, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._chk(load().dgx_batch_tile(self.h, i, ctypes.byref(e), ctypes.byref(w), ctypes.byref(t)), "batch_tile")
+        return e.value, w.value, t.value
 
     def close(self):
         if getattr(self, "h", None):

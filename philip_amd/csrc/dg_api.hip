@@ -1180,6 +1180,7 @@ struct dgx_batch {
     std::vector<double> dt;
     std::vector<BatchGroup> groups;
     std::vector<int> variant;            // per instance
+    int forced_tile = -1;                // dgx_batch_set_tile; -1: autotuned
     int method = -1;
     hipStream_t stream = nullptr;
     std::vector<hipStream_t> side;       // one per group after the first: the groups' launches of a
@@ -1244,6 +1245,7 @@ static int batch_blocks(dgx_batch *b, BatchGroup &g, int tile)
 // first RK stage (median of five launches after a warm-up) and keeps the fastest.
 static int batch_autotune(dgx_batch *b, BatchGroup &g, const Ops1D &ops)
 {
+    if (b->forced_tile >= 0 && batch_blocks(b, g, b->forced_tile) == DGX_OK) return DGX_OK;
     const long key = ((long)g.dim * 16 + g.n) * 4 + g.precision;
     for (auto &kv : g_variant_cache)
         if (kv.first == key) return batch_blocks(b, g, kv.second);
@@ -1336,11 +1338,20 @@ int dgx_batch_variant(const dgx_batch *b, int i)
     return b->variant[i];
 }
 
-int dgx_batch_tile(const dgx_batch *b, int i, int *elements_per_block, int *waves_bound)
+int dgx_batch_set_tile(dgx_batch *b, int tile)
 {
-    if (!b || i < 0 || i >= (int)b->inst.size() || !elements_per_block || !waves_bound) return DGX_ERR_INVALID;
+    if (!b || tile < -1 || tile >= BATCH_TILES_2D) return DGX_ERR_INVALID;
+    b->forced_tile = tile;
+    return DGX_OK;
+}
+
+int dgx_batch_tile(const dgx_batch *b, int i, int *elements_per_block, int *waves_bound, int *threads_per_block)
+{
+    if (!b || i < 0 || i >= (int)b->inst.size() || !elements_per_block || !waves_bound || !threads_per_block)
+        return DGX_ERR_INVALID;
     *elements_per_block = 0;
     *waves_bound = 0;
+    *threads_per_block = 0;
     const int v = b->variant[i];
     if (!v) return DGX_OK;
     const dgx_ctx *c = b->inst[i];
@@ -1348,7 +1359,7 @@ int dgx_batch_tile(const dgx_batch *b, int i, int *elements_per_block, int *wave
     const BatchTile t = batch_tile(c->dim, fast, v - 1);
     *waves_bound = t.waves;
     return launch_line_batch(c->dim, c->n, c->P.precision, v - 1, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr,
-                             elements_per_block);
+                             elements_per_block, threads_per_block);
 }
 
 int dgx_batch_allocate(dgx_batch *b, int method, const double *dt)

@@ -114,21 +114,22 @@ int launch_aux_dn(dgx_ctx *c, const double *u)
     return check_launch(c, "aux_kernel");
 }
 
-template <int D, int N, typename T, int ES, int W>
+template <int D, int N, typename T, int ES, int W, bool XJ>
 int line_batch_es(const BatchBlock *blocks, int nblocks, const StageArgs *args, const PhysConst *pcs,
-                  const Ops1D &ops, hipStream_t st, int *epb)
+                  const Ops1D &ops, hipStream_t st, int *epb, int *threads)
 {
-    using LG = LineGeo<D, N, T, ES>;
-    if constexpr (LG::SBUF * sizeof(T) > 160 * 1024 - 4096) {
-        return DGX_ERR_UNSUPPORTED;  // would not fit the 160 KB of LDS a block can hold
+    using LG = LineGeo<D, N, T, ES, false, XJ>;
+    if constexpr (LG::SBUF * sizeof(T) > 160 * 1024 - 4096 || LG::BLOCK > 1024) {
+        return DGX_ERR_UNSUPPORTED;  // would not fit the 160 KB of LDS or the 1024 threads of a block
     } else {
         if (epb) *epb = LG::EPB;
+        if (threads) *threads = LG::BLOCK;
         if (!blocks) return DGX_OK;  // query only
         if constexpr (W > 0)
-            hipLaunchKernelGGL((line_kernel_batch_wb<D, N, T, ES, W>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
-                               args, pcs, ops);
+            hipLaunchKernelGGL((line_kernel_batch_wb<D, N, T, ES, W, XJ>), dim3(nblocks), dim3(LG::BLOCK), 0, st,
+                               blocks, args, pcs, ops);
         else
-            hipLaunchKernelGGL((line_kernel_batch<D, N, T, ES>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
+            hipLaunchKernelGGL((line_kernel_batch<D, N, T, ES, XJ>), dim3(nblocks), dim3(LG::BLOCK), 0, st, blocks,
                                args, pcs, ops);
         return hipGetLastError() == hipSuccess ? DGX_OK : DGX_ERR_HIP;
     }
@@ -136,17 +137,17 @@ int line_batch_es(const BatchBlock *blocks, int nblocks, const StageArgs *args, 
 
 template <int D, int N, typename T>
 int line_batch_t(int tile, const BatchBlock *blocks, int nblocks, const StageArgs *args, const PhysConst *pcs,
-                 const Ops1D &ops, hipStream_t st, int *epb)
+                 const Ops1D &ops, hipStream_t st, int *epb, int *threads)
 {
     constexpr bool F = std::is_same<T, float>::value;
-    constexpr int W = batch_waves(D, F);
 #define LBT_CASE(V)                                                                                              \
     case V:                                                                                                      \
-        return line_batch_es<D, N, T, batch_tile(D, F, V).es, batch_tile(D, F, V).waves>(blocks, nblocks, args, pcs, \
-                                                                                         ops, st, epb);
-    if constexpr (W > 0) {
+        return line_batch_es<D, N, T, batch_tile(D, F, V).es, batch_tile(D, F, V).waves, batch_tile(D, F, V).xj>( \
+            blocks, nblocks, args, pcs, ops, st, epb, threads);
+    if constexpr (D == 2) {
         switch (tile) {
             LBT_CASE(0) LBT_CASE(1) LBT_CASE(2) LBT_CASE(3) LBT_CASE(4) LBT_CASE(5)
+            LBT_CASE(6) LBT_CASE(7) LBT_CASE(8) LBT_CASE(9) LBT_CASE(10) LBT_CASE(11)
         default: return DGX_ERR_UNSUPPORTED;
         }
     } else {
@@ -160,11 +161,12 @@ int line_batch_t(int tile, const BatchBlock *blocks, int nblocks, const StageArg
 
 template <int D, int N>
 int line_batch_dn(int precision, int tile, const BatchBlock *blocks, int nblocks, const StageArgs *args,
-                  const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb)
+                  const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb, int *threads)
 {
     if (tile < 0 || tile >= batch_tile_count(D)) return DGX_ERR_UNSUPPORTED;
-    if (precision == DGX_MIXED) return line_batch_t<D, N, float>(tile, blocks, nblocks, args, pcs, ops, st, epb);
-    return line_batch_t<D, N, double>(tile, blocks, nblocks, args, pcs, ops, st, epb);
+    if (precision == DGX_MIXED)
+        return line_batch_t<D, N, float>(tile, blocks, nblocks, args, pcs, ops, st, epb, threads);
+    return line_batch_t<D, N, double>(tile, blocks, nblocks, args, pcs, ops, st, epb, threads);
 }
 
 // explicit instantiation (in dg_inst_n<NN>.hip) and its extern declaration (in dg_launch.hip)
@@ -173,8 +175,8 @@ int line_batch_dn(int precision, int tile, const BatchBlock *blocks, int nblocks
     PREFIX template int launch_stage_dn<3, NN>(dgx_ctx *, const StageArgs &);                              \
     PREFIX template int launch_aux_dn<3, NN>(dgx_ctx *, const double *);                                   \
     PREFIX template int line_batch_dn<2, NN>(int, int, const BatchBlock *, int, const StageArgs *,          \
-                                             const PhysConst *, const Ops1D &, hipStream_t, int *);          \
+                                             const PhysConst *, const Ops1D &, hipStream_t, int *, int *);   \
     PREFIX template int line_batch_dn<3, NN>(int, int, const BatchBlock *, int, const StageArgs *,          \
-                                             const PhysConst *, const Ops1D &, hipStream_t, int *);
+                                             const PhysConst *, const Ops1D &, hipStream_t, int *, int *);
 
 } // namespace dgx

@@ -116,10 +116,12 @@ int launch_lsrk_error(dgx_ctx *c, const double *s1, const double *e, const doubl
                       double sum_delta, double atol, double rtol, double *sum_out);
 // Batched split-form line kernel over the element groups `blocks` of several solves of one
 // (dim, n, precision); tile = index of the candidate tile (batch_tile: elements per block x
-// waves/SIMD bound).  With blocks == nullptr only reports the tile's elements per block in *epb.
-// DGX_ERR_UNSUPPORTED if the tile does not exist for this shape.
+// waves/SIMD bound x threads per block).  With blocks == nullptr only reports the tile's elements
+// per block in *epb and threads per block in *threads (either may be null).  DGX_ERR_UNSUPPORTED
+// if the tile does not exist for this shape.
 int launch_line_batch(int dim, int n, int precision, int tile, const BatchBlock *blocks, int nblocks,
-                      const StageArgs *args, const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb);
+                      const StageArgs *args, const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb,
+                      int *threads = nullptr);
 int launch_aux(dgx_ctx *c, const double *u);
 int launch_integrated(dgx_ctx *c, const double *u, int overint, double *host_out4);
 int launch_integrated_static(dgx_ctx *c, const double *u, int overint, double *out4_dev);

@@ -39,12 +39,13 @@ int launch_stage(dgx_ctx *c, const StageArgs &a) { DISPATCH_N(launch_stage_dn, c
 // (dim, p, precision); es selects the elements-per-block variant.
 // ---------------------------------------------------------------------------
 int launch_line_batch(int dim, int n, int precision, int tile, const BatchBlock *blocks, int nblocks,
-                      const StageArgs *args, const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb)
+                      const StageArgs *args, const PhysConst *pcs, const Ops1D &ops, hipStream_t st, int *epb,
+                      int *threads)
 {
-#define LB_CASE(NN)                                                                                   \
-    case NN:                                                                                          \
-        return (dim == 3) ? line_batch_dn<3, NN>(precision, tile, blocks, nblocks, args, pcs, ops, st, epb) \
-                          : line_batch_dn<2, NN>(precision, tile, blocks, nblocks, args, pcs, ops, st, epb);
+#define LB_CASE(NN)                                                                                            \
+    case NN:                                                                                                   \
+        return (dim == 3) ? line_batch_dn<3, NN>(precision, tile, blocks, nblocks, args, pcs, ops, st, epb, threads) \
+                          : line_batch_dn<2, NN>(precision, tile, blocks, nblocks, args, pcs, ops, st, epb, threads);
     switch (n) {
         LB_CASE(2)
         LB_CASE(3)

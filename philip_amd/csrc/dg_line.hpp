@@ -49,12 +49,13 @@ constexpr bool LINE_PREFETCH_RK_F = true;
 constexpr bool LINE_PREFETCH_RK_D = false;
 template <int D, int N> constexpr bool line_late_prefetch_d() { return !(D == 3 && N == 6); }
 // ES: elements per block relative to the 256-node default (ES > 0: ES times,
-// ES < 0: 1/|ES|); per-element strides do not depend on it.
-template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
+// ES < 0: 1/|ES|); per-element strides do not depend on it.  XJ: the block has a
+// thread for every item of the line phase (one item pass, the extra threads idle in
+// the node phases) instead of one thread per node.
+template <int D, int N, typename T, int ES = 1, bool V = false, bool XJ = false> struct LineGeo {
     using G = Geo<D, N>;
     static constexpr int NP = G::NP, NS = D + 2, NL = G::NL;
     static constexpr int EPB = ES > 0 ? G::EPB * ES : (G::EPB / -ES > 0 ? G::EPB / -ES : 1);
-    static constexpr int BLOCK = ((EPB * NP + 63) / 64) * 64;
     static constexpr bool FAST = std::is_same<T, float>::value;
     // node cache fields: rho, E, vel[D], p, b, |v|^2, sqrt(rho), c, h
     static constexpr int NQ = D + 8;
@@ -82,6 +83,9 @@ template <int D, int N, typename T, int ES = 1, bool V = false> struct LineGeo {
     static constexpr int ESN = pad1(NS * NFN);
     static constexpr int ITEMS = EPB * D * NL * LS;
     static constexpr int FITEMS = EPB * NFN;
+    static constexpr int round64(int x) { return ((x + 63) / 64) * 64; }
+    static constexpr int BLOCK = XJ ? round64(EPB * NP > ITEMS + FITEMS ? EPB * NP : ITEMS + FITEMS)
+                                    : round64(EPB * NP);
     // Navier-Stokes (V, mixed path only): own viscous flux in every direction
     // [direction][state][node] and the neighbours' viscous flux along the face direction at
     // their face nodes [state][face][face node]
@@ -347,10 +351,10 @@ __device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, co
 // One block's work: elements blk0 .. blk0 + EPB - 1 of the solve described by a.
 // V: Navier-Stokes (split convective form plus the viscous volume and surface
 // terms, with the lifted gradient a.aux computed beforehand).
-template <int D, int N, typename T, int ES, bool V = false>
+template <int D, int N, typename T, int ES, bool V = false, bool XJ = false>
 __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops, const PhysConst &pc, const int blk0)
 {
-    using LG = LineGeo<D, N, T, ES, V>;
+    using LG = LineGeo<D, N, T, ES, V, XJ>;
     constexpr int NP = LG::NP, NS = LG::NS, NL = LG::NL, EPB = LG::EPB, BLOCK = LG::BLOCK, LS = LG::LS;
     constexpr bool FAST = LG::FAST;
     constexpr int NFI = (LG::FITEMS + BLOCK - 1) / BLOCK;  // face-trace loads per thread
@@ -664,40 +668,42 @@ line_kernel_ns_w4(StageArgs a, Ops1D ops, PhysConst pc)
 struct BatchBlock {
     int inst, blk0;
 };
-template <int D, int N, typename T, int ES>
-__global__ void __launch_bounds__((LineGeo<D, N, T, ES>::BLOCK))
+template <int D, int N, typename T, int ES, bool XJ>
+__global__ void __launch_bounds__((LineGeo<D, N, T, ES, false, XJ>::BLOCK))
 line_kernel_batch(const BatchBlock *__restrict__ blocks, const StageArgs *__restrict__ args,
                   const PhysConst *__restrict__ pcs, Ops1D ops)
 {
     const BatchBlock bb = blocks[xcd_block()];
-    line_block<D, N, T, ES>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
+    line_block<D, N, T, ES, false, XJ>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
 }
 // The same held to W waves/SIMD (amdgpu_waves_per_eu): a candidate tile of the batched
 // autotuner.
-template <int D, int N, typename T, int ES, int W>
-__global__ void __launch_bounds__((LineGeo<D, N, T, ES>::BLOCK)) __attribute__((amdgpu_waves_per_eu(W)))
+template <int D, int N, typename T, int ES, int W, bool XJ>
+__global__ void __launch_bounds__((LineGeo<D, N, T, ES, false, XJ>::BLOCK)) __attribute__((amdgpu_waves_per_eu(W)))
 line_kernel_batch_wb(const BatchBlock *__restrict__ blocks, const StageArgs *__restrict__ args,
                      const PhysConst *__restrict__ pcs, Ops1D ops)
 {
     const BatchBlock bb = blocks[xcd_block()];
-    line_block<D, N, T, ES>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
+    line_block<D, N, T, ES, false, XJ>(args[bb.inst], ops, pcs[bb.inst], bb.blk0);
 }
 
 // Tiles the batched autotuner times per (dim, nodes per direction, precision): elements per
 // block (ES: 1 = the 256-node default, 2 = twice as many, -2 = half as many) x a waves/SIMD bound
 // (0 = the compiler's register allocation; else amdgpu_waves_per_eu(W), trading spills for
-// occupancy).  Neither changes the arithmetic of an element (the line split LS is fixed per
-// shape for that reason: it reorders the pair sums), so every tile gives the single solve's bits.
+// occupancy) x threads per block (one per node, or one per line-phase item: XJ).  None changes
+// the arithmetic of an element (the line split LS is fixed per shape for that reason: it
+// reorders the pair sums), so every tile gives the single solve's bits.
 struct BatchTile {
     int es, waves;
+    bool xj;
 };
-constexpr int BATCH_TILES_2D = 6, BATCH_TILES_3D = 2;
+constexpr int BATCH_TILES_2D = 12, BATCH_TILES_3D = 2;
 constexpr int batch_tile_count(int dim) { return dim == 2 ? BATCH_TILES_2D : BATCH_TILES_3D; }
 constexpr int batch_waves(int dim, bool fast) { return dim == 2 ? (fast ? 5 : 4) : 0; }
 constexpr BatchTile batch_tile(int dim, bool fast, int v)
 {
     const int es = (v % 3 == 0) ? 1 : (v % 3 == 1 ? 2 : -2);
-    return BatchTile{es, v >= 3 ? batch_waves(dim, fast) : 0};
+    return BatchTile{es, (v / 3) % 2 ? batch_waves(dim, fast) : 0, v >= 6};
 }
 
 } // namespace dgx

@@ -107,6 +107,43 @@ def test_batch_fp64_instance_matches_oracle():
     b.close()
 
 
+def test_every_tile_gives_the_single_solve_bits():
+    """Each candidate tile of the autotuner (elements per block x waves bound x threads per block),
+    forced in turn, on 2D instances of every degree at both precisions and 3D p=3 (which has the
+    first two tiles): 3 RK4 steps equal the single solves bit for bit."""
+    cases = []
+    for p_ in (2, 3, 4, 5, 6):
+        for prec in (P.MIXED, P.FP64):
+            cases.append((dict(dim=2, poly_degree=p_, n_cells_1d=7, domain_left=-5.0, domain_right=5.0,
+                               mach_inf=MACH_VORTEX, use_split_form=1, two_point_flux=P.CH,
+                               conv_flux=P.TWO_POINT_ROE, precision=prec), (3.0, 0.7, 0.8)))
+    cases.append((dict(dim=3, poly_degree=3, n_cells_1d=4, use_split_form=1, two_point_flux=P.CH,
+                       conv_flux=P.TWO_POINT_ROE, precision=P.MIXED), None))
+    dts = [0.002] * len(cases)
+    alone = []
+    for (kw, ic), dt in zip(cases, dts):
+        s = P.DG(**kw)
+        setup(s, kw, ic)
+        s.allocate_ode_system("rk4")
+        s.step_in_time(dt, 3)
+        alone.append(s.solution())
+        s.close()
+    for tile in range(12):
+        b = P.Batch([kw for kw, _ in cases])
+        b.set_tile(tile)
+        for d, (kw, ic) in zip(b.instances, cases):
+            setup(d, kw, ic)
+        b.allocate("rk4", dts)
+        b.step_in_time(3)
+        b.synchronize()
+        shapes = set()
+        for k, d in enumerate(b.instances):
+            shapes.add(b.tile(k))
+            assert np.array_equal(d.solution(), alone[k]), (tile, k)
+        log(f"  tile {tile}: (elements/block, waves bound, threads/block) {sorted(shapes)}: all equal")
+        b.close()
+
+
 def test_batch_rejects_one_stage_methods_and_locks_registers():
     kw, ic = vortex_cases(3, 1)[0]
     b = P.Batch([kw])
