@@ -448,9 +448,14 @@ def main():
             line["serving"].append(serving(precision=pr, rate=rate))
             progress(f"serving {pr}: p50 {line['serving'][-1]['p50_s']:.3f} s, p99 {line['serving'][-1]['p99_s']:.3f} s")
         if args.serving_sweep:
-            # arrival rate swept at both precisions (same request stream per rate), shorter final times
-            line["serving_sweep"] = [serving(n_req=100, rate=r, precision=pr, time_scale=0.1)
-                                     for pr in ("mixed", "fp64") for r in (20.0, 40.0, 80.0)]
+            # arrival rate swept at both precisions (the same 100-request stream per rate, the
+            # configuration's final times)
+            line["serving_sweep"] = []
+            for pr, rates in (("mixed", (16.0, 32.0, 64.0)), ("fp64", (8.0, 16.0, 32.0))):
+                for r in rates:
+                    line["serving_sweep"].append(serving(n_req=100, rate=r, precision=pr))
+                    progress(f"serving sweep {pr} {r:g}/s: p50 {line['serving_sweep'][-1]['p50_s']:.3f} s, "
+                             f"p99 {line['serving_sweep'][-1]['p99_s']:.3f} s")
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or host_cores()[0]
         line["cpu_baseline"] = cpu_baseline(threads)

@@ -1221,28 +1221,37 @@ static bool batch_line_eligible(const dgx_ctx *c)
     return c->P.use_split_form && c->P.pde == DGX_EULER;
 }
 
-// block table of a group for tile `tile`: (slot, first element) per element group
-static int batch_blocks(dgx_batch *b, BatchGroup &g, int tile)
+// block table of a group for `epb` elements per block: (slot, first element) per element group
+static int group_table(dgx_batch *b, const BatchGroup &g, int epb, BatchBlock **out, int *nblocks)
 {
-    int epb = 0;
-    int rc = launch_line_batch(g.dim, g.n, g.precision, tile, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr, &epb);
-    if (rc) return rc;
     std::vector<BatchBlock> tab;
     for (int slot = 0; slot < (int)g.inst.size(); ++slot) {
         const dgx_ctx *c = b->inst[g.inst[slot]];
         for (int e = 0; e < c->ncells; e += epb) tab.push_back(BatchBlock{slot, e});
     }
+    *out = nullptr;
+    BHIPCHK(hipMalloc(out, sizeof(BatchBlock) * tab.size()));
+    BHIPCHK(hipMemcpy(*out, tab.data(), sizeof(BatchBlock) * tab.size(), hipMemcpyHostToDevice));
+    *nblocks = (int)tab.size();
+    return DGX_OK;
+}
+
+// block table of a group for tile `tile`
+static int batch_blocks(dgx_batch *b, BatchGroup &g, int tile)
+{
+    int epb = 0;
+    int rc = launch_line_batch(g.dim, g.n, g.precision, tile, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr, &epb);
+    if (rc) return rc;
     (void)hipFree(g.blocks);
     g.blocks = nullptr;
-    BHIPCHK(hipMalloc(&g.blocks, sizeof(BatchBlock) * tab.size()));
-    BHIPCHK(hipMemcpy(g.blocks, tab.data(), sizeof(BatchBlock) * tab.size(), hipMemcpyHostToDevice));
-    g.nblocks = (int)tab.size();
+    rc = group_table(b, g, epb, &g.blocks, &g.nblocks);
+    if (rc) return rc;
     g.tile = tile;
     return DGX_OK;
 }
 
-// Times every candidate tile (elements per block x waves/SIMD bound, batch_tile) on the group's
-// first RK stage (median of five launches after a warm-up) and keeps the fastest.
+// Times every candidate tile (batch_tile) on the group's first RK stage (median of seven
+// launches after a warm-up) and keeps the fastest.
 static int batch_autotune(dgx_batch *b, BatchGroup &g, const Ops1D &ops)
 {
     if (b->forced_tile >= 0 && batch_blocks(b, g, b->forced_tile) == DGX_OK) return DGX_OK;
@@ -1255,28 +1264,57 @@ static int batch_autotune(dgx_batch *b, BatchGroup &g, const Ops1D &ops)
     hipEvent_t e0, e1;
     BHIPCHK(hipEventCreate(&e0));
     BHIPCHK(hipEventCreate(&e1));
-    int best = 0;
-    float best_ms = 1e30f;
+    // candidates timed round-robin (one launch of each per round, the first round a warm-up), so a
+    // drift of the clock or of the other groups' caches during tuning biases no candidate
+    std::vector<int> cand, table;                 // candidate tiles, index of their block table
+    std::vector<std::pair<int, std::pair<BatchBlock *, int>>> tables;  // epb -> (table, blocks)
     for (int tile = 0; tile < batch_tile_count(g.dim); ++tile) {
-        if (batch_blocks(b, g, tile) != DGX_OK) continue;  // tile does not fit this shape
-        float ts[6];
-        for (int rep = 0; rep < 6; ++rep) {  // first launch is a warm-up
+        int epb = 0;
+        if (launch_line_batch(g.dim, g.n, g.precision, tile, nullptr, 0, nullptr, nullptr, Ops1D{}, nullptr, &epb))
+            continue;  // tile does not fit this shape
+        int t = -1;
+        for (size_t k = 0; k < tables.size(); ++k)
+            if (tables[k].first == epb) t = (int)k;
+        if (t < 0) {
+            BatchBlock *tab = nullptr;
+            int nb = 0;
+            const int rc = group_table(b, g, epb, &tab, &nb);
+            if (rc) return rc;
+            tables.push_back({epb, {tab, nb}});
+            t = (int)tables.size() - 1;
+        }
+        cand.push_back(tile);
+        table.push_back(t);
+    }
+    constexpr int ROUNDS = 8;
+    std::vector<std::vector<float>> ts(cand.size());
+    for (int rep = 0; rep < ROUNDS; ++rep) {
+        for (size_t k = 0; k < cand.size(); ++k) {
+            const auto &tb = tables[table[k]].second;
             BHIPCHK(hipEventRecord(e0, b->stream));
-            if (launch_line_batch(g.dim, g.n, g.precision, tile, g.blocks, g.nblocks, pa, g.pcs, ops, b->stream,
+            if (launch_line_batch(g.dim, g.n, g.precision, cand[k], tb.first, tb.second, pa, g.pcs, ops, b->stream,
                                   nullptr))
                 return DGX_ERR_HIP;
             BHIPCHK(hipEventRecord(e1, b->stream));
             BHIPCHK(hipEventSynchronize(e1));
-            BHIPCHK(hipEventElapsedTime(&ts[rep], e0, e1));
+            float t = 0.f;
+            BHIPCHK(hipEventElapsedTime(&t, e0, e1));
+            if (rep) ts[k].push_back(t);
         }
-        std::sort(ts + 1, ts + 6);
-        if (ts[3] < best_ms) {
-            best_ms = ts[3];
-            best = tile;
+    }
+    int best = 0;
+    float best_ms = 1e30f;
+    for (size_t k = 0; k < cand.size(); ++k) {
+        std::sort(ts[k].begin(), ts[k].end());
+        const float med = ts[k][ts[k].size() / 2];
+        if (med < best_ms) {
+            best_ms = med;
+            best = cand[k];
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    for (auto &t : tables) (void)hipFree(t.second.first);
     g_variant_cache.push_back({key, best});
     return batch_blocks(b, g, best);
 }

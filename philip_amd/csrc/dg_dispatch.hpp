@@ -29,7 +29,7 @@ void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
 {
     const bool split = c->P.use_split_form != 0;
     const bool nsv = c->P.pde == DGX_NAVIER_STOKES;
-    using LG = LineGeo<D, N, T>;
+    using LG = LineGeoS<D, N, T>;
     const dim3 lgrid((c->ncells + LG::EPB - 1) / LG::EPB), lblock(LG::BLOCK);
     if constexpr (D == 3) {
         if (split && nsv) {

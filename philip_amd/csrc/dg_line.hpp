@@ -618,12 +618,22 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc, pf || late_pf, pf2, pf3);
 }
 
+// Elements-per-block variant (ES of LineGeo) of the single-solve Euler line kernel: the mixed path
+// in 3D takes twice the default (at p=3 a 2x2x2 Morton brick, so that more faces stay inside the
+// block): p=3 32^3 110 -> 102.5 us per stage, p=4 24^3 125 -> 122 us, p=6 16^3 277 -> 181 us; at
+// p=2 it was slower (90 -> 95 us) and at p=5 no faster; FP64 at p=3 was slower (179 -> 216 us).
+template <int D, int N, typename T> constexpr int line_es_single()
+{
+    return (D == 3 && (N == 4 || N == 5 || N == 7) && std::is_same<T, float>::value) ? 2 : 1;
+}
+template <int D, int N, typename T> using LineGeoS = LineGeo<D, N, T, line_es_single<D, N, T>()>;
+
 // One solve: blocks in XCD-aware order over the element groups.
 template <int D, int N, typename T>
-__global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK))
+__global__ void __launch_bounds__((LineGeoS<D, N, T>::BLOCK))
 line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
-    line_block<D, N, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
+    line_block<D, N, T, line_es_single<D, N, T>(), false>(a, ops, pc, xcd_block() * LineGeoS<D, N, T>::EPB);
 }
 // Waves per SIMD the single-solve 3D line kernel is held to (amdgpu_waves_per_eu; 0: the
 // compiler's choice).  The mixed path keeps the compiler's choice (a 4-wave bound at p=6 was
@@ -638,10 +648,10 @@ template <int D, int N, typename T> constexpr int line_waves()
     return 0;
 }
 template <int D, int N, typename T, int W>
-__global__ void __launch_bounds__((LineGeo<D, N, T>::BLOCK)) __attribute__((amdgpu_waves_per_eu(W)))
+__global__ void __launch_bounds__((LineGeoS<D, N, T>::BLOCK)) __attribute__((amdgpu_waves_per_eu(W)))
 line_kernel_wb(StageArgs a, Ops1D ops, PhysConst pc)
 {
-    line_block<D, N, T, 1, false>(a, ops, pc, xcd_block() * LineGeo<D, N, T>::EPB);
+    line_block<D, N, T, line_es_single<D, N, T>(), false>(a, ops, pc, xcd_block() * LineGeoS<D, N, T>::EPB);
 }
 
 // Navier-Stokes, mixed path.  line_kernel_ns_w4 is held to 4 waves/SIMD (128 VGPRs; a few

@@ -15,6 +15,7 @@ state moves the step sizes by up to ~1e-2.  The loop is therefore checked by the
 the final state (1e-11 relative) and step sizes within five times the oracle's own spread under
 such perturbations (measured in the test).  The initial step itself is a ratio of right-hand-side
 max-norms and agrees to 1e-11."""
+import os
 import sys
 
 import numpy as np
@@ -24,6 +25,15 @@ import oracle as O
 import philip_amd as P
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True, scope="module")
+def deterministic_oracle():
+    """One oracle thread: the OpenMP neighbour scatter adds face terms in a varying order, and these
+    ill-conditioned comparisons would see that rounding as run-to-run noise in the reference."""
+    O.set_threads(1)
+    yield
+    O.set_threads(min(16, os.cpu_count() or 1))
 
 METHODS = ["rk3_2_5f_3sstarplus", "rk4_3_5_3sstar", "rk4_3_9f_3sstarplus"]
 KW = dict(dim=3, poly_degree=2, n_cells_1d=4, use_split_form=1, two_point_flux=O.CH, conv_flux=O.TWO_POINT_ROE)
@@ -79,7 +89,7 @@ def test_error_adaptive_run_matches_oracle(method):
     # the oracle's own step-size spread under rounding-level perturbations of the initial state
     rng = np.random.default_rng(5)
     spread = 0.0
-    for pert in (1e-15, 1e-14, 1e-13):
+    for pert in (1e-15, 1e-15, 1e-14, 1e-14, 1e-13, 1e-13):
         up = u0 * (1.0 + pert * rng.choice([-1.0, 1.0], u0.shape))
         dp = o.run_error_adaptive(method, final, up, atol=tol["atol"], rtol=tol["rtol"], beta=tol["beta"])
         assert len(dp) == len(dts)

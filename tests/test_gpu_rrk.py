@@ -9,8 +9,10 @@ over six steps).
 The relaxation parameter is the root of a difference of integrated entropies that nearly
 cancel, found to a 5e-10 residual tolerance: in the oracle itself a 1e-15 relative perturbation
 of the initial state moves gamma by 2e-8 .. 2.5e-7 over six steps, and the state by up to 1e-9.
-Bars: gamma, state and time within five times the oracle's own spread under such perturbations
-(measured in the test); a wrong entropy, direction or root moves gamma by O(dt^p) ~ 1e-3."""
+Bars: gamma, state and time within five times the oracle's own largest spread over six such
+perturbations (measured in the test, on a single-threaded and so run-to-run deterministic
+oracle); a wrong entropy, direction or root moves gamma by O(dt^p) ~ 1e-3."""
+import os
 import sys
 
 import numpy as np
@@ -20,6 +22,15 @@ import oracle as O
 import philip_amd as P
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True, scope="module")
+def deterministic_oracle():
+    """One oracle thread: the OpenMP neighbour scatter adds face terms in a varying order, and these
+    ill-conditioned comparisons would see that rounding as run-to-run noise in the reference."""
+    O.set_threads(1)
+    yield
+    O.set_threads(min(16, os.cpu_count() or 1))
 
 
 def log(msg):
@@ -55,7 +66,7 @@ def test_rrk_matches_oracle(flux, method):
     gs_ref, u, t_ref = oracle_run(u)
     rng = np.random.default_rng(3)
     sg = su = st = 0.0
-    for pert in (1e-15, 1e-14):
+    for pert in (1e-15, 1e-15, 1e-15, 1e-14, 1e-14, 1e-14):
         gp, up, tp = oracle_run(u0 * (1.0 + pert * rng.choice([-1.0, 1.0], u0.shape)))
         sg = max(sg, np.abs(gp - gs_ref).max())
         su = max(su, np.abs(up - u).max() / np.abs(u).max())
