@@ -73,6 +73,13 @@ typedef struct {
     double domain_right;
     int cell_order;
     int precision;              /* DGX_FP64 or DGX_MIXED (FP32 fluxes, FP64 state)  */
+    /* Boundary condition of each face of the box (deal.II face numbering: 2k = -e_k, 2k+1 =
+       +e_k): 0 = periodic (both faces of a direction), else a type of
+       Euler::boundary_face_values (euler.cpp:1511-1567) applied in assemble_boundary_term_strong
+       (strong_dg.cpp:1583-2364): 1001 wall, 1002 pressure outflow, 1003 inflow, 1004 Riemann
+       farfield, 1005 farfield, 1006 slip wall, 1007 extrapolation.  Euler only. */
+    int boundary_type[6];
+    double angle_of_attack;     /* radians: free stream (cos a, sin a[, 0]) of the boundary states */
 } dgx_params;
 
 typedef struct dgx_ctx dgx_ctx;

@@ -36,6 +36,7 @@ class Params(ctypes.Structure):
         ("temperature_inf", ctypes.c_double), ("sipg_penalty_factor", ctypes.c_double),
         ("n_cells_1d", ctypes.c_int), ("domain_left", ctypes.c_double), ("domain_right", ctypes.c_double),
         ("cell_order", ctypes.c_int),
+        ("boundary_type", ctypes.c_int * 6), ("angle_of_attack", ctypes.c_double),
     ]
 
 
@@ -53,6 +54,7 @@ def _load():
     vp = ctypes.c_void_p
     dp = P(d)
     lib.dgo_create.restype = vp
+    lib.dgo_boundary_state.argtypes = [vp, i, dp, dp, dp]
     lib.dgo_create.argtypes = [P(Params)]
     lib.dgo_destroy.argtypes = [vp]
     lib.dgo_default_params.argtypes = [P(Params)]
@@ -135,6 +137,8 @@ def params(**kw):
     p = Params()
     lib().dgo_default_params(ctypes.byref(p))
     for k, v in kw.items():
+        if k == "boundary_type":
+            v = (ctypes.c_int * 6)(*v)
         setattr(p, k, v)
     return p
 
@@ -189,6 +193,8 @@ class Oracle:
     def __init__(self, **kw):
         self.p = params(**kw)
         self.h = lib().dgo_create(ctypes.byref(self.p))
+        if not self.h:
+            raise ValueError("invalid oracle parameters")
         self.dim = self.p.dim
         self.ns = self.dim + 2
         self.n = self.p.poly_degree + 1
@@ -291,6 +297,15 @@ class Oracle:
         u = self.vec()
         lib().dgo_ic_vortex(self.h, strength, u_inf, v_inf, _dp(u))
         return u
+
+    def boundary_state(self, bc_type, u_int, normal):
+        """Euler::boundary_face_values: exterior state of a boundary face node."""
+        ui = np.ascontiguousarray(u_int, dtype=np.float64)
+        nrm = np.zeros(3)
+        nrm[:len(normal)] = normal
+        out = np.zeros(self.ns)
+        lib().dgo_boundary_state(self.h, bc_type, _dp(ui), _dp(nrm), _dp(out))
+        return out
 
     def residual(self, u):
         r = self.vec()

@@ -63,6 +63,7 @@ struct dgo_ctx {
     double *aux_rhs[3];
     /* physics constants (src/physics/euler.cpp:13-67, navier_stokes.cpp:16-55) */
     double gam, gamm1, mach2, pinf, temperature_ratio;
+    double vinf[3];               /* free-stream velocity (Euler constructor, euler.cpp:53-61) */
     int fsal;                     /* 3S*+ methods: evaluate the FSAL residual (reference: always) */
 };
 
@@ -401,6 +402,11 @@ static void build_mesh(dgo_ctx *c)
         for (int f = 0; f < 2 * d; ++f) {
             int ijk[3] = {c->cell_ijk[e * 3], c->cell_ijk[e * 3 + 1], c->cell_ijk[e * 3 + 2]};
             const int k = f / 2;
+            /* a face on the box boundary of a non-periodic direction: -(boundary type) */
+            if (c->P.boundary_type[f] && ijk[k] == (f % 2 ? N - 1 : 0)) {
+                c->nbr[e * 2 * d + f] = -c->P.boundary_type[f];
+                continue;
+            }
             ijk[k] = (ijk[k] + (f % 2 ? 1 : N - 1)) % N;
             const int lin = ijk[0] + N * (ijk[1] + (d == 3 ? N * ijk[2] : 0));
             c->nbr[e * 2 * d + f] = lookup[lin];
@@ -411,6 +417,134 @@ static void build_mesh(dgo_ctx *c)
 
 void dgo_cell_ijk(const dgo_ctx *c, int *ijk) { memcpy(ijk, c->cell_ijk, sizeof(int) * 3 * c->ncells); }
 void dgo_neighbors(const dgo_ctx *c, int *nbr) { memcpy(nbr, c->nbr, sizeof(int) * 2 * c->dim * c->ncells); }
+
+/* ------------------------------------------------------------------------ */
+/* boundary states: Euler::boundary_face_values (euler.cpp:1511-1567)        */
+/* ------------------------------------------------------------------------ */
+static void prim_of(const dgo_ctx *c, const double *u, double *w)
+{
+    /* convert_conservative_to_primitive: rho, velocity, pressure */
+    const int d = c->dim;
+    double v2 = 0.0;
+    w[0] = u[0];
+    for (int k = 0; k < d; ++k) {
+        w[1 + k] = u[1 + k] / u[0];
+        v2 += w[1 + k] * w[1 + k];
+    }
+    w[d + 1] = c->gamm1 * (u[d + 1] - 0.5 * u[0] * v2);
+}
+
+static void cons_of(const dgo_ctx *c, const double *w, double *u)
+{
+    /* convert_primitive_to_conservative */
+    const int d = c->dim;
+    double v2 = 0.0;
+    u[0] = w[0];
+    for (int k = 0; k < d; ++k) {
+        u[1 + k] = w[0] * w[1 + k];
+        v2 += w[1 + k] * w[1 + k];
+    }
+    u[d + 1] = w[d + 1] / c->gamm1 + 0.5 * w[0] * v2;
+}
+
+void dgo_boundary_state(const dgo_ctx *c, int type, const double *u_int, const double *n, double *u_bc)
+{
+    const int d = c->dim, ns = c->ns;
+    const double gam = c->gam, gamm1 = c->gamm1, pinf = c->pinf, minf2 = c->mach2;
+    /* total inlet pressure and temperature (euler.cpp:1518-1519); temperature_inf = 1 */
+    const double p0 = pinf * pow(1.0 + 0.5 * gamm1 * minf2, gam / gamm1);
+    const double T0 = 1.0 * pow(p0 / pinf, gamm1 / gam);
+    double wi[NS_MAX], wb[NS_MAX];
+    prim_of(c, u_int, wi);
+    if (type == 1001 || type == 1006) {
+        /* slip wall (euler.cpp:1171-1215; 1001 is the same for Euler, :1219-1228): the normal
+           velocity reflected, density and pressure copied */
+        double vn = 0.0;
+        for (int k = 0; k < d; ++k) vn += wi[1 + k] * (-n[k]);
+        wb[0] = wi[0];
+        for (int k = 0; k < d; ++k) wb[1 + k] = wi[1 + k] - 2.0 * vn * (-n[k]);
+        wb[d + 1] = wi[d + 1];
+        cons_of(c, wb, u_bc);
+    } else if (type == 1002) {
+        /* pressure outflow, back pressure 0.99 of the total inlet pressure (:1290-1325) */
+        double v2 = 0.0;
+        for (int k = 0; k < d; ++k) v2 += wi[1 + k] * wi[1 + k];
+        const double mach_int = sqrt(v2) / sqrt(gam * wi[d + 1] / wi[0]);
+        if (mach_int > 1.0) {
+            for (int s = 0; s < ns; ++s) u_bc[s] = u_int[s];
+            return;
+        }
+        const double p_inlet = p0 * pow(1.0 + 0.5 * gamm1 * minf2, -gam / gamm1);
+        const double p_bc = (mach_int >= 1.0) ? wi[d + 1] : 0.99 * p_inlet;
+        const double T_int = gam * minf2 * (wi[d + 1] / wi[0]);
+        wb[0] = gam * minf2 * (p_bc / T_int);
+        for (int k = 0; k < d; ++k) wb[1 + k] = wi[1 + k];
+        wb[d + 1] = p_bc;
+        cons_of(c, wb, u_bc);
+    } else if (type == 1003) {
+        /* inflow (:1327-1431, Carlson 2011 sec. 2.7 / 2.9) */
+        double vn = 0.0;
+        for (int k = 0; k < d; ++k) vn += wi[1 + k] * (-n[k]);
+        const double snd = sqrt(gam * wi[d + 1] / wi[0]);
+        if (minf2 < 1.0) {
+            const double rp = vn + 2.0 * snd / gamm1;
+            const double H = u_int[d + 1] / wi[0] + wi[d + 1] / wi[0];
+            const double A = 1.0 + 2.0 / gamm1, B = -2.0 * rp, C = 0.5 * gamm1 * (rp * rp - 2.0 * H);
+            const double disc = sqrt(B * B - 4.0 * A * C);
+            const double c1 = (-B + disc) / (2.0 * A), c2 = (-B - disc) / (2.0 * A);
+            const double cb = c1 > c2 ? c1 : c2;
+            const double vb = rp - 2.0 * cb / gamm1;
+            const double mb = vb / cb;
+            const double rad = 1.0 + 0.5 * gamm1 * mb * mb;
+            const double pb = p0 * pow(rad, -gam / gamm1), Tb = T0 / rad;
+            wb[0] = gam * minf2 * (pb / Tb);
+            for (int k = 0; k < d; ++k) wb[1 + k] = vb * (-n[k]);
+            wb[d + 1] = pb;
+        } else {
+            const double rad = 1.0 + 0.5 * gamm1 * minf2;
+            const double pb = p0 * pow(rad, -gam / gamm1), Tb = T0 / rad;
+            wb[0] = gam * minf2 * (pb / Tb);
+            const double vmag = sqrt(minf2) * sqrt(gam * pb / wb[0]);
+            for (int k = 0; k < d; ++k) wb[1 + k] = -vmag * n[k];
+            wb[d + 1] = pb;
+        }
+        cons_of(c, wb, u_bc);
+    } else if (type == 1004) {
+        /* Riemann-invariant farfield (:1097-1169) */
+        double we[NS_MAX];
+        we[0] = 1.0;
+        for (int k = 0; k < d; ++k) we[1 + k] = c->vinf[k];
+        we[d + 1] = pinf;
+        const double ci = sqrt(gam * wi[d + 1] / wi[0]), ce = sqrt(gam * we[d + 1] / we[0]);
+        double vni = 0.0, vne = 0.0;
+        for (int k = 0; k < d; ++k) {
+            vni += wi[1 + k] * n[k];
+            vne += we[1 + k] * n[k];
+        }
+        const double Rout = vni + 2.0 / gamm1 * ci, Rinc = vne - 2.0 / gamm1 * ce;
+        const double vnb = 0.5 * (Rout + Rinc), cb = 0.25 * gamm1 * (Rout - Rinc);
+        if (fabs(vnb) >= fabs(cb)) {
+            for (int s = 0; s < ns; ++s) wb[s] = (vnb < 0.0) ? we[s] : wi[s];
+        } else {
+            const double *w_src = (vnb < 0.0) ? we : wi;
+            const double vn_src = (vnb < 0.0) ? vne : vni;
+            const double ent = w_src[d + 1] * pow(w_src[0], -gam);  /* compute_entropy_measure */
+            wb[0] = pow(1.0 / gam * cb * cb / ent, 1.0 / gamm1);
+            for (int k = 0; k < d; ++k) wb[1 + k] = (w_src[1 + k] - vn_src * n[k]) + vnb * n[k];
+            wb[d + 1] = 1.0 / gam * cb * cb * wb[0];
+        }
+        cons_of(c, wb, u_bc);
+    } else if (type == 1005) {
+        /* free stream (:1433-1447) */
+        wb[0] = 1.0;
+        for (int k = 0; k < d; ++k) wb[1 + k] = c->vinf[k];
+        wb[d + 1] = 1.0 / (gam * minf2);
+        cons_of(c, wb, u_bc);
+    } else {
+        /* 1007: extrapolation (:1449-1460) */
+        for (int s = 0; s < ns; ++s) u_bc[s] = u_int[s];
+    }
+}
 
 /* owner rule current_cell_should_do_the_work (dg_base.cpp:417-449): lower index works */
 int dgo_n_faces(const dgo_ctx *c)
@@ -477,6 +611,8 @@ void dgo_default_params(dgo_params *p)
     p->domain_left = 0.0;
     p->domain_right = 6.28318530717958623200;
     p->cell_order = DGO_ORDER_DEALII;
+    for (int f = 0; f < 6; ++f) p->boundary_type[f] = 0;
+    p->angle_of_attack = 0.0;
 }
 
 dgo_ctx *dgo_create(const dgo_params *prm)
@@ -503,6 +639,17 @@ dgo_ctx *dgo_create(const dgo_params *prm)
     c->mach2 = prm->mach_inf * prm->mach_inf;
     c->pinf = 1.0 / (c->gam * c->mach2);
     c->temperature_ratio = 110.4 / prm->temperature_inf;
+    c->vinf[0] = cos(prm->angle_of_attack);
+    c->vinf[1] = sin(prm->angle_of_attack);
+    c->vinf[2] = 0.0;
+    /* boundaries: Euler only, both faces of a direction periodic or both boundaries, known types */
+    for (int f = 0; f < 2 * c->dim; ++f) {
+        const int t = prm->boundary_type[f], o = prm->boundary_type[f ^ 1];
+        if ((t == 0) != (o == 0) || (t && (t < 1001 || t > 1007)) || (t && prm->pde != DGO_EULER)) {
+            free(c);
+            return NULL;
+        }
+    }
 
     const int n = c->n, nq = c->nq;
     /* FE_DGQ(p): Lagrange on Gauss-Lobatto support points (dg_base.cpp:209-217) */
@@ -1159,20 +1306,25 @@ static double penalty_scaling(const dgo_ctx *c)
 }
 
 /* face term (strong_dg.cpp:2368-3576); rhs_ext is the global vector */
+/* Interior face (ce >= 0) or boundary face of type -ce (assemble_boundary_term_strong,
+   strong_dg.cpp:1583-2364: the interior side's terms only, with the exterior state of the
+   numerical flux from boundary_face_values of the interpolated interior trace). */
 static void face_term(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *u, double *lrhs, double *rhs_glob)
 {
     const int ns = c->ns, Np = c->Np, nq = c->nq, n = c->n, d = c->dim, nfp = c->nfp;
     const int split = c->P.use_split_form;
     const int ns_visc = (c->P.pde == DGO_NAVIER_STOKES);
     const double penalty = 0.5 * (penalty_scaling(c) + penalty_scaling(c));
+    const int boundary = ce < 0, nsides = boundary ? 1 : 2;
     const int cells[2] = {ci, ce}, faces[2] = {fi, fe};
-    double *svol[2], *ssurf[2], *avol[2], *asurf[2], *Fc_f[2], *Fd_f[2];
+    double *svol[2] = {NULL, NULL}, *ssurf[2] = {NULL, NULL}, *avol[2] = {NULL, NULL}, *asurf[2] = {NULL, NULL};
+    double *Fc_f[2] = {NULL, NULL}, *Fd_f[2] = {NULL, NULL};
     double *pev_vol[2] = {NULL, NULL}, *pev_surf[2] = {NULL, NULL};
     double *tmp = (double *)malloc(sizeof(double) * Np);
     double *tmp2 = (double *)malloc(sizeof(double) * Np);
     double C[3][3];
     metric_cofactor(c, C);
-    for (int side = 0; side < 2; ++side) {
+    for (int side = 0; side < nsides; ++side) {
         const int cell = cells[side], face = faces[side];
         svol[side] = (double *)malloc(sizeof(double) * ns * Np);
         ssurf[side] = (double *)malloc(sizeof(double) * ns * nfp);
@@ -1241,7 +1393,7 @@ static void face_term(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *
     /* surface-volume hybrid two-point terms (:3162-3350) */
     double *sv_surf[2] = {NULL, NULL}, *sv_vol[2] = {NULL, NULL};
     if (split) {
-        for (int side = 0; side < 2; ++side) {
+        for (int side = 0; side < nsides; ++side) {
             const int face = faces[side], dnz = face / 2;
             const double nref = (face % 2) ? 1.0 : -1.0;
             const unsigned *srow = c->had_srows[dnz], *scol = c->had_scols[dnz];
@@ -1302,14 +1454,6 @@ static void face_term(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *
         nref[dnz] = (fi % 2) ? 1.0 : -1.0;
         for (int f = 0; f < nfp; ++f) {
             double ui[NS_MAX], ue[NS_MAX];
-            if (split) {
-                double vi[NS_MAX], ve[NS_MAX];
-                for (int s = 0; s < ns; ++s) { vi[s] = pev_surf[0][s * nfp + f]; ve[s] = pev_surf[1][s * nfp + f]; }
-                dgo_conservative_from_entropy(c, vi, ui);
-                dgo_conservative_from_entropy(c, ve, ue);
-            } else {
-                for (int s = 0; s < ns; ++s) { ui[s] = ssurf[0][s * nfp + f]; ue[s] = ssurf[1][s * nfp + f]; }
-            }
             double nphys[3];
             for (int a = 0; a < d; ++a) {
                 nphys[a] = 0.0;
@@ -1319,6 +1463,19 @@ static void face_term(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *
             for (int a = 0; a < d; ++a) nrm += nphys[a] * nphys[a];
             nrm = sqrt(nrm);
             for (int a = 0; a < d; ++a) nphys[a] /= nrm;
+            if (boundary) {
+                /* the interpolated interior trace (use_projected_entropy_variables_for_nsfr_boundary_term
+                   defaults to false) and its boundary state */
+                for (int s = 0; s < ns; ++s) ui[s] = ssurf[0][s * nfp + f];
+                dgo_boundary_state(c, -ce, ui, nphys, ue);
+            } else if (split) {
+                double vi[NS_MAX], ve[NS_MAX];
+                for (int s = 0; s < ns; ++s) { vi[s] = pev_surf[0][s * nfp + f]; ve[s] = pev_surf[1][s * nfp + f]; }
+                dgo_conservative_from_entropy(c, vi, ui);
+                dgo_conservative_from_entropy(c, ve, ue);
+            } else {
+                for (int s = 0; s < ns; ++s) { ui[s] = ssurf[0][s * nfp + f]; ue[s] = ssurf[1][s * nfp + f]; }
+            }
             double fl[NS_MAX];
             dgo_numerical_flux(c, c->P.conv_flux, ui, ue, nphys, fl);
             for (int s = 0; s < ns; ++s) cnf[s * nfp + f] = nrm * fl[s];
@@ -1339,7 +1496,7 @@ static void face_term(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *
     }
     /* lift into interior and exterior (:3454-3575) */
     double *r = (double *)malloc(sizeof(double) * Np);
-    for (int side = 0; side < 2; ++side) {
+    for (int side = 0; side < nsides; ++side) {
         const int face = faces[side];
         const double sg = side == 0 ? -1.0 : 1.0;
         for (int s = 0; s < ns; ++s) {
@@ -1494,7 +1651,7 @@ void dgo_assemble_residual(dgo_ctx *c, const double *u, double *rhs)
         volume_term(c, e, u, lrhs);
         for (int f = 0; f < 2 * d; ++f) {
             const int nb = c->nbr[e * 2 * d + f];
-            if (e < nb) face_term(c, e, f, nb, f ^ 1, u, lrhs, rhs);
+            if (e < nb || nb < 0) face_term(c, e, f, nb, f ^ 1, u, lrhs, rhs);
         }
         for (int i = 0; i < ns * Np; ++i) {
 #ifdef _OPENMP
@@ -1580,11 +1737,16 @@ void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
                 cj[k] = side ? 0 : n - 1;
                 const int j = cj[0] + n * (cj[1] + n * cj[2]);
                 double ue[NS_MAX], ge[NS_MAX * 3], nrm[3] = {0, 0, 0}, fl[NS_MAX], F[NS_MAX * 3];
-                for (int s = 0; s < ns; ++s) {
-                    ue[s] = u[IDX(c, nb, s, j)];
-                    for (int kk = 0; kk < d; ++kk) ge[s * d + kk] = visc ? c->aux[kk][IDX(c, nb, s, j)] : 0.0;
-                }
                 nrm[k] = side ? 1.0 : -1.0;
+                if (nb < 0) {
+                    dgo_boundary_state(c, -nb, ui, nrm, ue);
+                    for (int s = 0; s < ns * 3; ++s) ge[s] = 0.0;
+                } else {
+                    for (int s = 0; s < ns; ++s) {
+                        ue[s] = u[IDX(c, nb, s, j)];
+                        for (int kk = 0; kk < d; ++kk) ge[s * d + kk] = visc ? c->aux[kk][IDX(c, nb, s, j)] : 0.0;
+                    }
+                }
                 dgo_numerical_flux(c, c->P.conv_flux, ui, ue, nrm, fl);
                 dgo_convective_flux(c, ui, F);
                 for (int s = 0; s < ns; ++s) acc[s] += (fabs(fl[s]) + fabs(F[s * d + k])) / c->wq[0];

@@ -12,6 +12,8 @@
  *   inverse mass             src/dg/dg_base.cpp:4089-4260
  *   1D operators / sum fact. src/operators/operators.cpp:294-649, :931-1196, :1217-2005
  *   Euler physics            src/physics/euler.cpp:177-937, :1033-1062
+ *   boundary faces (Euler)   src/physics/euler.cpp:1097-1567 (boundary_face_values),
+ *                            src/dg/strong_dg.cpp:1583-2364 (assemble_boundary_term_strong)
  *   Navier-Stokes viscous    src/physics/navier_stokes.cpp:240-385, :700-830, :1197-1350
  *   numerical fluxes         src/numerical_flux/convective_numerical_flux.cpp:34-480,
  *                            src/numerical_flux/viscous_numerical_flux.cpp:144-328
@@ -74,11 +76,20 @@ typedef struct {
     double domain_left;
     double domain_right;
     int cell_order;          /* DGO_ORDER_DEALII / DGO_ORDER_LEXICOGRAPHIC */
+    /* boundary condition of each face of the box (deal.II face numbering: 2k = -e_k, 2k+1 = +e_k):
+       0 = periodic (both faces of a direction), else a boundary type of
+       Euler::boundary_face_values (euler.cpp:1511-1567): 1001 wall, 1002 pressure outflow,
+       1003 inflow, 1004 Riemann farfield, 1005 farfield, 1006 slip wall, 1007 extrapolation */
+    int boundary_type[6];
+    double angle_of_attack;  /* radians; free stream (cos a, sin a[, 0]) */
 } dgo_params;
 
 typedef struct dgo_ctx dgo_ctx;
 
 void dgo_default_params(dgo_params *p);
+/* Euler::boundary_face_values (euler.cpp:1511-1567) for the boundary types above: the exterior
+   state of a boundary face node from the interior state and the outward unit normal. */
+void dgo_boundary_state(const dgo_ctx *c, int type, const double *u_int, const double *normal, double *u_bc);
 dgo_ctx *dgo_create(const dgo_params *p);
 void dgo_destroy(dgo_ctx *c);
 

@@ -53,6 +53,7 @@ class Params(ctypes.Structure):
         ("temperature_inf", ctypes.c_double), ("sipg_penalty_factor", ctypes.c_double),
         ("n_cells_1d", ctypes.c_int), ("domain_left", ctypes.c_double), ("domain_right", ctypes.c_double),
         ("cell_order", ctypes.c_int), ("precision", ctypes.c_int),
+        ("boundary_type", ctypes.c_int * 6), ("angle_of_attack", ctypes.c_double),
     ]
 
 
@@ -178,6 +179,8 @@ def params(**kw):
     p = Params()
     load().dgx_default_params(ctypes.byref(p))
     for k, v in kw.items():
+        if k == "boundary_type":
+            v = (ctypes.c_int * 6)(*v)
         setattr(p, k, v)
     return p
 
@@ -413,6 +416,7 @@ class DG:
         solution in the reference DoF order, time, iteration, the parameters, the table so far."""
         import json
         params = {f: getattr(self.p, f) for f, _ in Params._fields_}
+        params = {k: (list(v) if isinstance(v, ctypes.Array) else v) for k, v in params.items()}
         cols = sorted({k for r in (table or []) for k in r})
         rows = np.array([[r.get(c, np.nan) for c in cols] for r in (table or [])], dtype=np.float64)
         np.savez(path, solution=self.solution(), time=np.float64(self.current_time),
@@ -425,9 +429,11 @@ class DG:
         import json
         with np.load(path, allow_pickle=False) as z:
             saved = json.loads(str(z["params"]))
-            for k in ("dim", "poly_degree", "n_cells_1d", "domain_left", "domain_right"):
-                if saved[k] != getattr(self.p, k):
-                    raise DeviceError(f"restart data is for {k}={saved[k]}, this solver has {getattr(self.p, k)}")
+            for k in ("dim", "poly_degree", "n_cells_1d", "domain_left", "domain_right", "boundary_type"):
+                mine = getattr(self.p, k)
+                mine = list(mine) if isinstance(mine, ctypes.Array) else mine
+                if saved.get(k, mine) != mine:
+                    raise DeviceError(f"restart data is for {k}={saved[k]}, this solver has {mine}")
             self.set_solution(z["solution"])
             self.set_time(float(z["time"]), int(z["iteration"]))
             cols = [str(c) for c in z["table_columns"]]

@@ -261,6 +261,8 @@ void dgx_default_params(dgx_params *p)
     p->domain_right = 6.28318530717958623200;
     p->cell_order = DGX_ORDER_DEALII;
     p->precision = DGX_FP64;
+    for (int f = 0; f < 6; ++f) p->boundary_type[f] = 0;
+    p->angle_of_attack = 0.0;
 }
 
 // DGFactory::create_discontinuous_galerkin + allocate_system: operators, device buffers (from
@@ -282,6 +284,12 @@ static int create_ctx(const dgx_params *p, dgx::Arena *arena, hipStream_t stream
         return bad(DGX_ERR_INVALID, "two point flux and not using split form are not compatible");
     if (p->pde == DGX_NAVIER_STOKES && p->dim != 3) return bad(DGX_ERR_UNSUPPORTED, "Navier-Stokes device path is 3D");
     if (p->precision != DGX_FP64 && p->precision != DGX_MIXED) return bad(DGX_ERR_INVALID, "unknown precision");
+    for (int f = 0; f < 2 * p->dim; ++f) {
+        const int t = p->boundary_type[f], o = p->boundary_type[f ^ 1];
+        if ((t == 0) != (o == 0)) return bad(DGX_ERR_INVALID, "both faces of a direction periodic or both boundaries");
+        if (t && (t < 1001 || t > 1007)) return bad(DGX_ERR_UNSUPPORTED, "boundary types 1001..1007 only");
+        if (t && p->pde != DGX_EULER) return bad(DGX_ERR_UNSUPPORTED, "boundaries on the Euler path only");
+    }
     dgx_ctx *c = new dgx_ctx();
     c->P = *p;
     c->arena = arena;
@@ -313,6 +321,10 @@ static int create_ctx(const dgx_params *p, dgx::Arena *arena, hipStream_t stream
     pc.two_point = p->two_point_flux;
     pc.conv_flux = p->conv_flux;
     pc.diss_flux = p->diss_flux;
+    pc.vinf[0] = std::cos(p->angle_of_attack);
+    pc.vinf[1] = std::sin(p->angle_of_attack);
+    pc.vinf[2] = 0.0;
+    for (int f = 0; f < 2 * c->dim; ++f) c->has_boundary = c->has_boundary || p->boundary_type[f] != 0;
     c->nbr_count = (int64_t)c->ncells * 2 * c->dim;
     const size_t bytes = sizeof(double) * (size_t)c->ndofs;
     c->nbr_dev = static_cast<int *>(dev_alloc(c, sizeof(int) * (size_t)c->nbr_count));
@@ -1218,7 +1230,7 @@ static void batch_free_groups(dgx_batch *b)
 
 static bool batch_line_eligible(const dgx_ctx *c)
 {
-    return c->P.use_split_form && c->P.pde == DGX_EULER;
+    return c->P.use_split_form && c->P.pde == DGX_EULER && !c->has_boundary;
 }
 
 // block table of a group for `epb` elements per block: (slot, first element) per element group

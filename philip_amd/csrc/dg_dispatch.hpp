@@ -29,6 +29,7 @@ void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
 {
     const bool split = c->P.use_split_form != 0;
     const bool nsv = c->P.pde == DGX_NAVIER_STOKES;
+    const bool bnd = c->has_boundary;  // Euler only
     using LG = LineGeoS<D, N, T>;
     const dim3 lgrid((c->ncells + LG::EPB - 1) / LG::EPB), lblock(LG::BLOCK);
     if constexpr (D == 3) {
@@ -50,18 +51,37 @@ void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
         else if (!split && nsv)
             hipLaunchKernelGGL((stage2_kernel<D, N, false, true, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
         else if (split) {
-            if constexpr (line_waves<D, N, T>() > 0)
-                hipLaunchKernelGGL((line_kernel_wb<D, N, T, line_waves<D, N, T>()>), lgrid, lblock, 0, c->stream, a,
-                                   c->ops, c->pc);
+            if constexpr (line_waves<D, N, T>() > 0) {
+                if (bnd)
+                    hipLaunchKernelGGL((line_kernel_wb<D, N, T, line_waves<D, N, T>(), true>), lgrid, lblock, 0,
+                                       c->stream, a, c->ops, c->pc);
+                else
+                    hipLaunchKernelGGL((line_kernel_wb<D, N, T, line_waves<D, N, T>()>), lgrid, lblock, 0,
+                                       c->stream, a, c->ops, c->pc);
+            } else {
+                if (bnd)
+                    hipLaunchKernelGGL((line_kernel<D, N, T, true>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
+                else
+                    hipLaunchKernelGGL((line_kernel<D, N, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
+            }
+        } else if (bnd) {
+            hipLaunchKernelGGL((stage2_kernel<D, N, false, false, T, true>), grid, block, 0, c->stream, a, c->ops,
+                               c->pc);
+        } else {
+            hipLaunchKernelGGL((stage2_kernel<D, N, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
+        }
+    } else {
+        if (split) {
+            if (bnd)
+                hipLaunchKernelGGL((line_kernel<D, N, T, true>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
             else
                 hipLaunchKernelGGL((line_kernel<D, N, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
-        } else
+        } else if (bnd) {
+            hipLaunchKernelGGL((stage2_kernel<D, N, false, false, T, true>), grid, block, 0, c->stream, a, c->ops,
+                               c->pc);
+        } else {
             hipLaunchKernelGGL((stage2_kernel<D, N, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
-    } else {
-        if (split)
-            hipLaunchKernelGGL((line_kernel<D, N, T>), lgrid, lblock, 0, c->stream, a, c->ops, c->pc);
-        else
-            hipLaunchKernelGGL((stage2_kernel<D, N, false, false, T>), grid, block, 0, c->stream, a, c->ops, c->pc);
+        }
     }
 }
 

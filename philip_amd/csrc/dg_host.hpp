@@ -63,6 +63,7 @@ struct dgx_ctx {
     dgx::Ops1D ops{};
     dgx::PhysConst pc{};
     int64_t nbr_count = 0;         // ncells * 2 * dim
+    bool has_boundary = false;     // some box face is not periodic (nbr < 0 there: -boundary type)
     int *nbr_dev = nullptr;
     double *u_dev = nullptr;       // DGBase::solution
     double *r_dev = nullptr;       // DGBase::right_hand_side

@@ -216,6 +216,7 @@ int launch_lsrk_error(dgx_ctx *c, const double *s1, const double *e, const doubl
 // ---------------------------------------------------------------------------
 struct MeshDesc {
     int dim, N, morton, levels;
+    int bc[6];  // boundary type of each box face (0: periodic)
 };
 
 // lattice coordinates of cell e: Z order (refine_global, children x-fastest, coarsest level in
@@ -252,6 +253,10 @@ __global__ void __launch_bounds__(256) mesh_kernel(int *__restrict__ nbr, int nc
     for (int f = 0; f < 2 * m.dim; ++f) {
         int q[3] = {c[0], c[1], c[2]};
         const int k = f / 2;
+        if (m.bc[f] && q[k] == ((f & 1) ? m.N - 1 : 0)) {  // box boundary: -(boundary type)
+            nbr[(long)e * 2 * m.dim + f] = -m.bc[f];
+            continue;
+        }
         q[k] = (q[k] + ((f & 1) ? 1 : m.N - 1)) % m.N;
         nbr[(long)e * 2 * m.dim + f] = cell_index(m, q);
     }
@@ -315,6 +320,7 @@ static MeshDesc mesh_desc(const dgx_ctx *c)
     m.morton = (c->P.cell_order == DGX_ORDER_DEALII) && m.N > 0 && (m.N & (m.N - 1)) == 0;
     m.levels = 0;
     while ((1 << m.levels) < m.N) ++m.levels;
+    for (int f = 0; f < 6; ++f) m.bc[f] = (f < 2 * c->dim) ? c->P.boundary_type[f] : 0;
     return m;
 }
 

@@ -304,7 +304,7 @@ template <typename T> struct FaceIn {
 // the same canonical node, so a face evaluated in two blocks (once from each
 // side) sees the same bits on both and the flux is the same whichever block
 // holds the pair.
-template <typename T, int D, int N, int K, bool V>
+template <typename T, int D, int N, int K, bool V, bool BND = false>
 __device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, const FastConst<T> &fc,
                                        const FaceIn<T> &in, int le, int side, int fn, int e, int nb, int nbl,
                                        T pen, T (&out)[D + 2], T (&out_nb)[D + 2])
@@ -315,9 +315,25 @@ __device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, co
     const int fj = line_node<D, N>(K, fn, side ? 0 : N - 1);
     NodeC<T, D> qi, qn, L, R;
     nodef_load<T, D>(in.sq + le * LG::ESQ + qpos<D, N>(fi), NP, qi);
-    if (nbl >= 0) nodef_load<T, D>(in.sq + nbl * LG::ESQ + qpos<D, N>(fj), NP, qn);
-    else trace_node<T, D>(in.tr, in.tst, fc, qn);
-    const bool own = e < nb;
+    bool own = e < nb;
+    if constexpr (BND) {
+        if (nb < 0) {
+            // boundary face of type -nb: the exterior state from the interior node
+            T ub[NS];
+            boundary_state<T, D>(-nb, qi.u, K, side ? T(1) : T(-1), pc, ub);
+            for (int s = 0; s < NS; ++s) qn.u[s] = ub[s];
+            node_fill<T, D>(qn, fc);
+            node_canon<T, D>(qn);
+            own = true;
+        } else if (nbl >= 0) {
+            nodef_load<T, D>(in.sq + nbl * LG::ESQ + qpos<D, N>(fj), NP, qn);
+        } else {
+            trace_node<T, D>(in.tr, in.tst, fc, qn);
+        }
+    } else {
+        if (nbl >= 0) nodef_load<T, D>(in.sq + nbl * LG::ESQ + qpos<D, N>(fj), NP, qn);
+        else trace_node<T, D>(in.tr, in.tst, fc, qn);
+    }
     nodef_select<T, D>(own, qi, qn, L);
     nodef_select<T, D>(own, qn, qi, R);
     const T sgn = side ? T(1) : T(-1);
@@ -351,7 +367,7 @@ __device__ __forceinline__ void face_f(const Ops1D &ops, const PhysConst &pc, co
 // One block's work: elements blk0 .. blk0 + EPB - 1 of the solve described by a.
 // V: Navier-Stokes (split convective form plus the viscous volume and surface
 // terms, with the lifted gradient a.aux computed beforehand).
-template <int D, int N, typename T, int ES, bool V = false, bool XJ = false>
+template <int D, int N, typename T, int ES, bool V = false, bool XJ = false, bool BND = false>
 __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops, const PhysConst &pc, const int blk0)
 {
     using LG = LineGeo<D, N, T, ES, V, XJ>;
@@ -415,7 +431,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
                 const int f = r / NL, fn = r % NL;
                 const int nb = nbv[j];
                 if (fn == 0) sNb[fle][f] = nb;
-                if (!in_block(nb, fe)) {
+                if (nb >= 0 && !in_block(nb, fe)) {
                     const int fj = line_node<D, N>(f >> 1, fn, (f & 1) ? 0 : N - 1);
                     for (int s = 0; s < NS; ++s) un[j][s] = a.u[((long)nb * NS + s) * NP + fj];
                     fdst[j] = t;  // = fle * NFN + r
@@ -541,7 +557,7 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
             T rf[NS], rfn[NS];
             auto fitem = [&](auto kc) {
                 constexpr int K = decltype(kc)::value;
-                face_f<T, D, N, K, V>(ops, pc, fc, in, fle, side, fn, blk0 + fle, nb, nbl, pen, rf, rfn);
+                face_f<T, D, N, K, V, BND>(ops, pc, fc, in, fle, side, fn, blk0 + fle, nb, nbl, pen, rf, rfn);
             };
             if (k == 0) fitem(std::integral_constant<int, 0>{});
             else if (k == 1) fitem(std::integral_constant<int, 1>{});
@@ -558,6 +574,14 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
     __syncthreads();  // node cache and traces consumed; their storage now takes the partials
     constexpr bool LATE_PF = !PREFETCH && !FAST && line_late_prefetch_d<D, N>();
     const bool late_pf = LATE_PF && active && a.mode == MODE_LSRK && !a.lsrk_first;
+    // FP64 at N >= 4: the stage state is read again here for the epilogue (an L2 hit: the block
+    // read it in phase A) instead of being held in registers through the line phase (TGV 32^3
+    // p=3: 177 -> 170 us per stage, 64 -> 16 B/lane of spills; p=4 24^3 294 -> 244 us; at p=2
+    // slower)
+    constexpr bool REREAD = LATE_PF && N >= 4;
+    double ue[NS];
+    if (REREAD && active && a.mode == MODE_LSRK)
+        for (int s = 0; s < NS; ++s) ue[s] = a.u[((long)e * NS + s) * NP + i];
     if (late_pf)
         for (int s = 0; s < NS; ++s) {
             const long idx = ((long)e * NS + s) * NP + i;
@@ -615,7 +639,8 @@ __device__ __forceinline__ void line_block(const StageArgs &a, const Ops1D &ops,
         const T *w = fb + f * NL + node_line<D, N>(k, i);
         for (int s = 0; s < NS; ++s) acc[s] += w[s * LG::NFN];
     }
-    stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc, pf || late_pf, pf2, pf3);
+    if constexpr (REREAD) stage_epilogue<D, N, T>(a, ops, e, i, c, ue, acc, late_pf, pf2, pf3);
+    else stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc, pf || late_pf, pf2, pf3);
 }
 
 // Elements-per-block variant (ES of LineGeo) of the single-solve Euler line kernel: the mixed path
@@ -628,12 +653,14 @@ template <int D, int N, typename T> constexpr int line_es_single()
 }
 template <int D, int N, typename T> using LineGeoS = LineGeo<D, N, T, line_es_single<D, N, T>()>;
 
-// One solve: blocks in XCD-aware order over the element groups.
-template <int D, int N, typename T>
+// One solve: blocks in XCD-aware order over the element groups.  BND: the mesh has boundary
+// faces (a separate instantiation, so the periodic kernels carry no boundary code).
+template <int D, int N, typename T, bool BND = false>
 __global__ void __launch_bounds__((LineGeoS<D, N, T>::BLOCK))
 line_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
-    line_block<D, N, T, line_es_single<D, N, T>(), false>(a, ops, pc, xcd_block() * LineGeoS<D, N, T>::EPB);
+    line_block<D, N, T, line_es_single<D, N, T>(), false, false, BND>(a, ops, pc,
+                                                                       xcd_block() * LineGeoS<D, N, T>::EPB);
 }
 // Waves per SIMD the single-solve 3D line kernel is held to (amdgpu_waves_per_eu; 0: the
 // compiler's choice).  The mixed path keeps the compiler's choice (a 4-wave bound at p=6 was
@@ -647,11 +674,12 @@ template <int D, int N, typename T> constexpr int line_waves()
     if constexpr (D == 3 && N >= 4 && N <= 6 && std::is_same<T, double>::value) return 3;
     return 0;
 }
-template <int D, int N, typename T, int W>
+template <int D, int N, typename T, int W, bool BND = false>
 __global__ void __launch_bounds__((LineGeoS<D, N, T>::BLOCK)) __attribute__((amdgpu_waves_per_eu(W)))
 line_kernel_wb(StageArgs a, Ops1D ops, PhysConst pc)
 {
-    line_block<D, N, T, line_es_single<D, N, T>(), false>(a, ops, pc, xcd_block() * LineGeoS<D, N, T>::EPB);
+    line_block<D, N, T, line_es_single<D, N, T>(), false, false, BND>(a, ops, pc,
+                                                                       xcd_block() * LineGeoS<D, N, T>::EPB);
 }
 
 // Navier-Stokes, mixed path.  line_kernel_ns_w4 is held to 4 waves/SIMD (128 VGPRs; a few

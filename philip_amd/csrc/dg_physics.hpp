@@ -14,6 +14,7 @@ namespace dgx {
 
 struct PhysConst {
     double gam, gamm1, mach2, reynolds, prandtl, mu_const, temperature_ratio;
+    double vinf[3];  // free-stream velocity (cos a, sin a, 0): farfield, Riemann and inflow states
     int use_constant_viscosity;
     int two_point;   // KG=0 IR=1 CH=2 RA=3
     int conv_flux;   // LXF=0 ROE=1 CENTRAL=2 TWO_POINT=3 TP+LXF=4 TP+ROE=5
@@ -93,6 +94,122 @@ __device__ __forceinline__ T log_mean(T a, T b)
 
 // Primitive cache used by the pair loops: rho, velocity, pressure.
 template <typename T, int D> struct Prim { T rho, vel[D], p; };
+
+// Euler::boundary_face_values (euler.cpp:1511-1567) on a Cartesian face whose outward normal is
+// sg e_K: the exterior state ub of a boundary face node from the interior state ui.  Types:
+// 1001 / 1006 slip wall (normal velocity reflected, :1171-1228), 1002 pressure outflow at 0.99 of
+// the inlet static pressure (:1290-1325), 1003 inflow at the free-stream total pressure and
+// temperature (:1327-1431), 1004 Riemann-invariant farfield (:1097-1169), 1005 free stream
+// (:1433-1447), 1007 extrapolation (:1449-1460).  Free stream: rho = 1, p = 1/(gamma M^2), T = 1.
+// Every branch sets the exterior primitive state (rho_b, vel_b, p_b) as scalars chosen by value,
+// with the face direction picked by (k == K) in unrolled loops: a form that selected between two
+// private arrays through a pointer and stored through a computed index (wb[1 + K]) compiled at
+// -O3 for gfx950 to a slip-wall state carrying the free-stream velocity component instead of the
+// reflected one (right at -O1 and on the host).
+template <typename T, int D>
+__device__ __forceinline__ void boundary_state(int type, const T *ui, int K, T sg, const PhysConst &pc, T *ub)
+{
+    constexpr int NS = D + 2;
+    const T gam = T(pc.gam), gamm1 = T(pc.gamm1), m2 = T(pc.mach2);
+    const T pinf = T(1) / (gam * m2);
+    const T rho_i = ui[0];
+    T vel_i[D], v2 = T(0), vn_i = T(0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        vel_i[k] = ui[1 + k] / rho_i;
+        v2 += vel_i[k] * vel_i[k];
+        if (k == K) vn_i = vel_i[k] * sg;  // velocity along the outward normal
+    }
+    const T p_i = gamm1 * (ui[D + 1] - T(0.5) * rho_i * v2);
+    T rho_b = rho_i, p_b = p_i, vel_b[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) vel_b[k] = vel_i[k];
+    bool copy = false;  // exterior = interior conservative state
+    if (type == 1001 || type == 1006) {
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            if (k == K) vel_b[k] = -vel_i[k];  // v - 2 (v.n) n with n = +-e_K
+    } else if (type == 1002) {
+        const T mach = dsqrt<T>(v2) / dsqrt<T>(gam * p_i / rho_i);
+        if (mach > T(1)) {
+            copy = true;
+        } else {
+            // the inlet static pressure from the total inlet pressure (the free stream's)
+            const T rad = T(1) + T(0.5) * gamm1 * m2;
+            const T p_inlet = (pinf * dpow<T>(rad, gam / gamm1)) * dpow<T>(rad, -gam / gamm1);
+            p_b = (mach >= T(1)) ? p_i : T(0.99) * p_inlet;
+            const T Ti = gam * m2 * (p_i / rho_i);
+            rho_b = gam * m2 * (p_b / Ti);
+        }
+    } else if (type == 1003) {
+        const T p0 = pinf * dpow<T>(T(1) + T(0.5) * gamm1 * m2, gam / gamm1);
+        const T T0 = dpow<T>(p0 / pinf, gamm1 / gam);
+        T speed;  // along the inward normal
+        if (m2 < T(1)) {
+            const T snd = dsqrt<T>(gam * p_i / rho_i);
+            const T rp = -vn_i + T(2) * snd / gamm1;
+            const T H = ui[D + 1] / rho_i + p_i / rho_i;
+            const T A = T(1) + T(2) / gamm1, B = T(-2) * rp, C = T(0.5) * gamm1 * (rp * rp - T(2) * H);
+            const T disc = dsqrt<T>(B * B - T(4) * A * C);
+            const T c1 = (-B + disc) / (T(2) * A), c2 = (-B - disc) / (T(2) * A);
+            const T cb = c1 > c2 ? c1 : c2;
+            const T vb = rp - T(2) * cb / gamm1;
+            const T mb = vb / cb;
+            const T rad = T(1) + T(0.5) * gamm1 * mb * mb;
+            p_b = p0 * dpow<T>(rad, -gam / gamm1);
+            rho_b = gam * m2 * (p_b / (T0 / rad));
+            speed = vb;
+        } else {
+            const T rad = T(1) + T(0.5) * gamm1 * m2;
+            p_b = p0 * dpow<T>(rad, -gam / gamm1);
+            rho_b = gam * m2 * (p_b / (T0 / rad));
+            speed = dsqrt<T>(m2) * dsqrt<T>(gam * p_b / rho_b);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) vel_b[k] = (k == K) ? -speed * sg : T(0);
+    } else if (type == 1004) {
+        T vn_e = T(0);
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            if (k == K) vn_e = T(pc.vinf[k]) * sg;
+        const T ci = dsqrt<T>(gam * p_i / rho_i), ce = dsqrt<T>(gam * pinf);
+        const T Rout = vn_i + T(2) / gamm1 * ci, Rinc = vn_e - T(2) / gamm1 * ce;
+        const T vnb = T(0.5) * (Rout + Rinc), cb = T(0.25) * gamm1 * (Rout - Rinc);
+        const bool inlet = vnb < T(0);
+        // the upwind side's state, by value
+        const T rho_s = inlet ? T(1) : rho_i, p_s = inlet ? pinf : p_i, vn_s = inlet ? vn_e : vn_i;
+        T vel_s[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) vel_s[k] = inlet ? T(pc.vinf[k]) : vel_i[k];
+        if (dabs<T>(vnb) >= dabs<T>(cb)) {
+            rho_b = rho_s;
+            p_b = p_s;
+#pragma unroll
+            for (int k = 0; k < D; ++k) vel_b[k] = vel_s[k];
+        } else {
+            const T ent = p_s * dpow<T>(rho_s, -gam);
+            rho_b = dpow<T>(T(1) / gam * cb * cb / ent, T(1) / gamm1);
+#pragma unroll
+            for (int k = 0; k < D; ++k) vel_b[k] = (k == K) ? (vel_s[k] - vn_s * sg) + vnb * sg : vel_s[k];
+            p_b = T(1) / gam * cb * cb * rho_b;
+        }
+    } else if (type == 1005) {
+        rho_b = T(1);
+        p_b = pinf;
+#pragma unroll
+        for (int k = 0; k < D; ++k) vel_b[k] = T(pc.vinf[k]);
+    } else {
+        copy = true;
+    }
+    T w2 = T(0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) w2 += vel_b[k] * vel_b[k];
+    ub[0] = copy ? ui[0] : rho_b;
+#pragma unroll
+    for (int k = 0; k < D; ++k) ub[1 + k] = copy ? ui[1 + k] : rho_b * vel_b[k];
+    ub[D + 1] = copy ? ui[D + 1] : p_b / gamm1 + T(0.5) * rho_b * w2;
+    (void)NS;
+}
 
 template <typename T, int D>
 __device__ __forceinline__ Prim<T, D> to_prim(const T *u, T gamm1)

@@ -78,7 +78,7 @@ __device__ __forceinline__ int pair_index(int a, int b)
     return a * (2 * N - a - 1) / 2 + (b - a - 1);
 }
 
-template <int D, int N, bool SPLIT, bool NSV, typename T>
+template <int D, int N, bool SPLIT, bool NSV, typename T, bool BND = false>
 __global__ void __launch_bounds__((Geo<D, N>::BLOCK))
 stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
@@ -163,16 +163,23 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
             const int fj = line_node<D, N>(K, fn, side ? 0 : N - 1);
             const T sgn = side ? T(1) : T(-1);
             T us[NS], un[NS], unt[NS];
-            double und[NS];
-            for (int s = 0; s < NS; ++s) {
-                us[s] = sU[fle][s][fi];
-                und[s] = a.u[((long)nb * NS + s) * NP + fj];
-                un[s] = T(und[s]);
+            for (int s = 0; s < NS; ++s) us[s] = sU[fle][s][fi];
+            bool own = fe < nb;
+            if constexpr (BND) {
+                if (nb >= 0) {
+                    for (int s = 0; s < NS; ++s) un[s] = T(a.u[((long)nb * NS + s) * NP + fj]);
+                } else {
+                    // boundary face of type -nb (Euler): the exterior state from the interior trace
+                    boundary_state<T, D>(-nb, us, K, sgn, pc, un);
+                    own = true;
+                }
+            } else {
+                for (int s = 0; s < NS; ++s) un[s] = T(a.u[((long)nb * NS + s) * NP + fj]);
             }
             for (int s = 0; s < NS; ++s) unt[s] = un[s];
-            // reference orientation: the lower cell index is the interior side; selected by value so
-            // the flux evaluation itself is not under a divergent branch
-            const bool own = fe < nb;
+            // reference orientation: the lower cell index is the interior side (a boundary face's
+            // interior is its cell); selected by value so the flux evaluation itself is not under a
+            // divergent branch
             T ua[NS], ub[NS];
             for (int s = 0; s < NS; ++s) {
                 ua[s] = own ? us[s] : unt[s];

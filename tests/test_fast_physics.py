@@ -68,3 +68,42 @@ def test_cached_node_fluxes_match_oracle(host_exe, tp, cf, mach, spread, prec, b
             worst = max(worst, np.abs(got[i, k, 0] - F[:, k]).max() / scale_f,
                         np.abs(got[i, k, 1] - G).max() / scale_g)
     assert worst < bar, worst
+
+
+@pytest.fixture(scope="module")
+def boundary_exe(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("bnd") / "boundary_state_host")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(HERE, "host", "boundary_state_host.cpp")],
+                   check=True)
+    return exe
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("mach", [0.5, 1.6])
+@pytest.mark.parametrize("prec,bar", [(0, 2e-5), (1, 1e-14)], ids=["float", "double"])
+def test_device_boundary_states_match_oracle(boundary_exe, dim, mach, prec, bar):
+    """The device's boundary states (every type, every face direction and orientation) against the
+    oracle's, which tests/test_oracle_boundary.py pins by what each condition is defined to do."""
+    rng = np.random.default_rng(dim + int(10 * mach))
+    aoa = 0.3
+    o = O.Oracle(dim=dim, poly_degree=1, n_cells_1d=2, mach_inf=mach, angle_of_attack=aoa)
+    pinf = 1.0 / (1.4 * mach * mach)
+    rows, expect = [], []
+    for _ in range(20):
+        rho = rng.uniform(0.7, 1.3)
+        v = rng.uniform(-2.0, 2.0, dim) * (1.0 if mach < 1 else 0.8)
+        p = pinf * rng.uniform(0.7, 1.3)
+        u = np.r_[rho, rho * v, p / 0.4 + 0.5 * rho * v @ v]
+        for bc in range(1001, 1008):
+            for k in range(dim):
+                for sg in (-1.0, 1.0):
+                    nrm = np.zeros(dim)
+                    nrm[k] = sg
+                    rows.append(f"{bc} {k} {sg} " + " ".join(f"{x:.17g}" for x in u))
+                    expect.append(o.boundary_state(bc, u, nrm))
+    inp = f"{len(rows)} {dim} {mach} {aoa} {prec}\n" + "\n".join(rows)
+    out = subprocess.run([boundary_exe], input=inp, capture_output=True, text=True, check=True).stdout
+    got = np.array([[float(x) for x in line.split()] for line in out.strip().splitlines()])
+    expect = np.array(expect)
+    err = (np.abs(got - expect) / np.abs(expect).max(axis=1, keepdims=True)).max()
+    assert err < bar, err

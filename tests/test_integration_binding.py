@@ -22,7 +22,7 @@ template <typename T> struct Vector {
 };
 }}}
 namespace PHiLiP { namespace Parameters {
-struct EulerParam { double mach_inf = 0.1, gamma_gas = 1.4; };
+struct EulerParam { double mach_inf = 0.1, gamma_gas = 1.4, angle_of_attack = 0.0; };
 struct NavierStokesParam {
     double prandtl_number = 0.72, reynolds_number_inf = 1600.0, temperature_inf = 273.15;
     bool use_constant_viscosity = false;
@@ -58,6 +58,8 @@ int main(int argc, char **) {
     if (argc < 100) return 0;  // linked, never run here
     PHiLiP::Parameters::AllParameters prm;
     PHiLiP::DGDevice<3, 1, double, MeshType> dg(prm, 3, 4, 0.0, 6.283185307179586);
+    const int channel[6] = {1003, 1002, 1006, 1006, 0, 0};  // inflow, outflow, slip walls, periodic z
+    PHiLiP::DGDevice<3, 1, double, MeshType> dg_channel(prm, 3, 4, 0.0, 1.0, channel);
     dealii::LinearAlgebra::distributed::Vector<double> u, r;
     dg.assemble_residual(u, r);
     dg.apply_inverse_global_mass_matrix(r);
