@@ -36,7 +36,7 @@ class Params(ctypes.Structure):
         ("temperature_inf", ctypes.c_double), ("sipg_penalty_factor", ctypes.c_double),
         ("n_cells_1d", ctypes.c_int), ("domain_left", ctypes.c_double), ("domain_right", ctypes.c_double),
         ("cell_order", ctypes.c_int),
-        ("boundary_type", ctypes.c_int * 6), ("angle_of_attack", ctypes.c_double),
+        ("boundary_type", ctypes.c_int * 6), ("angle_of_attack", ctypes.c_double), ("grid_warp", ctypes.c_int),
     ]
 
 
@@ -73,6 +73,9 @@ def _load():
     lib.dgo_neighbors.argtypes = [vp, P(i)]
     lib.dgo_face_list.argtypes = [vp, P(i)]
     lib.dgo_node_coords.argtypes = [vp, dp]
+    lib.dgo_set_grid_nodes.argtypes = [vp, dp]
+    lib.dgo_set_grid_nodes.restype = i
+    lib.dgo_metric_terms.argtypes = [vp, dp, dp]
     lib.dgo_pressure.restype = d
     lib.dgo_pressure.argtypes = [vp, dp]
     lib.dgo_convective_flux.argtypes = [vp, dp, dp]
@@ -298,6 +301,19 @@ class Oracle:
         lib().dgo_ic_vortex(self.h, strength, u_inf, v_inf, _dp(u))
         return u
 
+    def set_grid_nodes(self, xyz):
+        """Curvilinear grid from node coordinates [cell][node][3] (GLL, Euler)."""
+        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.ncells, self.Np, 3)
+        if lib().dgo_set_grid_nodes(self.h, _dp(x)) != 0:
+            raise ValueError("curvilinear grids need GLL flux nodes and the Euler equations")
+
+    def metric_terms(self):
+        """(C [cell][node][3][3] with C[m][n] = J dxi_n/dx_m, det J [cell][node])."""
+        C = np.zeros((self.ncells, self.Np, 3, 3))
+        J = np.zeros((self.ncells, self.Np))
+        lib().dgo_metric_terms(self.h, _dp(C), _dp(J))
+        return C, J
+
     def boundary_state(self, bc_type, u_int, normal):
         """Euler::boundary_face_values: exterior state of a boundary face node."""
         ui = np.ascontiguousarray(u_int, dtype=np.float64)
@@ -413,7 +429,8 @@ def set_threads(n):
     lib().dgo_set_threads(int(n))
 
 
-MUT_NONE, MUT_LXF_MIN, MUT_LXF_NO_HALF, MUT_ROE_NO_HALF, MUT_ROE_NO_HARTEN, MUT_ROE_NO_SHEAR = range(6)
+(MUT_NONE, MUT_LXF_MIN, MUT_LXF_NO_HALF, MUT_ROE_NO_HALF, MUT_ROE_NO_HARTEN, MUT_ROE_NO_SHEAR,
+ MUT_NO_METRIC_AVERAGE, MUT_CROSS_PRODUCT_METRIC) = range(8)
 
 
 def set_mutation(m):

@@ -64,6 +64,11 @@ struct dgo_ctx {
     /* physics constants (src/physics/euler.cpp:13-67, navier_stokes.cpp:16-55) */
     double gam, gamm1, mach2, pinf, temperature_ratio;
     double vinf[3];               /* free-stream velocity (Euler constructor, euler.cpp:53-61) */
+    /* geometry: node coordinates [cell][node][3]; on a curvilinear grid (curv) the metric cofactor
+       [cell][node][3][3] and det J [cell][node] at the GLL nodes (metric_operators,
+       operators.cpp:2520-2884) */
+    double *xn, *metC, *metJ;
+    int curv;
     int fsal;                     /* 3S*+ methods: evaluate the FSAL residual (reference: always) */
 };
 
@@ -546,6 +551,134 @@ void dgo_boundary_state(const dgo_ctx *c, int type, const double *u_int, const d
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* geometry: node coordinates and metric terms                              */
+/* ------------------------------------------------------------------------ */
+/* NonsymmetricCurvedGridManifold::mapping (nonsymmetric_curved_periodic_grid.cpp:66-87), beta = 0.1:
+   the reference's curved periodic grid on [-1,1]^2 / [0,2pi]^3 */
+static void warp_point(int dim, double *x)
+{
+    const double beta = 1.0 / 10.0, pi = 3.14159265358979323846;
+    const double p0 = x[0], p1 = x[1], p2 = x[2];
+    if (dim == 2) {
+        x[0] = p0 + beta * cos(pi / 2.0 * p0) * cos(3.0 * pi / 2.0 * p1);
+        x[1] = p1 + beta * sin(2.0 * pi * p0) * cos(pi / 2.0 * p1);
+    } else {
+        x[0] = p0 + beta * sin(p0) * sin(p1) * sin(2.0 * p2);
+        x[1] = p1 + beta * sin(4.0 * p0) * sin(p1) * sin(3.0 * p2);
+        x[2] = p2 + beta * sin(2.0 * p0) * sin(5.0 * p1) * sin(p2);
+    }
+}
+
+static void build_coords(dgo_ctx *c)
+{
+    const int n = c->n, Np = c->Np;
+    c->xn = (double *)malloc(sizeof(double) * (size_t)c->ncells * Np * 3);
+    for (int e = 0; e < c->ncells; ++e)
+        for (int node = 0; node < Np; ++node) {
+            int id[3] = {node % n, (node / n) % n, (c->dim == 3) ? node / (n * n) : 0};
+            double *x = c->xn + ((long)e * Np + node) * 3;
+            for (int k = 0; k < 3; ++k)
+                x[k] = (k < c->dim) ? c->P.domain_left + (c->cell_ijk[e * 3 + k] + c->xs[id[k]]) * c->h : 0.0;
+            if (c->P.grid_warp) warp_point(c->dim, x);
+        }
+}
+
+/* Metric terms at the GLL flux nodes from the mapping support points at the same nodes (grid
+   degree = p): the Jacobian dX_i/dxi_j by the 1D derivative operator (build_metric_Jacobian,
+   operators.cpp:2600-2650), det J, and the cofactor C[m][n] = J dxi_n/dx_m -- in 2D from the
+   Jacobian (operators.cpp:2700-2714), in 3D by the conservative curl form (compute_local_3D_cofactor,
+   operators.cpp:2737-2884; use_invariant_curl_form = false), which satisfies the discrete metric
+   identities sum_n D_n C[m][n] = 0. */
+static void compute_metrics(dgo_ctx *c)
+{
+    const int d = c->dim, n = c->n, Np = c->Np;
+    free(c->metC); free(c->metJ);
+    c->metC = (double *)calloc((size_t)c->ncells * Np * 9, sizeof(double));
+    c->metJ = (double *)malloc(sizeof(double) * (size_t)c->ncells * Np);
+    double *X = (double *)malloc(sizeof(double) * 3 * Np);
+    double *Jm = (double *)malloc(sizeof(double) * 9 * Np);   /* Jm[(i*3+j)*Np + q] = dX_i/dxi_j */
+    double *P = (double *)malloc(sizeof(double) * 9 * Np);    /* curl-form products */
+    double *Cm = (double *)malloc(sizeof(double) * 9 * Np);
+    for (int e = 0; e < c->ncells; ++e) {
+        for (int i = 0; i < d; ++i)
+            for (int q = 0; q < Np; ++q) X[i * Np + q] = c->xn[((long)e * Np + q) * 3 + i];
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j < d; ++j) dir_apply(c, j, c->chi_grad, c->chi, n, n, X + i * Np, Jm + (i * 3 + j) * Np, 0, 1.0);
+        if (d == 2) {
+            for (int q = 0; q < Np; ++q) {
+                Cm[(0 * 3 + 0) * Np + q] = Jm[(1 * 3 + 1) * Np + q];
+                Cm[(1 * 3 + 0) * Np + q] = -Jm[(0 * 3 + 1) * Np + q];
+                Cm[(0 * 3 + 1) * Np + q] = -Jm[(1 * 3 + 0) * Np + q];
+                Cm[(1 * 3 + 1) * Np + q] = Jm[(0 * 3 + 0) * Np + q];
+            }
+        } else {
+            /* X_l dX_m/dxi_j at the grid nodes: z dy/d(xi,eta,zeta), x dz/d(..), y dx/d(..) */
+            const int lm[3][2] = {{2, 1}, {0, 2}, {1, 0}};
+            for (int r = 0; r < 3; ++r)
+                for (int j = 0; j < 3; ++j)
+                    for (int q = 0; q < Np; ++q)
+                        P[(r * 3 + j) * Np + q] = X[lm[r][0] * Np + q] * Jm[(lm[r][1] * 3 + j) * Np + q];
+            for (int r = 0; r < 3; ++r) {
+                const double *Pxi = P + (r * 3 + 0) * Np, *Peta = P + (r * 3 + 1) * Np, *Pzeta = P + (r * 3 + 2) * Np;
+                double *C0 = Cm + (r * 3 + 0) * Np, *C1 = Cm + (r * 3 + 1) * Np, *C2 = Cm + (r * 3 + 2) * Np;
+                /* C[r][0] = -d_eta(P_zeta) + d_zeta(P_eta); C[r][1] = d_xi(P_zeta) - d_zeta(P_xi);
+                   C[r][2] = -d_xi(P_eta) + d_eta(P_xi) */
+                dir_apply(c, 1, c->chi_grad, c->chi, n, n, Pzeta, C0, 0, -1.0);
+                dir_apply(c, 2, c->chi_grad, c->chi, n, n, Peta, C0, 1, 1.0);
+                dir_apply(c, 0, c->chi_grad, c->chi, n, n, Pzeta, C1, 0, 1.0);
+                dir_apply(c, 2, c->chi_grad, c->chi, n, n, Pxi, C1, 1, -1.0);
+                dir_apply(c, 0, c->chi_grad, c->chi, n, n, Peta, C2, 0, -1.0);
+                dir_apply(c, 1, c->chi_grad, c->chi, n, n, Pxi, C2, 1, 1.0);
+            }
+        }
+        for (int q = 0; q < Np; ++q) {
+            double J[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) J[i][j] = (i < d && j < d) ? Jm[(i * 3 + j) * Np + q] : 0.0;
+            double det;
+            if (d == 2) det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+            else det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0])
+                     + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+            c->metJ[(long)e * Np + q] = det;
+            if (d == 3 && g_mutation == DGO_MUT_CROSS_PRODUCT_METRIC)
+                /* planted: the pointwise cofactor det(J) J^-T (cross products of the Jacobian's columns) */
+                for (int m = 0; m < 3; ++m)
+                    for (int nn = 0; nn < 3; ++nn) {
+                        const int m1 = (m + 1) % 3, m2 = (m + 2) % 3, n1 = (nn + 1) % 3, n2 = (nn + 2) % 3;
+                        Cm[(m * 3 + nn) * Np + q] = J[m1][n1] * J[m2][n2] - J[m1][n2] * J[m2][n1];
+                    }
+            for (int a = 0; a < d; ++a)
+                for (int b = 0; b < d; ++b) c->metC[((long)e * Np + q) * 9 + a * 3 + b] = Cm[(a * 3 + b) * Np + q];
+        }
+    }
+    free(X); free(Jm); free(P); free(Cm);
+}
+
+int dgo_set_grid_nodes(dgo_ctx *c, const double *xyz)
+{
+    if (c->P.flux_nodes != DGO_GLL || c->P.pde != DGO_EULER) return -1;
+    memcpy(c->xn, xyz, sizeof(double) * (size_t)c->ncells * c->Np * 3);
+    compute_metrics(c);
+    c->curv = 1;
+    return 0;
+}
+
+void dgo_metric_terms(const dgo_ctx *c, double *C, double *J)
+{
+    const long nn = (long)c->ncells * c->Np;
+    for (long q = 0; q < nn; ++q) {
+        const int e = (int)(q / c->Np), node = (int)(q % c->Np);
+        for (int a = 0; a < 9; ++a) {
+            double v = 0.0;
+            if (c->curv) v = c->metC[q * 9 + a];
+            else if (a / 3 == a % 3 && a / 3 < c->dim) v = c->cof;
+            C[q * 9 + a] = v;
+        }
+        J[q] = c->curv ? c->metJ[(long)e * c->Np + node] : c->detJ;
+    }
+}
+
 /* owner rule current_cell_should_do_the_work (dg_base.cpp:417-449): lower index works */
 int dgo_n_faces(const dgo_ctx *c)
 {
@@ -574,15 +707,7 @@ void dgo_face_list(const dgo_ctx *c, int *faces)
 
 void dgo_node_coords(const dgo_ctx *c, double *xyz)
 {
-    for (int e = 0; e < c->ncells; ++e)
-        for (int node = 0; node < c->Np; ++node) {
-            int id[3] = {node % c->n, (node / c->n) % c->n, (c->dim == 3) ? node / (c->n * c->n) : 0};
-            for (int k = 0; k < 3; ++k) {
-                double x = 0.0;
-                if (k < c->dim) x = c->P.domain_left + (c->cell_ijk[e * 3 + k] + c->xs[id[k]]) * c->h;
-                xyz[((long)e * c->Np + node) * 3 + k] = x;
-            }
-        }
+    memcpy(xyz, c->xn, sizeof(double) * (size_t)c->ncells * c->Np * 3);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -613,6 +738,7 @@ void dgo_default_params(dgo_params *p)
     p->cell_order = DGO_ORDER_DEALII;
     for (int f = 0; f < 6; ++f) p->boundary_type[f] = 0;
     p->angle_of_attack = 0.0;
+    p->grid_warp = 0;
 }
 
 dgo_ctx *dgo_create(const dgo_params *prm)
@@ -649,6 +775,11 @@ dgo_ctx *dgo_create(const dgo_params *prm)
             free(c);
             return NULL;
         }
+    }
+    /* curvilinear grids: collocated GLL nodes (grid degree = p), Euler */
+    if (prm->grid_warp && (prm->flux_nodes != DGO_GLL || prm->pde != DGO_EULER || prm->grid_warp != 1)) {
+        free(c);
+        return NULL;
     }
 
     const int n = c->n, nq = c->nq;
@@ -719,6 +850,11 @@ dgo_ctx *dgo_create(const dgo_params *prm)
         dgo_hadamard_surface_pattern(c->dim, nq, k, c->had_srows[k], c->had_scols[k]);
     }
     build_mesh(c);
+    build_coords(c);
+    if (prm->grid_warp) {
+        compute_metrics(c);
+        c->curv = 1;
+    }
     if (prm->pde == DGO_NAVIER_STOKES) {
         for (int k = 0; k < c->dim; ++k) {
             c->aux[k] = (double *)calloc((size_t)dgo_n_dofs(c), sizeof(double));
@@ -738,6 +874,7 @@ void dgo_destroy(dgo_ctx *c)
         free(c->aux[k]); free(c->aux_rhs[k]);
     }
     free(c->cell_ijk); free(c->nbr);
+    free(c->xn); free(c->metC); free(c->metJ);
     free(c);
 }
 
@@ -1155,6 +1292,34 @@ static inline void metric_cofactor(const dgo_ctx *c, double C[3][3])
         for (int b = 0; b < 3; ++b) C[a][b] = (a == b && a < c->dim) ? c->cof : 0.0;
 }
 
+/* the metric cofactor at node q of a cell (constant on the affine box) */
+static inline void metric_at(const dgo_ctx *c, int cell, int q, double C[3][3])
+{
+    if (!c->curv) {
+        metric_cofactor(c, C);
+        return;
+    }
+    const double *m = c->metC + ((long)cell * c->Np + q) * 9;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) C[a][b] = m[a * 3 + b];
+}
+
+static inline double jac_at(const dgo_ctx *c, int cell, int q)
+{
+    return c->curv ? c->metJ[(long)cell * c->Np + q] : c->detJ;
+}
+
+/* volume node of face node f of a face (GLL: the face nodes are volume nodes) */
+static int face_vol_node(const dgo_ctx *c, int face, int f)
+{
+    const int n = c->n, dir = face / 2, end = (face % 2) ? n - 1 : 0;
+    const int a = f % n, b = (c->dim == 3) ? f / n : 0;
+    if (c->dim == 2) return dir == 0 ? end + n * a : a + n * end;
+    if (dir == 0) return end + n * (a + n * b);
+    if (dir == 1) return a + n * (end + n * b);
+    return a + n * (b + n * end);
+}
+
 /* transform_physical_to_reference (operators.cpp:2354-2365): ref_i = sum_j C_ji phys_j */
 static inline double phys_to_ref(const dgo_ctx *c, const double C[3][3], const double *phys, int i)
 {
@@ -1207,10 +1372,10 @@ static void volume_term(dgo_ctx *c, int cell, const double *u, double *lrhs)
         }
         free(ev);
     }
-    double C[3][3];
-    metric_cofactor(c, C);
     for (int q = 0; q < Np; ++q) {
         double st[NS_MAX], ag[NS_MAX * 3];
+        double C[3][3];
+        metric_at(c, cell, q, C);
         for (int s = 0; s < ns; ++s) {
             st[s] = sq[s * Np + q];
             for (int k = 0; k < d; ++k) ag[s * d + k] = aq[(s * d + k) * Np + q];
@@ -1227,9 +1392,11 @@ static void volume_term(dgo_ctx *c, int cell, const double *u, double *lrhs)
                     for (int s = 0; s < ns; ++s) evc[s] = pev[s * Np + col];
                     dgo_conservative_from_entropy(c, evc, uc);
                     dgo_two_point_flux(c, c->P.two_point_flux, st, uc, F);
-                    double Cs[3][3];
+                    /* the two-point average of the metric cofactor (strong_dg.cpp:1348-1372) */
+                    double Cc[3][3], Cs[3][3];
+                    metric_at(c, cell, g_mutation == DGO_MUT_NO_METRIC_AVERAGE ? q : col, Cc);
                     for (int a = 0; a < 3; ++a)
-                        for (int b = 0; b < 3; ++b) Cs[a][b] = 0.5 * (C[a][b] + C[a][b]);
+                        for (int b = 0; b < 3; ++b) Cs[a][b] = 0.5 * (C[a][b] + Cc[a][b]);
                     for (int s = 0; s < ns; ++s)
                         F2[((s * d) + rd) * Np * nq + q * nq + l] = phys_to_ref(c, Cs, F + s * d, rd);
                 }
@@ -1322,8 +1489,6 @@ static void face_term(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *
     double *pev_vol[2] = {NULL, NULL}, *pev_surf[2] = {NULL, NULL};
     double *tmp = (double *)malloc(sizeof(double) * Np);
     double *tmp2 = (double *)malloc(sizeof(double) * Np);
-    double C[3][3];
-    metric_cofactor(c, C);
     for (int side = 0; side < nsides; ++side) {
         const int cell = cells[side], face = faces[side];
         svol[side] = (double *)malloc(sizeof(double) * ns * Np);
@@ -1350,6 +1515,8 @@ static void face_term(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *
         double *Fd = (double *)calloc((size_t)ns * Np, sizeof(double));
         for (int q = 0; q < Np; ++q) {
             double st[NS_MAX], ag[NS_MAX * 3];
+            double C[3][3];
+            metric_at(c, cell, q, C);
             for (int s = 0; s < ns; ++s) {
                 st[s] = svol[side][s * Np + q];
                 for (int k = 0; k < d; ++k) ag[s * d + k] = avol[side][(s * d + k) * Np + q];
@@ -1410,9 +1577,12 @@ static void face_term(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *
                     for (int s = 0; s < ns; ++s) evv[s] = pev_vol[side][s * Np + iv];
                     dgo_conservative_from_entropy(c, evv, uv);
                     dgo_two_point_flux(c, c->P.two_point_flux, uv, uf, F);
-                    double Cs[3][3];
+                    /* metric average of the face node and the volume node (strong_dg.cpp:3172-3200) */
+                    double Cf[3][3], Cv[3][3], Cs[3][3];
+                    metric_at(c, cells[side], face_vol_node(c, face, f), Cf);
+                    metric_at(c, cells[side], iv, Cv);
                     for (int a = 0; a < 3; ++a)
-                        for (int b = 0; b < 3; ++b) Cs[a][b] = 0.5 * (C[a][b] + C[a][b]);
+                        for (int b = 0; b < 3; ++b) Cs[a][b] = 0.5 * (Cf[a][b] + Cv[a][b]);
                     for (int s = 0; s < ns; ++s) S2[(s * nfp + f) * nq + l] = phys_to_ref(c, Cs, F + s * d, dnz);
                 }
             }
@@ -1455,6 +1625,9 @@ static void face_term(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *
         for (int f = 0; f < nfp; ++f) {
             double ui[NS_MAX], ue[NS_MAX];
             double nphys[3];
+            /* the interior cell's metric at the face node (metric_oper_int) */
+            double C[3][3];
+            metric_at(c, ci, face_vol_node(c, fi, f), C);
             for (int a = 0; a < d; ++a) {
                 nphys[a] = 0.0;
                 for (int b = 0; b < d; ++b) nphys[a] += C[a][b] * nref[b];
@@ -1715,14 +1888,33 @@ void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
                         uj[s] = u[IDX(c, e, s, j)];
                         for (int kk = 0; kk < d; ++kk) gj[s * d + kk] = visc ? c->aux[kk][IDX(c, e, s, j)] : 0.0;
                     }
+                    /* reference-direction component of a physical flux: F_k on the affine box
+                       (h^(d-1) applied below), sum_m F_m Cbar[m][k] on a curvilinear grid */
+                    double Ci[3][3], Cj[3][3];
+                    metric_at(c, e, i, Ci);
+                    metric_at(c, e, j, Cj);
                     if (split) {
                         if (b != ci[k]) {
                             dgo_two_point_flux(c, c->P.two_point_flux, ui, uj, F);
-                            for (int s = 0; s < ns; ++s) acc[s] += fabs(Sk[ci[k]][b] * F[s * d + k]);
+                            for (int s = 0; s < ns; ++s) {
+                                double fk = F[s * d + k];
+                                if (c->curv) {
+                                    fk = 0.0;
+                                    for (int m = 0; m < d; ++m) fk += F[s * d + m] * 0.5 * (Ci[m][k] + Cj[m][k]);
+                                }
+                                acc[s] += fabs(Sk[ci[k]][b] * fk);
+                            }
                         }
                     } else {
                         dgo_convective_flux(c, uj, F);
-                        for (int s = 0; s < ns; ++s) acc[s] += fabs(D[ci[k]][b] * F[s * d + k]);
+                        for (int s = 0; s < ns; ++s) {
+                            double fk = F[s * d + k];
+                            if (c->curv) {
+                                fk = 0.0;
+                                for (int m = 0; m < d; ++m) fk += F[s * d + m] * Cj[m][k];
+                            }
+                            acc[s] += fabs(D[ci[k]][b] * fk);
+                        }
                     }
                     if (visc) {
                         dgo_dissipative_flux(c, uj, gj, F);
@@ -1738,6 +1930,17 @@ void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
                 const int j = cj[0] + n * (cj[1] + n * cj[2]);
                 double ue[NS_MAX], ge[NS_MAX * 3], nrm[3] = {0, 0, 0}, fl[NS_MAX], F[NS_MAX * 3];
                 nrm[k] = side ? 1.0 : -1.0;
+                double area = 1.0;  /* |C^T n_ref| on a curvilinear grid */
+                if (c->curv) {
+                    double Ci[3][3], a2 = 0.0;
+                    metric_at(c, e, i, Ci);
+                    for (int m = 0; m < d; ++m) {
+                        nrm[m] = Ci[m][k] * (side ? 1.0 : -1.0);
+                        a2 += nrm[m] * nrm[m];
+                    }
+                    area = sqrt(a2);
+                    for (int m = 0; m < d; ++m) nrm[m] /= area;
+                }
                 if (nb < 0) {
                     dgo_boundary_state(c, -nb, ui, nrm, ue);
                     for (int s = 0; s < ns * 3; ++s) ge[s] = 0.0;
@@ -1749,12 +1952,19 @@ void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
                 }
                 dgo_numerical_flux(c, c->P.conv_flux, ui, ue, nrm, fl);
                 dgo_convective_flux(c, ui, F);
-                for (int s = 0; s < ns; ++s) acc[s] += (fabs(fl[s]) + fabs(F[s * d + k])) / c->wq[0];
+                for (int s = 0; s < ns; ++s) {
+                    double fn = F[s * d + k];
+                    if (c->curv) {
+                        fn = 0.0;
+                        for (int m = 0; m < d; ++m) fn += F[s * d + m] * nrm[m];
+                    }
+                    acc[s] += area * (fabs(fl[s]) + fabs(fn)) / c->wq[0];
+                }
                 const int cf = c->P.conv_flux;
                 if (cf == DGO_LXF || cf == DGO_ROE || cf == DGO_TWO_POINT_LXF || cf == DGO_TWO_POINT_ROE) {
                     const double li = max_normal_eig(c, ui, nrm), le = max_normal_eig(c, ue, nrm);
                     const double lam = li > le ? li : le;
-                    for (int s = 0; s < ns; ++s) acc[s] += 0.5 * lam * (fabs(ui[s]) + fabs(ue[s])) / c->wq[0];
+                    for (int s = 0; s < ns; ++s) acc[s] += area * 0.5 * lam * (fabs(ui[s]) + fabs(ue[s])) / c->wq[0];
                 }
                 if (visc) {
                     viscous_num_flux(c, ui, ue, gi, ge, nrm, penalty, fl);
@@ -1774,7 +1984,7 @@ void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
                     }
                 }
             }
-            for (int s = 0; s < ns; ++s) mag[IDX(c, e, s, i)] = c->wvol[i] * hd1 * acc[s];
+            for (int s = 0; s < ns; ++s) mag[IDX(c, e, s, i)] = c->wvol[i] * (c->curv ? 1.0 : hd1) * acc[s];
         }
     }
 }
@@ -1788,8 +1998,16 @@ void dgo_apply_inverse_mass(const dgo_ctx *c, const double *in, double *out, int
 #pragma omp parallel for schedule(static) num_threads(g_threads)
 #endif
     for (int e = 0; e < c->ncells; ++e)
-        for (int s = 0; s < ns; ++s)
-            vol_apply(c, c->minv, n, n, in + IDX(c, e, s, 0), out + IDX(c, e, s, 0), 0, 1.0 / c->detJ);
+        for (int s = 0; s < ns; ++s) {
+            if (c->curv) {
+                /* curvilinear, collocated GLL: M = W det J (diagonal; the weight-adjusted inverse,
+                   dg_base.cpp:4220-4247, is the same operator here) */
+                for (int i = 0; i < c->Np; ++i)
+                    out[IDX(c, e, s, i)] = in[IDX(c, e, s, i)] / (c->wvol[i] * c->metJ[(long)e * c->Np + i]);
+            } else {
+                vol_apply(c, c->minv, n, n, in + IDX(c, e, s, 0), out + IDX(c, e, s, 0), 0, 1.0 / c->detJ);
+            }
+        }
 }
 
 void dgo_get_auxiliary(const dgo_ctx *c, int k, double *out)
@@ -2146,6 +2364,11 @@ double dgo_vortex_time_step(const dgo_ctx *c, double cfl, double speed)
 /* ------------------------------------------------------------------------ */
 void dgo_integrated_quantities(const dgo_ctx *c, const double *u, int overint, double *out)
 {
+    if (c->curv) {
+        /* over-integration on a curvilinear grid needs the metric at the QGauss points: not restated */
+        for (int q = 0; q < 4; ++q) out[q] = NAN;
+        return;
+    }
     const int d = c->dim, ns = c->ns, n = c->n, Np = c->Np;
     const int nx = c->p + 1 + overint;
     double xq[64], wq[64];
@@ -2295,7 +2518,7 @@ double dgo_numerical_entropy(const dgo_ctx *c, const double *u)
             const double rho = us[0];
             const double p = dgo_pressure(c, us);
             const double ent = log(p * pow(rho, -c->gam));
-            acc += (-rho * ent) * c->wvol[q] * c->detJ;
+            acc += (-rho * ent) * c->wvol[q] * jac_at(c, e, q);
         }
     }
     free(uq);

@@ -49,10 +49,14 @@ enum { DGO_SIPG = 0, DGO_BR2 = 1 };
 enum { DGO_ORDER_DEALII = 0, DGO_ORDER_LEXICOGRAPHIC = 1 };
 /* planted mistakes (tests of the tests only): LxF wave speed min instead of max, LxF or Roe
    dissipation without its factor 1/2, Roe without Harten's entropy fix, Roe without the shear
-   (tangential velocity jump) wave */
+   (tangential velocity jump) wave, the curvilinear two-point flux without its metric average, the 3D
+   cofactor without the conservative curl form */
 enum {
     DGO_MUT_NONE = 0, DGO_MUT_LXF_MIN = 1, DGO_MUT_LXF_NO_HALF = 2, DGO_MUT_ROE_NO_HALF = 3,
-    DGO_MUT_ROE_NO_HARTEN = 4, DGO_MUT_ROE_NO_SHEAR = 5
+    DGO_MUT_ROE_NO_HARTEN = 4, DGO_MUT_ROE_NO_SHEAR = 5,
+    DGO_MUT_NO_METRIC_AVERAGE = 6,  /* curvilinear split form: the node's own cofactor in place of the
+                                       two-point average */
+    DGO_MUT_CROSS_PRODUCT_METRIC = 7 /* 3D cofactor det(J) J^-T pointwise instead of the curl form */
 };
 
 typedef struct {
@@ -82,6 +86,9 @@ typedef struct {
        1003 inflow, 1004 Riemann farfield, 1005 farfield, 1006 slip wall, 1007 extrapolation */
     int boundary_type[6];
     double angle_of_attack;  /* radians; free stream (cos a, sin a[, 0]) */
+    int grid_warp;           /* 0: the affine box; 1: the reference's curved periodic grid
+                                (NonsymmetricCurvedGridManifold, nonsymmetric_curved_periodic_grid.cpp:66-87,
+                                beta = 0.1) applied to the box's nodes; GLL, Euler */
 } dgo_params;
 
 typedef struct dgo_ctx dgo_ctx;
@@ -123,7 +130,12 @@ void dgo_cell_ijk(const dgo_ctx *c, int *ijk);            /* [ncells][3] */
 void dgo_neighbors(const dgo_ctx *c, int *nbr);            /* [ncells][2*dim] */
 void dgo_face_list(const dgo_ctx *c, int *faces);          /* [nfaces][4]: owner, iface, nbr, nbr_iface */
 int  dgo_n_faces(const dgo_ctx *c);
-void dgo_node_coords(const dgo_ctx *c, double *xyz);       /* [ncells][Np][3] */
+void dgo_node_coords(const dgo_ctx *c, double *xyz);
+/* curvilinear grid from the caller's node coordinates [cell][node][3] (mapping support points of
+   grid degree p at the GLL nodes); 0, or -1 if the discretisation is not GLL Euler */
+int dgo_set_grid_nodes(dgo_ctx *c, const double *xyz);
+/* metric cofactor C[m][n] = J dxi_n/dx_m [cell][node][3][3] and det J [cell][node] */
+void dgo_metric_terms(const dgo_ctx *c, double *C, double *J);       /* [ncells][Np][3] */
 
 /* point physics (nstate = dim+2) */
 double dgo_pressure(const dgo_ctx *c, const double *u);
