@@ -12,8 +12,9 @@ Time step: the reference's TGV rule dt = CFL*L/(n_dofs/nstate)^(1/3) with CFL 0.
 CFL 0.1 diverges within ten steps in the FP64 oracle as well).
 
 Metric: DOF-updates/s per RK stage = DOF x stages x steps / wall time.  The
-headline `value` is the FP64 path (the reference's precision: every flux, the
-entropy round trip and the state in double).  The mixed path (FP32 fluxes and
+headline `value` is the FP64 path (the reference's precision: every flux and the
+state in double; the entropy round trip u(v(u)) is the identity on collocated GLL
+nodes and is not formed).  The mixed path (FP32 fluxes and
 contractions, FP64 state, RK registers and reductions) is reported beside it in
 `mixed_path`, with its own roofline and its error against the FP64 path.
 Roofline: HBM-bound; algorithmic bytes per stage launch (FP64 registers of the
@@ -294,6 +295,37 @@ def navier_stokes_tgv(precision="mixed", steps=3):
                 precision=precision, workload="NS TGV Re1600 p=5 16^3, BR2, Ranocha split + EC/Roe, RK4")
 
 
+def curvilinear_tgv(precision="fp64", steps=3):
+    """Euler TGV p=3 on the reference's curved periodic grid (NonsymmetricCurvedGridManifold, beta
+    0.1) of [0, 2 pi]^3 with 32^3 cells (10,485,760 DOF), CH split form + EC/Roe surface flux,
+    RK3(2)5F[3S*+]: the curvilinear stage kernel (stage2_kernel with the metric terms).  Algorithmic
+    bytes per DOF-update: the RK registers' 38.4 B plus the metric (9 cofactors and det J, 80 B per
+    node = 16 B per DOF)."""
+    import philip_amd as P
+    d = P.DG(precision=P.MIXED if precision == "mixed" else P.FP64, dim=3, poly_degree=3, n_cells_1d=32,
+             grid_warp=1, use_split_form=1, two_point_flux=P.CH, conv_flux=P.TWO_POINT_ROE, mach_inf=0.1)
+    d.set_initial_condition_tgv()
+    dt = d.constant_time_step(0.01)
+    d.allocate_ode_system("rk3_2_5f_3sstarplus")
+    d.step_in_time(dt, 1)
+    d.synchronize()
+    t0 = time.perf_counter()
+    d.step_in_time(dt, steps)
+    d.synchronize()
+    t = time.perf_counter() - t0
+    d.enable_timing(True)
+    d.step_in_time(dt, 1)
+    avg_ms, _ = d.kernel_timing()
+    d.enable_timing(False)
+    n = d.ndofs
+    d.close()
+    bytes_per_launch = n * (38.4 + 16.0)
+    return dict(value=n * 5 * steps / t, unit="DOF-updates/s", dof=n, ms_per_stage=1e3 * t / (5 * steps),
+                kernel_ms=avg_ms, hbm_frac=bytes_per_launch / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                precision=precision,
+                workload="Euler TGV p=3 on the curved periodic grid 32^3, CH split + EC/Roe, RK3(2)5F[3S*+]")
+
+
 def navier_stokes_history(final_time=2.0, every=100):
     """The Navier-Stokes TGV above run to t = final_time (CFL 0.01, last step clipped) at both
     precisions, enstrophy and the vorticity-based dissipation rate (QGauss(p+11)) every `every`
@@ -376,6 +408,8 @@ def main():
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     ap.add_argument("--navier-stokes-only", choices=["fp64", "mixed"], default=None,
                     help="only run the Navier-Stokes TGV stage rate at this precision (for profiling)")
+    ap.add_argument("--curvilinear-only", choices=["fp64", "mixed"], default=None,
+                    help="only run the curved-grid Euler TGV stage rate at this precision (for profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances line")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream line")
     ap.add_argument("--serving-rate", type=float, default=8.0,
@@ -386,6 +420,9 @@ def main():
     args = ap.parse_args()
     if args.navier_stokes_only:
         print(json.dumps(navier_stokes_tgv(args.navier_stokes_only)), flush=True)
+        return
+    if args.curvilinear_only:
+        print(json.dumps(curvilinear_tgv(args.curvilinear_only)), flush=True)
         return
     if args.copy_calibration:
         copy_calibration()
@@ -431,6 +468,8 @@ def main():
         progress(f"batched: {line['batched']['value']:.3e} DOF-updates/s")
         line["navier_stokes"] = [navier_stokes_tgv("fp64"), navier_stokes_tgv("mixed")]
         progress("Navier-Stokes stage rates done")
+        line["curvilinear"] = [curvilinear_tgv("fp64"), curvilinear_tgv("mixed")]
+        progress("curvilinear-grid stage rates done")
     if not args.no_ns_history:
         line["tgv_1000_steps"] = tgv_history()
         progress("TGV 1000-step histories done")

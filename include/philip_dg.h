@@ -80,6 +80,10 @@ typedef struct {
        farfield, 1005 farfield, 1006 slip wall, 1007 extrapolation.  Euler only. */
     int boundary_type[6];
     double angle_of_attack;     /* radians: free stream (cos a, sin a[, 0]) of the boundary states */
+    /* 0: the affine box; 1: the box's nodes through the reference's curved periodic grid
+       (NonsymmetricCurvedGridManifold, nonsymmetric_curved_periodic_grid.cpp:66-87, beta = 0.1,
+       made for [-1,1]^2 and [0,2pi]^3) with curvilinear metric terms.  Euler only. */
+    int grid_warp;
 } dgx_params;
 
 typedef struct dgx_ctx dgx_ctx;
@@ -117,6 +121,18 @@ int dgx_get_right_hand_side(const dgx_ctx *ctx, double *host_r);
 int dgx_get_auxiliary_solution(const dgx_ctx *ctx, int idim, double *host_q);
 /* integer connectivity used by the kernels: nbr[cell][2*dim] (deal.II face numbering). */
 int dgx_get_neighbors(const dgx_ctx *ctx, int32_t *host_nbr);
+/* Curvilinear grid: the mapping support points of every cell at its GLL nodes (grid degree = p),
+   host array [cell][node][3] in the reference's cell order -- what HighOrderGrid holds
+   (high_order_grid.cpp) for grid_degree = poly_degree -- from which the device builds the metric
+   terms of metric_operators (operators.cpp:2520-2884: Jacobian, det J, and in 3D the conservative
+   curl form of the cofactor).  The box topology (periodic or boundary faces) is kept.  Euler only;
+   the batched line kernel and the integrated quantities do not take curvilinear contexts. */
+int dgx_set_grid_nodes(dgx_ctx *ctx, const double *host_xyz);
+/* node coordinates in use [cell][node][3] (the affine box if none were set) */
+int dgx_get_grid_nodes(dgx_ctx *ctx, double *host_xyz);
+/* metric terms [cell][dim*dim + 1][node]: cofactor C[m][n] = J dxi_n/dx_m in slot m*dim + n, det J
+   in slot dim*dim; DGX_ERR_STATE on the affine box */
+int dgx_get_metric_terms(dgx_ctx *ctx, double *host_met);
 
 /* DGBase::assemble_residual(false,false,false,0.0) (dg_base.hpp:571,
    dg_base.cpp:2599-2914): right_hand_side = R(solution). */

@@ -2573,7 +2573,7 @@ double dgo_step_rrk(dgo_ctx *c, int m, double dt, double *u, double *gamma_io, d
                 double us[NS_MAX], vs[NS_MAX];
                 for (int st = 0; st < ns; ++st) us[st] = stage_sol[i][IDX(c, e, st, q)];
                 dgo_entropy_variables(c, us, vs);
-                for (int st = 0; st < ns; ++st) ent += vs[st] * (c->wvol[q] * c->detJ * stage[i][IDX(c, e, st, q)]);
+                for (int st = 0; st < ns; ++st) ent += vs[st] * (c->wvol[q] * jac_at(c, e, q) * stage[i][IDX(c, e, st, q)]);
             }
         est += B.b[i] * ent;
     }

@@ -25,7 +25,8 @@ METHODS = {"euler": 0, "heun": 1, "ssprk3": 2, "rk4": 3,
 EXPORTED = [
     "dgx_default_params", "dgx_create", "dgx_destroy", "dgx_last_error", "dgx_n_dofs", "dgx_n_cells",
     "dgx_set_solution", "dgx_get_solution", "dgx_get_right_hand_side", "dgx_get_auxiliary_solution",
-    "dgx_get_neighbors", "dgx_assemble_residual", "dgx_apply_inverse_mass_to_rhs",
+    "dgx_get_neighbors", "dgx_set_grid_nodes", "dgx_get_grid_nodes", "dgx_get_metric_terms",
+    "dgx_assemble_residual", "dgx_apply_inverse_mass_to_rhs",
     "dgx_assemble_residual_device", "dgx_rhs_device", "dgx_apply_inverse_mass_device",
     "dgx_allocate_ode_system", "dgx_step_in_time", "dgx_current_time", "dgx_current_iteration", "dgx_set_time",
     "dgx_set_track_error_estimate", "dgx_constant_time_step", "dgx_integrated_quantities",
@@ -53,7 +54,7 @@ class Params(ctypes.Structure):
         ("temperature_inf", ctypes.c_double), ("sipg_penalty_factor", ctypes.c_double),
         ("n_cells_1d", ctypes.c_int), ("domain_left", ctypes.c_double), ("domain_right", ctypes.c_double),
         ("cell_order", ctypes.c_int), ("precision", ctypes.c_int),
-        ("boundary_type", ctypes.c_int * 6), ("angle_of_attack", ctypes.c_double),
+        ("boundary_type", ctypes.c_int * 6), ("angle_of_attack", ctypes.c_double), ("grid_warp", ctypes.c_int),
     ]
 
 
@@ -106,7 +107,8 @@ def load():
     lib.dgx_n_dofs.restype = i64
     lib.dgx_n_dofs.argtypes = [vp]
     lib.dgx_n_cells.argtypes = [vp]
-    for fn in ["dgx_set_solution", "dgx_get_solution", "dgx_get_right_hand_side", "dgx_apply_inverse_mass_to_rhs"]:
+    for fn in ["dgx_set_solution", "dgx_get_solution", "dgx_get_right_hand_side", "dgx_apply_inverse_mass_to_rhs",
+               "dgx_set_grid_nodes", "dgx_get_grid_nodes", "dgx_get_metric_terms"]:
         getattr(lib, fn).argtypes = [vp, dp]
     lib.dgx_get_auxiliary_solution.argtypes = [vp, i, dp]
     lib.dgx_set_initial_condition_tgv.argtypes = [vp, i]
@@ -288,6 +290,29 @@ class DG:
         a = np.empty((self.ncells, 2 * self.dim), dtype=np.int32)
         self._chk(load().dgx_get_neighbors(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "neighbors")
         return a
+
+    def set_grid_nodes(self, xyz):
+        """Curvilinear grid from node coordinates [cell][node][3] (grid degree = p, Euler)."""
+        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.ncells, -1, 3)
+        self._chk(load().dgx_set_grid_nodes(self.h, _dp(x)), "set_grid_nodes")
+
+    def grid_nodes(self):
+        """Node coordinates in use, [cell][node][3]."""
+        x = np.empty((self.ncells, self.ndofs // (self.ncells * (self.dim + 2)), 3))
+        self._chk(load().dgx_get_grid_nodes(self.h, _dp(x)), "grid_nodes")
+        return x
+
+    def metric_terms(self):
+        """(C [cell][node][3][3] with C[m][n] = J dxi_n/dx_m, det J [cell][node]) of a curvilinear grid."""
+        d = self.dim
+        npn = self.ndofs // (self.ncells * (d + 2))
+        raw = np.empty((self.ncells, d * d + 1, npn))
+        self._chk(load().dgx_get_metric_terms(self.h, _dp(raw)), "metric_terms")
+        C = np.zeros((self.ncells, npn, 3, 3))
+        for m in range(d):
+            for n in range(d):
+                C[:, :, m, n] = raw[:, m * d + n, :]
+        return C, raw[:, d * d, :].copy()
 
     def allocate_ode_system(self, method):
         m = METHODS[method] if isinstance(method, str) else method

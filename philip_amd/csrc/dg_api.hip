@@ -263,6 +263,7 @@ void dgx_default_params(dgx_params *p)
     p->precision = DGX_FP64;
     for (int f = 0; f < 6; ++f) p->boundary_type[f] = 0;
     p->angle_of_attack = 0.0;
+    p->grid_warp = 0;
 }
 
 // DGFactory::create_discontinuous_galerkin + allocate_system: operators, device buffers (from
@@ -290,6 +291,8 @@ static int create_ctx(const dgx_params *p, dgx::Arena *arena, hipStream_t stream
         if (t && (t < 1001 || t > 1007)) return bad(DGX_ERR_UNSUPPORTED, "boundary types 1001..1007 only");
         if (t && p->pde != DGX_EULER) return bad(DGX_ERR_UNSUPPORTED, "boundaries on the Euler path only");
     }
+    if (p->grid_warp != 0 && p->grid_warp != 1) return bad(DGX_ERR_INVALID, "grid_warp is 0 or 1");
+    if (p->grid_warp && p->pde != DGX_EULER) return bad(DGX_ERR_UNSUPPORTED, "curvilinear grids on the Euler path only");
     dgx_ctx *c = new dgx_ctx();
     c->P = *p;
     c->arena = arena;
@@ -349,6 +352,18 @@ static int create_ctx(const dgx_params *p, dgx::Arena *arena, hipStream_t stream
         dgx_destroy(c);
         return bad(DGX_ERR_HIP, "device set-up failed");
     }
+    if (p->grid_warp) {
+        // the reference's curved periodic grid: node coordinates and metric terms on the device
+        c->xn_dev = static_cast<double *>(dev_alloc(c, sizeof(double) * 3 * (size_t)c->ncells * c->Np));
+        c->met_dev = static_cast<double *>(
+            dev_alloc(c, sizeof(double) * (size_t)(c->dim * c->dim + 1) * c->ncells * c->Np));
+        if (!c->xn_dev || !c->met_dev || launch_coords(c, c->xn_dev, 1) != DGX_OK || launch_metrics(c) != DGX_OK) {
+            c->stream = nullptr;
+            dgx_destroy(c);
+            return arena ? DGX_ERR_STATE : bad(DGX_ERR_HIP, "curvilinear grid set-up failed");
+        }
+        c->curv = true;
+    }
     *out = c;
     return DGX_OK;
 }
@@ -374,6 +389,8 @@ int dgx_destroy(dgx_ctx *c)
     dev_free(c, c->u_dev);
     dev_free(c, c->r_dev);
     dev_free(c, c->status_dev);
+    dev_free(c, c->xn_dev);
+    dev_free(c, c->met_dev);
     for (int k = 0; k < 3; ++k) {
         dev_free(c, c->aux_dev[k]);
         dev_free(c, c->auxf_dev[k]);
@@ -448,6 +465,64 @@ int dgx_get_neighbors(const dgx_ctx *cc, int32_t *host_nbr)
     return DGX_OK;
 }
 
+int dgx_set_grid_nodes(dgx_ctx *c, const double *host_xyz)
+{
+    if (!c || !host_xyz) return DGX_ERR_INVALID;
+    if (c->P.pde != DGX_EULER) {
+        c->err = "curvilinear grids on the Euler path only";
+        return DGX_ERR_UNSUPPORTED;
+    }
+    if (c->batch_locked) {
+        c->err = "the context is captured in a batch graph";
+        return DGX_ERR_STATE;
+    }
+    const size_t nx = 3 * (size_t)c->ncells * c->Np, nm = (size_t)(c->dim * c->dim + 1) * c->ncells * c->Np;
+    if (!c->xn_dev) c->xn_dev = static_cast<double *>(dev_alloc(c, sizeof(double) * nx));
+    if (!c->met_dev) c->met_dev = static_cast<double *>(dev_alloc(c, sizeof(double) * nm));
+    if (!c->xn_dev || !c->met_dev) {
+        c->err = "device allocation failed";
+        return DGX_ERR_HIP;
+    }
+    HIPCHK(hipMemcpyAsync(c->xn_dev, host_xyz, sizeof(double) * nx, hipMemcpyHostToDevice, c->stream));
+    const int rc = launch_metrics(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->curv = true;
+    return DGX_OK;
+}
+
+int dgx_get_grid_nodes(dgx_ctx *c, double *host_xyz)
+{
+    if (!c || !host_xyz) return DGX_ERR_INVALID;
+    const size_t nx = 3 * (size_t)c->ncells * c->Np;
+    if (!c->curv) {  // the affine box: computed into a temporary buffer
+        double *tmp = nullptr;
+        HIPCHK(hipMalloc(&tmp, sizeof(double) * nx));
+        int rc = launch_coords(c, tmp, 0);
+        if (!rc && hipMemcpyAsync(host_xyz, tmp, sizeof(double) * nx, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            rc = DGX_ERR_HIP;
+        if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = DGX_ERR_HIP;
+        (void)hipFree(tmp);
+        return rc;
+    }
+    HIPCHK(hipMemcpyAsync(host_xyz, c->xn_dev, sizeof(double) * nx, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DGX_OK;
+}
+
+int dgx_get_metric_terms(dgx_ctx *c, double *host_met)
+{
+    if (!c || !host_met) return DGX_ERR_INVALID;
+    if (!c->curv) {
+        c->err = "no curvilinear grid: the metric of the affine box is h^(dim-1) I, det J = h^dim";
+        return DGX_ERR_STATE;
+    }
+    HIPCHK(hipMemcpyAsync(host_met, c->met_dev, sizeof(double) * (size_t)(c->dim * c->dim + 1) * c->ncells * c->Np,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DGX_OK;
+}
+
 static StageArgs base_args(dgx_ctx *c, const double *u)
 {
     StageArgs a{};
@@ -464,6 +539,7 @@ static StageArgs base_args(dgx_ctx *c, const double *u)
     a.penalty = 0.5 * (pen + pen);
     a.resid_scale = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
     a.status = c->status_dev;
+    a.met = c->curv ? c->met_dev : nullptr;
     return a;
 }
 
@@ -1075,15 +1151,24 @@ int dgx_set_initial_condition_vortex(dgx_ctx *c, double strength, double u_inf, 
     return DGX_OK;
 }
 
+static int curv_unsupported(dgx_ctx *c)
+{
+    c->err = "integrated quantities on a curvilinear grid need the metric at the over-integration points: "
+             "not built";
+    return DGX_ERR_UNSUPPORTED;
+}
+
 int dgx_integrated_quantities(dgx_ctx *c, int overint, double *out4)
 {
     if (!c || !out4) return DGX_ERR_INVALID;
+    if (c->curv) return curv_unsupported(c);
     return launch_integrated(c, c->u_dev, overint, out4);
 }
 
 int dgx_integrated_quantities_static(dgx_ctx *c, const double *u_dev, int overint, double *out4_dev)
 {
     if (!c || !out4_dev) return DGX_ERR_INVALID;
+    if (c->curv) return curv_unsupported(c);
     return launch_integrated_static(c, u_dev ? u_dev : c->u_dev, overint, out4_dev);
 }
 
@@ -1230,7 +1315,7 @@ static void batch_free_groups(dgx_batch *b)
 
 static bool batch_line_eligible(const dgx_ctx *c)
 {
-    return c->P.use_split_form && c->P.pde == DGX_EULER && !c->has_boundary;
+    return c->P.use_split_form && c->P.pde == DGX_EULER && !c->has_boundary && !c->curv;
 }
 
 // block table of a group for `epb` elements per block: (slot, first element) per element group

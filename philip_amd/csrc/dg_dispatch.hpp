@@ -32,6 +32,15 @@ void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
     const bool bnd = c->has_boundary;  // Euler only
     using LG = LineGeoS<D, N, T>;
     const dim3 lgrid((c->ncells + LG::EPB - 1) / LG::EPB), lblock(LG::BLOCK);
+    if (c->curv) {  // curvilinear grid (Euler): stage2_kernel with the metric terms, boundaries included
+        if (split)
+            hipLaunchKernelGGL((stage2_kernel<D, N, true, false, T, true, true>), grid, block, 0, c->stream, a, c->ops,
+                               c->pc);
+        else
+            hipLaunchKernelGGL((stage2_kernel<D, N, false, false, T, true, true>), grid, block, 0, c->stream, a,
+                               c->ops, c->pc);
+        return;
+    }
     if constexpr (D == 3) {
         if (split && nsv) {
             // mixed: line kernel with the viscous terms; FP64: stage2_kernel (the FP64 line

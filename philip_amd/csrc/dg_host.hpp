@@ -64,6 +64,9 @@ struct dgx_ctx {
     dgx::PhysConst pc{};
     int64_t nbr_count = 0;         // ncells * 2 * dim
     bool has_boundary = false;     // some box face is not periodic (nbr < 0 there: -boundary type)
+    bool curv = false;             // curvilinear grid: metric terms per node in met_dev
+    double *xn_dev = nullptr;      // node coordinates [cell][node][3] (allocated on demand)
+    double *met_dev = nullptr;     // [cell][dim*dim + 1][node]: cofactor C[m][n] (slot m*dim+n), det J
     int *nbr_dev = nullptr;
     double *u_dev = nullptr;       // DGBase::solution
     double *r_dev = nullptr;       // DGBase::right_hand_side
@@ -110,6 +113,11 @@ int launch_stage(dgx_ctx *c, const StageArgs &a);
 int launch_inverse_mass(dgx_ctx *c, const double *in, double *out);
 int launch_fill(dgx_ctx *c, double *out, int64_t n, double value);
 int launch_mesh(dgx_ctx *c);
+// node coordinates of the box's cells into xn [cell][node][3]; warp = 1: through the reference's
+// curved periodic grid mapping
+int launch_coords(dgx_ctx *c, double *xn, int warp);
+// metric terms of c->xn_dev into c->met_dev
+int launch_metrics(dgx_ctx *c);
 int launch_ic(dgx_ctx *c, int kind, int isothermal, double strength, double u_inf, double v_inf);
 int launch_maxabs(dgx_ctx *c, const double *a, const double *b, int64_t n, double *out);
 int launch_axpy(dgx_ctx *c, const double *x, double h, const double *f, double *out, int64_t n);

@@ -63,6 +63,9 @@ struct StageArgs {
     // set to 1 by any thread that finds a non-physical stage state (density or pressure
     // negative or NaN; euler.cpp:155-173, 486-500); read and cleared by the host
     int *status;
+    // curvilinear grid: [cell][dim*dim + 1][node], the metric cofactor C[m][n] = J dxi_n/dx_m in
+    // slot m*dim + n and det J in slot dim*dim; null on the affine box
+    const double *met;
 };
 
 // Non-physical state check of one node (the reference checks every density and pressure it
@@ -140,21 +143,26 @@ __device__ __forceinline__ void st_stream(double *p, double v)
 // Phase E of the stage kernels, in FP64: du/dt = acc / h and the output of the
 // selected mode (residual, lifted right-hand side, low-storage RK 3S* register
 // update, Butcher-tableau stage update, or the FSAL error-register accumulation).
+// jac > 0: the node's det J on a curvilinear grid (acc then holds the cofactor-weighted terms):
+// du/dt = acc / J and the un-lifted residual W J du/dt.
 template <int D, int N, typename T>
 __device__ __forceinline__ void stage_epilogue(const StageArgs &a, const Ops1D &ops, int e, int i, const int *c,
                                                const double *ud, const T *acc, bool use_pf = false,
-                                               const double *pf2 = nullptr, const double *pf3 = nullptr)
+                                               const double *pf2 = nullptr, const double *pf3 = nullptr,
+                                               double jac = 0.0)
 {
     constexpr int NP = Shape<D, N>::NP, NS = D + 2;
+    const double inv_h = jac > 0.0 ? 1.0 / jac : a.inv_h;
+    const double resid_scale = jac > 0.0 ? jac : a.resid_scale;
     double dudt[NS];
-    for (int s = 0; s < NS; ++s) dudt[s] = double(acc[s]) * a.inv_h;
+    for (int s = 0; s < NS; ++s) dudt[s] = double(acc[s]) * inv_h;
     const long base = (long)e * NS * NP + i;
     if (a.mode == MODE_RHS) {
         for (int s = 0; s < NS; ++s) a.out[base + s * NP] = dudt[s];
     } else if (a.mode == MODE_RESIDUAL) {
         double W = ops.w[c[0]] * ops.w[c[1]];
         if (D == 3) W *= ops.w[c[2]];
-        for (int s = 0; s < NS; ++s) a.out[base + s * NP] = dudt[s] * (W * a.resid_scale);
+        for (int s = 0; s < NS; ++s) a.out[base + s * NP] = dudt[s] * (W * resid_scale);
     } else if (a.mode == MODE_ACCUM) {
         for (int s = 0; s < NS; ++s) {
             const double r = dudt[s] * a.dt;

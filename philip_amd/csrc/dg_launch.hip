@@ -58,19 +58,27 @@ int launch_line_batch(int dim, int n, int precision, int tile, const BatchBlock 
 #undef LB_CASE
 }
 
+// det J of node i of cell e: the scalar of the affine box, or slot dim*dim of the curvilinear
+// metric [cell][dim*dim + 1][node]
+__device__ __forceinline__ double node_jac(const double *met, double detJ, int dim, int Np, long e, int i)
+{
+    return met ? met[(e * (dim * dim + 1) + dim * dim) * Np + i] : detJ;
+}
+
 // Inverse of the collocated GLL mass matrix (diagonal): out_q = in_q / (W_q det J),
-// W_q the tensor-product GLL weight of the node (dg_base.cpp:4089-4260).
+// W_q the tensor-product GLL weight of the node (dg_base.cpp:4089-4260; on a curvilinear grid the
+// weight-adjusted inverse, dg_base.cpp:4220-4247, is the same diagonal for collocated nodes).
 __global__ void __launch_bounds__(256)
 inverse_mass_kernel(const double *__restrict__ in, double *__restrict__ out, long ndofs, int n, int dim,
-                    double detJ, Ops1D ops)
+                    double detJ, Ops1D ops, const double *__restrict__ met)
 {
-    const int Np = (dim == 3) ? n * n * n : n * n;
+    const int Np = (dim == 3) ? n * n * n : n * n, ns = dim + 2;
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < ndofs; q += (long)gridDim.x * blockDim.x) {
         const int node = (int)(q % Np);
         const int a = node % n, b = (node / n) % n, c = node / (n * n);
         double W = ops.w[a] * ops.w[b];
         if (dim == 3) W *= ops.w[c];
-        out[q] = in[q] / (W * detJ);
+        out[q] = in[q] / (W * node_jac(met, detJ, dim, Np, q / ((long)ns * Np), node));
     }
 }
 
@@ -271,7 +279,9 @@ struct IcDesc {
 };
 
 // initial data interpolated at the GLL support points of each cell, one thread per node
-__global__ void __launch_bounds__(256) ic_kernel(double *__restrict__ u, int ncells, int n, MeshDesc m, IcDesc ic)
+// xn: node coordinates of a curvilinear grid [cell][node][3] (null: the affine box)
+__global__ void __launch_bounds__(256) ic_kernel(double *__restrict__ u, int ncells, int n, MeshDesc m, IcDesc ic,
+                                                 const double *__restrict__ xn)
 {
     const int Np = (m.dim == 3) ? n * n * n : n * n, ns = m.dim + 2;
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -282,6 +292,8 @@ __global__ void __launch_bounds__(256) ic_kernel(double *__restrict__ u, int nce
     const int id[3] = {i % n, (i / n) % n, i / (n * n)};
     double x[3];
     for (int k = 0; k < 3; ++k) x[k] = (k < m.dim) ? ic.left + (c[k] + ic.xs[id[k]]) * ic.h : 0.0;
+    if (xn)
+        for (int k = 0; k < 3; ++k) x[k] = xn[t * 3 + k];
     double cons[5];
     if (ic.kind == 0) {
         // InitialConditionFunction_TaylorGreenVortex[_Isothermal] (initial_condition_function.cpp:261-336)
@@ -348,8 +360,142 @@ int launch_ic(dgx_ctx *c, int kind, int isothermal, double strength, double u_in
     gauss_lobatto(c->n, ic.xs, ws);
     const long n = (long)c->ncells * c->Np;
     hipLaunchKernelGGL(ic_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->u_dev, c->ncells,
-                       c->n, mesh_desc(c), ic);
+                       c->n, mesh_desc(c), ic, c->curv ? c->xn_dev : nullptr);
     return check_launch(c, "ic_kernel");
+}
+
+// Node coordinates of the box's cells [cell][node][3] (the affine map of the GLL support points),
+// optionally through the reference's curved periodic grid NonsymmetricCurvedGridManifold::mapping
+// (nonsymmetric_curved_periodic_grid.cpp:66-87, beta = 0.1).
+__global__ void __launch_bounds__(256) coords_kernel(double *__restrict__ xn, int ncells, int n, MeshDesc m,
+                                                     IcDesc ic, int warp)
+{
+    const int Np = (m.dim == 3) ? n * n * n : n * n;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)ncells * Np) return;
+    const int e = (int)(t / Np), i = (int)(t % Np);
+    int c[3];
+    cell_coords(m, e, c);
+    const int id[3] = {i % n, (i / n) % n, i / (n * n)};
+    double x[3];
+    for (int k = 0; k < 3; ++k) x[k] = (k < m.dim) ? ic.left + (c[k] + ic.xs[id[k]]) * ic.h : 0.0;
+    if (warp) {
+        const double beta = 1.0 / 10.0, pi = 3.14159265358979323846;
+        const double p0 = x[0], p1 = x[1], p2 = x[2];
+        if (m.dim == 2) {
+            x[0] = p0 + beta * cos(pi / 2.0 * p0) * cos(3.0 * pi / 2.0 * p1);
+            x[1] = p1 + beta * sin(2.0 * pi * p0) * cos(pi / 2.0 * p1);
+        } else {
+            x[0] = p0 + beta * sin(p0) * sin(p1) * sin(2.0 * p2);
+            x[1] = p1 + beta * sin(4.0 * p0) * sin(p1) * sin(3.0 * p2);
+            x[2] = p2 + beta * sin(2.0 * p0) * sin(5.0 * p1) * sin(p2);
+        }
+    }
+    for (int k = 0; k < 3; ++k) xn[t * 3 + k] = x[k];
+}
+
+// Metric terms at the GLL nodes from the node coordinates (mapping support points of grid degree
+// p at the same nodes): Jacobian dX_a/dxi_b by the derivative operator along the node's lines,
+// det J, and the cofactor C[m][n] = J dxi_n/dx_m -- in 2D from the Jacobian, in 3D by the
+// conservative curl form of metric_operators::compute_local_3D_cofactor (operators.cpp:2737-2884):
+// C[r][0] = d_zeta(P_eta) - d_eta(P_zeta), C[r][1] = d_xi(P_zeta) - d_zeta(P_xi),
+// C[r][2] = d_eta(P_xi) - d_xi(P_eta) with P_j = X_l dX_m/dxi_j for (l, m) = (z, y), (x, z), (y, x),
+// which satisfies the discrete metric identities.  Output [cell][dim*dim + 1][node].
+template <int D, int N>
+__global__ void __launch_bounds__(512) metric_kernel(const double *__restrict__ xn, double *__restrict__ met,
+                                                     int ncells, Ops1D ops)
+{
+    constexpr int NP = (D == 3) ? N * N * N : N * N;
+    constexpr int EPB = (NP >= 256) ? 1 : (256 / NP);
+    constexpr int MS = D * D + 1;
+    __shared__ double sX[EPB][D][NP];
+    __shared__ double sPr[EPB][D == 3 ? 9 : 1][NP];
+    const int le = threadIdx.x / NP, i = threadIdx.x % NP;
+    const long e = (long)blockIdx.x * EPB + le;
+    const bool active = le < EPB && e < ncells;
+    if (active)
+        for (int k = 0; k < D; ++k) sX[le][k][i] = xn[(e * NP + i) * 3 + k];
+    __syncthreads();
+    int c[3] = {0, 0, 0};
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if (active) {
+        node_coords<D, N>(i, c);
+        for (int a = 0; a < D; ++a)
+            for (int b = 0; b < D; ++b) {
+                double acc = 0.0;
+                for (int l = 0; l < N; ++l) acc += ops.D[c[b]][l] * sX[le][a][node_with<D, N>(i, b, l)];
+                J[a][b] = acc;
+            }
+    }
+    double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if constexpr (D == 2) {
+        C[0][0] = J[1][1];
+        C[1][0] = -J[0][1];
+        C[0][1] = -J[1][0];
+        C[1][1] = J[0][0];
+    } else {
+        constexpr int LM[3][2] = {{2, 1}, {0, 2}, {1, 0}};
+        if (active)
+            for (int r = 0; r < 3; ++r)
+                for (int j = 0; j < 3; ++j) sPr[le][r * 3 + j][i] = sX[le][LM[r][0]][i] * J[LM[r][1]][j];
+        __syncthreads();
+        if (active) {
+            auto dref = [&](int slot, int dir) {
+                double acc = 0.0;
+                for (int l = 0; l < N; ++l) acc += ops.D[c[dir]][l] * sPr[le][slot][node_with<D, N>(i, dir, l)];
+                return acc;
+            };
+            for (int r = 0; r < 3; ++r) {
+                C[r][0] = -dref(r * 3 + 2, 1) + dref(r * 3 + 1, 2);
+                C[r][1] = dref(r * 3 + 2, 0) - dref(r * 3 + 0, 2);
+                C[r][2] = -dref(r * 3 + 1, 0) + dref(r * 3 + 0, 1);
+            }
+        }
+    }
+    if (!active) return;
+    double det;
+    if (D == 2) det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    else det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    for (int m = 0; m < D; ++m)
+        for (int nn = 0; nn < D; ++nn) met[(e * MS + m * D + nn) * NP + i] = C[m][nn];
+    met[(e * MS + D * D) * NP + i] = det;
+}
+
+int launch_coords(dgx_ctx *c, double *xn, int warp)
+{
+    IcDesc ic{};
+    ic.left = c->P.domain_left;
+    ic.h = c->h;
+    double ws[MAXN1D];
+    gauss_lobatto(c->n, ic.xs, ws);
+    const long n = (long)c->ncells * c->Np;
+    hipLaunchKernelGGL(coords_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xn, c->ncells, c->n,
+                       mesh_desc(c), ic, warp);
+    return check_launch(c, "coords_kernel");
+}
+
+template <int D, int N> static void metric_dn(dgx_ctx *c)
+{
+    constexpr int NP = (D == 3) ? N * N * N : N * N;
+    constexpr int EPB = (NP >= 256) ? 1 : (256 / NP);
+    hipLaunchKernelGGL((metric_kernel<D, N>), dim3((c->ncells + EPB - 1) / EPB), dim3(EPB * NP <= 256 ? 256 : EPB * NP),
+                       0, c->stream, c->xn_dev, c->met_dev, c->ncells, c->ops);
+}
+
+int launch_metrics(dgx_ctx *c)
+{
+#define MET_CASE(NN)                                                                                        \
+    case NN:                                                                                                \
+        if (c->dim == 3) metric_dn<3, NN>(c);                                                               \
+        else metric_dn<2, NN>(c);                                                                           \
+        break;
+    switch (c->n) {
+        MET_CASE(2) MET_CASE(3) MET_CASE(4) MET_CASE(5) MET_CASE(6) MET_CASE(7)
+    default: c->err = "no metric kernel for this degree"; return DGX_ERR_UNSUPPORTED;
+    }
+#undef MET_CASE
+    return check_launch(c, "metric_kernel");
 }
 
 int launch_inverse_mass(dgx_ctx *c, const double *in, double *out)
@@ -357,7 +503,7 @@ int launch_inverse_mass(dgx_ctx *c, const double *in, double *out)
     const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
     const long blocks = std::min<long>((c->ndofs + 255) / 256, 4096);
     hipLaunchKernelGGL(inverse_mass_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, in, out, (long)c->ndofs,
-                       c->n, c->dim, detJ, c->ops);
+                       c->n, c->dim, detJ, c->ops, c->curv ? c->met_dev : nullptr);
     return check_launch(c, "inverse_mass_kernel");
 }
 
@@ -602,7 +748,8 @@ max_reduce_kernel(const double *__restrict__ v, int n, double *__restrict__ out)
 // partial slots of a cell hold (entropy, 0, 0) so the fixed-order reduction below applies.
 __global__ void __launch_bounds__(IQ_THREADS)
 entropy_kernel(const double *__restrict__ u, const double *__restrict__ dir, double gamma, int n, int dim,
-               int ncells, double detJ, double gam, Ops1D ops, double *__restrict__ partial)
+               int ncells, double detJ, double gam, Ops1D ops, double *__restrict__ partial,
+               const double *__restrict__ met)
 {
     __shared__ double red[IQ_THREADS];
     const int e = blockIdx.x;
@@ -625,7 +772,7 @@ entropy_kernel(const double *__restrict__ u, const double *__restrict__ dir, dou
         const int a = i % n, b = (i / n) % n, c = i / (n * n);
         double W = ops.w[a] * ops.w[b];
         if (dim == 3) W *= ops.w[c];
-        acc += (-rho * ent) * W * detJ;
+        acc += (-rho * ent) * W * node_jac(met, detJ, dim, Np, e, i);
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -668,7 +815,7 @@ int launch_entropy(dgx_ctx *c, const double *u, double *out) { return launch_ent
 template <int D>
 __global__ void __launch_bounds__(VR_THREADS)
 rrk_estimate_kernel(const double *__restrict__ u, const double *__restrict__ k, int n, int ncells, double detJ,
-                    Ops1D ops, PhysConst pc, double *__restrict__ partial)
+                    Ops1D ops, PhysConst pc, double *__restrict__ partial, const double *__restrict__ met)
 {
     constexpr int NS = D + 2;
     __shared__ double red[VR_THREADS];
@@ -683,7 +830,8 @@ rrk_estimate_kernel(const double *__restrict__ u, const double *__restrict__ k, 
         entropy_variables<double, D>(us, pc, v);
         double W = ops.w[i % n] * ops.w[(i / n) % n];
         if (D == 3) W *= ops.w[i / (n * n)];
-        for (int s = 0; s < NS; ++s) acc += v[s] * (W * detJ * k[(e * NS + s) * Np + i]);
+        const double jac = node_jac(met, detJ, D, Np, e, i);
+        for (int s = 0; s < NS; ++s) acc += v[s] * (W * jac * k[(e * NS + s) * Np + i]);
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -699,10 +847,10 @@ int launch_rrk_estimate(dgx_ctx *c, const double *u, const double *k, double *pa
     const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
     if (c->dim == 3)
         hipLaunchKernelGGL(rrk_estimate_kernel<3>, dim3(VR_BLOCKS), dim3(VR_THREADS), 0, c->stream, u, k, c->n,
-                           c->ncells, detJ, c->ops, c->pc, partial_dev);
+                           c->ncells, detJ, c->ops, c->pc, partial_dev, c->curv ? c->met_dev : nullptr);
     else
         hipLaunchKernelGGL(rrk_estimate_kernel<2>, dim3(VR_BLOCKS), dim3(VR_THREADS), 0, c->stream, u, k, c->n,
-                           c->ncells, detJ, c->ops, c->pc, partial_dev);
+                           c->ncells, detJ, c->ops, c->pc, partial_dev, c->curv ? c->met_dev : nullptr);
     return check_launch(c, "rrk_estimate_kernel");
 }
 
@@ -731,7 +879,7 @@ int launch_entropy_combo(dgx_ctx *c, const double *u, const double *dir, double 
     }
     const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
     hipLaunchKernelGGL(entropy_kernel, dim3(c->ncells), dim3(IQ_THREADS), 0, c->stream, u, dir, gamma, c->n, c->dim,
-                       c->ncells, detJ, c->P.gamma_gas, c->ops, c->iq_partial);
+                       c->ncells, detJ, c->P.gamma_gas, c->ops, c->iq_partial, c->curv ? c->met_dev : nullptr);
     int rc = check_launch(c, "entropy_kernel");
     if (rc) return rc;
     double *sums = c->iq_partial + 3 * (size_t)c->ncells;

@@ -211,6 +211,107 @@ __device__ __forceinline__ void boundary_state(int type, const T *ui, int K, T s
     (void)NS;
 }
 
+// The same boundary states on a face of a curvilinear grid, n the outward unit normal (the
+// reference's general form, euler.cpp:1097-1460); written like boundary_state with scalar,
+// by-value selections.
+template <typename T, int D>
+__device__ __forceinline__ void boundary_state_n(int type, const T *ui, const T *n, const PhysConst &pc, T *ub)
+{
+    const T gam = T(pc.gam), gamm1 = T(pc.gamm1), m2 = T(pc.mach2);
+    const T pinf = T(1) / (gam * m2);
+    const T rho_i = ui[0];
+    T vel_i[D], v2 = T(0), vn_i = T(0), vn_e = T(0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        vel_i[k] = ui[1 + k] / rho_i;
+        v2 += vel_i[k] * vel_i[k];
+        vn_i += vel_i[k] * n[k];
+        vn_e += T(pc.vinf[k]) * n[k];
+    }
+    const T p_i = gamm1 * (ui[D + 1] - T(0.5) * rho_i * v2);
+    T rho_b = rho_i, p_b = p_i, vel_b[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) vel_b[k] = vel_i[k];
+    bool copy = false;
+    if (type == 1001 || type == 1006) {
+        // v - 2 (v.n) n
+#pragma unroll
+        for (int k = 0; k < D; ++k) vel_b[k] = vel_i[k] - T(2) * vn_i * n[k];
+    } else if (type == 1002) {
+        const T mach = dsqrt<T>(v2) / dsqrt<T>(gam * p_i / rho_i);
+        if (mach > T(1)) {
+            copy = true;
+        } else {
+            const T rad = T(1) + T(0.5) * gamm1 * m2;
+            const T p_inlet = (pinf * dpow<T>(rad, gam / gamm1)) * dpow<T>(rad, -gam / gamm1);
+            p_b = (mach >= T(1)) ? p_i : T(0.99) * p_inlet;
+            const T Ti = gam * m2 * (p_i / rho_i);
+            rho_b = gam * m2 * (p_b / Ti);
+        }
+    } else if (type == 1003) {
+        const T p0 = pinf * dpow<T>(T(1) + T(0.5) * gamm1 * m2, gam / gamm1);
+        const T T0 = dpow<T>(p0 / pinf, gamm1 / gam);
+        T speed;
+        if (m2 < T(1)) {
+            const T snd = dsqrt<T>(gam * p_i / rho_i);
+            const T rp = -vn_i + T(2) * snd / gamm1;
+            const T H = ui[D + 1] / rho_i + p_i / rho_i;
+            const T A = T(1) + T(2) / gamm1, B = T(-2) * rp, C = T(0.5) * gamm1 * (rp * rp - T(2) * H);
+            const T disc = dsqrt<T>(B * B - T(4) * A * C);
+            const T c1 = (-B + disc) / (T(2) * A), c2 = (-B - disc) / (T(2) * A);
+            const T cb = c1 > c2 ? c1 : c2;
+            const T vb = rp - T(2) * cb / gamm1;
+            const T mb = vb / cb;
+            const T rad = T(1) + T(0.5) * gamm1 * mb * mb;
+            p_b = p0 * dpow<T>(rad, -gam / gamm1);
+            rho_b = gam * m2 * (p_b / (T0 / rad));
+            speed = vb;
+        } else {
+            const T rad = T(1) + T(0.5) * gamm1 * m2;
+            p_b = p0 * dpow<T>(rad, -gam / gamm1);
+            rho_b = gam * m2 * (p_b / (T0 / rad));
+            speed = dsqrt<T>(m2) * dsqrt<T>(gam * p_b / rho_b);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) vel_b[k] = speed * (-n[k]);
+    } else if (type == 1004) {
+        const T ci = dsqrt<T>(gam * p_i / rho_i), ce = dsqrt<T>(gam * pinf);
+        const T Rout = vn_i + T(2) / gamm1 * ci, Rinc = vn_e - T(2) / gamm1 * ce;
+        const T vnb = T(0.5) * (Rout + Rinc), cb = T(0.25) * gamm1 * (Rout - Rinc);
+        const bool inlet = vnb < T(0);
+        const T rho_s = inlet ? T(1) : rho_i, p_s = inlet ? pinf : p_i, vn_s = inlet ? vn_e : vn_i;
+        T vel_s[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) vel_s[k] = inlet ? T(pc.vinf[k]) : vel_i[k];
+        if (dabs<T>(vnb) >= dabs<T>(cb)) {
+            rho_b = rho_s;
+            p_b = p_s;
+#pragma unroll
+            for (int k = 0; k < D; ++k) vel_b[k] = vel_s[k];
+        } else {
+            const T ent = p_s * dpow<T>(rho_s, -gam);
+            rho_b = dpow<T>(T(1) / gam * cb * cb / ent, T(1) / gamm1);
+#pragma unroll
+            for (int k = 0; k < D; ++k) vel_b[k] = (vel_s[k] - vn_s * n[k]) + vnb * n[k];
+            p_b = T(1) / gam * cb * cb * rho_b;
+        }
+    } else if (type == 1005) {
+        rho_b = T(1);
+        p_b = pinf;
+#pragma unroll
+        for (int k = 0; k < D; ++k) vel_b[k] = T(pc.vinf[k]);
+    } else {
+        copy = true;
+    }
+    T w2 = T(0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) w2 += vel_b[k] * vel_b[k];
+    ub[0] = copy ? ui[0] : rho_b;
+#pragma unroll
+    for (int k = 0; k < D; ++k) ub[1 + k] = copy ? ui[1 + k] : rho_b * vel_b[k];
+    ub[D + 1] = copy ? ui[D + 1] : p_b / gamm1 + T(0.5) * rho_b * w2;
+}
+
 template <typename T, int D>
 __device__ __forceinline__ Prim<T, D> to_prim(const T *u, T gamm1)
 {
@@ -290,16 +391,13 @@ __device__ __forceinline__ void two_point_flux_dir(const T *u1, const Prim<T, D>
     }
 }
 
-// Roe-Pike dissipation with Harten's entropy fix along the unit normal n = sgn * e_fd
+// Roe-Pike dissipation with Harten's entropy fix along the unit normal n
 // (convective_numerical_flux.cpp:216-232, :332-480)
 template <typename T, int D>
-__device__ __forceinline__ void roe_dissipation(const T *ui, const Prim<T, D> &L, const T *ue,
-                                                const Prim<T, D> &R, int fd, T sgn, const PhysConst &pc,
-                                                T *out)
+__device__ __forceinline__ void roe_dissipation_n(const T *ui, const Prim<T, D> &L, const T *ue,
+                                                  const Prim<T, D> &R, const T *n, const PhysConst &pc, T *out)
 {
     const T gam = T(pc.gam), gamm1 = T(pc.gamm1);
-    T n[D];
-    for (int k = 0; k < D; ++k) n[k] = (k == fd) ? sgn : T(0);
     T vnL = T(0), vnR = T(0);
     for (int k = 0; k < D; ++k) vnL += L.vel[k] * n[k];
     for (int k = 0; k < D; ++k) vnR += R.vel[k] * n[k];
@@ -351,6 +449,17 @@ __device__ __forceinline__ void roe_dissipation(const T *ui, const Prim<T, D> &L
     for (int s = 0; s < D + 2; ++s) out[s] = T(-0.5) * A[s];
 }
 
+// the same along the unit normal sgn * e_fd
+template <typename T, int D>
+__device__ __forceinline__ void roe_dissipation(const T *ui, const Prim<T, D> &L, const T *ue,
+                                                const Prim<T, D> &R, int fd, T sgn, const PhysConst &pc,
+                                                T *out)
+{
+    T n[D];
+    for (int k = 0; k < D; ++k) n[k] = (k == fd) ? sgn : T(0);
+    roe_dissipation_n<T, D>(ui, L, ue, R, n, pc, out);
+}
+
 // NumericalFluxConvective::evaluate_flux = baseline + dissipation (convective_numerical_flux.cpp:34-54)
 // with the unit normal sgn * e_fd.
 template <typename T, int D>
@@ -383,6 +492,104 @@ __device__ __forceinline__ void numerical_flux(const T *ui, const Prim<T, D> &L,
         for (int s = 0; s < D + 2; ++s) diss[s] = T(-0.5) * lam * (ue[s] - ui[s]);
     }
     if (cf == 1 || cf == 5) roe_dissipation<T, D>(ui, L, ue, R, fd, sgn, pc, diss);
+    for (int s = 0; s < D + 2; ++s) out[s] = base[s] + diss[s];
+}
+
+// Curvilinear grids: the two-point flux contracted with a direction vector (a metric-cofactor
+// column or its two-point average, not necessarily unit): sum_m nv[m] F#_m (the contraction of
+// transform_physical_to_reference, operators.cpp:2354-2365).
+template <typename T, int D>
+__device__ __forceinline__ void two_point_flux_vec(const T *u1, const Prim<T, D> &q1, const T *u2,
+                                                   const Prim<T, D> &q2, const T *nv, const PhysConst &pc, T *f)
+{
+    if (pc.two_point == 2 || pc.two_point == 3) {
+        // Chandrashekar / Ranocha: rho_ln (vbar.n) (1, vbar, H) + p_hat (0, n, 0) [- Ranocha's
+        // pressure-work term], the components of two_point_flux_dir contracted in one evaluation
+        const T gamm1 = T(pc.gamm1);
+        const bool ranocha = (pc.two_point == 3);
+        const T rho_log = log_mean<T>(q1.rho, q2.rho);
+        const T b1 = ranocha ? fdiv<T>(q1.rho, q1.p) : fdiv<T>(q1.rho, T(2) * q1.p);
+        const T b2 = ranocha ? fdiv<T>(q2.rho, q2.p) : fdiv<T>(q2.rho, T(2) * q2.p);
+        const T beta_log = log_mean<T>(b1, b2);
+        const T p_hat = ranocha ? T(0.5) * (q1.p + q2.p)
+                                : fdiv<T>(T(0.5) * (q1.rho + q2.rho), T(2) * T(0.5) * (b1 + b2));
+        T va[D], vsq = T(0), vn = T(0), w1 = T(0), w2 = T(0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            va[k] = T(0.5) * (q1.vel[k] + q2.vel[k]);
+            vsq += va[k] * va[k];
+            vn += va[k] * nv[k];
+            w1 += q1.vel[k] * nv[k];
+            w2 += q2.vel[k] * nv[k];
+        }
+        T H = ranocha ? fdiv<T>(T(1), beta_log * gamm1) + vsq + fdiv<T>(T(2) * p_hat, rho_log)
+                      : fdiv<T>(T(1), T(2) * beta_log * gamm1) + vsq + fdiv<T>(p_hat, rho_log);
+        for (int k = 0; k < D; ++k) H -= T(0.5) * (T(0.5) * (q1.vel[k] * q1.vel[k] + q2.vel[k] * q2.vel[k]));
+        const T m = rho_log * vn;
+        f[0] = m;
+#pragma unroll
+        for (int vd = 0; vd < D; ++vd) f[1 + vd] = m * va[vd] + p_hat * nv[vd];
+        f[D + 1] = m * H;
+        if (ranocha) f[D + 1] -= T(0.5) * (q1.p * w1 + q2.p * w2);
+        return;
+    }
+    for (int s = 0; s < D + 2; ++s) f[s] = T(0);
+#pragma unroll
+    for (int m = 0; m < D; ++m) {
+        T fm[D + 2];
+        two_point_flux_dir<T, D>(u1, q1, u2, q2, m, pc, fm);
+        for (int s = 0; s < D + 2; ++s) f[s] += fm[s] * nv[m];
+    }
+}
+
+// sum_m nv[m] F_m of the convective flux
+template <typename T, int D>
+__device__ __forceinline__ void convective_flux_vec(const T *u, const T *nv, T gamm1, T *f)
+{
+    for (int s = 0; s < D + 2; ++s) f[s] = T(0);
+#pragma unroll
+    for (int m = 0; m < D; ++m) {
+        T fm[D + 2];
+        convective_flux_dir<T, D>(u, m, gamm1, fm);
+        for (int s = 0; s < D + 2; ++s) f[s] += fm[s] * nv[m];
+    }
+}
+
+// NumericalFluxConvective::evaluate_flux along a general unit normal n (curvilinear faces)
+template <typename T, int D>
+__device__ __forceinline__ void numerical_flux_n(const T *ui, const Prim<T, D> &L, const T *ue,
+                                                 const Prim<T, D> &R, const T *n, const PhysConst &pc, T *out)
+{
+    const T gamm1 = T(pc.gamm1);
+    const int cf = pc.conv_flux;
+    T base[D + 2];
+    if (cf >= 3) {
+        two_point_flux_vec<T, D>(ui, L, ue, R, n, pc, base);
+    } else {
+        T f1[D + 2], f2[D + 2];
+        for (int s = 0; s < D + 2; ++s) base[s] = T(0);
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            convective_flux_dir<T, D>(ui, m, gamm1, f1);
+            convective_flux_dir<T, D>(ue, m, gamm1, f2);
+            for (int s = 0; s < D + 2; ++s) base[s] += (T(0.5) * (f1[s] + f2[s])) * n[m];
+        }
+    }
+    T diss[D + 2];
+    for (int s = 0; s < D + 2; ++s) diss[s] = T(0);
+    if (cf == 0 || cf == 4) {
+        const T gam = T(pc.gam);
+        T vnL = T(0), vnR = T(0);
+        for (int k = 0; k < D; ++k) {
+            vnL += L.vel[k] * n[k];
+            vnR += R.vel[k] * n[k];
+        }
+        const T ei = dabs<T>(vnL) + dsqrt<T>(fdiv<T>(L.p * gam, L.rho));
+        const T ee = dabs<T>(vnR) + dsqrt<T>(fdiv<T>(R.p * gam, R.rho));
+        const T lam = (ei > ee) ? ei : ee;
+        for (int s = 0; s < D + 2; ++s) diss[s] = T(-0.5) * lam * (ue[s] - ui[s]);
+    }
+    if (cf == 1 || cf == 5) roe_dissipation_n<T, D>(ui, L, ue, R, n, pc, diss);
     for (int s = 0; s < D + 2; ++s) out[s] = base[s] + diss[s];
 }
 

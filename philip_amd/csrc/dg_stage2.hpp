@@ -78,7 +78,10 @@ __device__ __forceinline__ int pair_index(int a, int b)
     return a * (2 * N - a - 1) / 2 + (b - a - 1);
 }
 
-template <int D, int N, bool SPLIT, bool NSV, typename T, bool BND = false>
+// CURV: curvilinear grid (Euler): the metric cofactor C[m][n] and det J of every node from a.met;
+// two-point fluxes contracted with the pair's average cofactor column, conservative fluxes with the
+// node's, face fluxes along the owner's physical normal C n_ref scaled by its length, du/dt = acc/J.
+template <int D, int N, bool SPLIT, bool NSV, typename T, bool BND = false, bool CURV = false>
 __global__ void __launch_bounds__((Geo<D, N>::BLOCK))
 stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 {
@@ -88,6 +91,8 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
     __shared__ T sU[EPB][NS][NP];                       // state for pairs and faces
     __shared__ T sP[EPB][SPLIT ? D + 1 : 1][NP];          // velocity, pressure (split form)
     __shared__ T sF[EPB][NFLUX][NS][NP];                // volume fluxes for the D-divergence
+    __shared__ T sC[EPB][CURV ? D * D : 1][NP];         // CURV: metric cofactor, slot m * D + n
+    constexpr int MS = D * D + 1;                       // metric slots per node in a.met (C, det J)
     // face terms per face direction (phase C) and, once they are summed into the node's
     // registers, the pair fluxes of one direction (phase B) in the same storage
     constexpr int PFN = SPLIT ? NL * NPAIR : 1;
@@ -110,6 +115,14 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
         T ut[NS];
         for (int s = 0; s < NS; ++s) ut[s] = T(ud[s]);
         for (int s = 0; s < NS; ++s) sU[le][s][i] = ut[s];
+        T Cn[CURV ? D * D : 1];
+        if constexpr (CURV) {
+#pragma unroll
+            for (int slot = 0; slot < D * D; ++slot) {
+                Cn[slot] = T(a.met[((long)e * MS + slot) * NP + i]);
+                sC[le][slot][i] = Cn[slot];
+            }
+        }
         const T pr = pressure<T, D>(ut, gamm1);
         flag_nonphysical<T>(a, ut[0], pr);
         if (SPLIT) {
@@ -130,7 +143,15 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                 if (!SPLIT) {
                     T uu[NS];
                     for (int s = 0; s < NS; ++s) uu[s] = T(ud[s]);
-                    convective_flux_dir<T, D>(uu, k, gamm1, f);
+                    if constexpr (CURV) {
+                        // contravariant flux sum_m C[m][k] F_m
+                        T ck[D];
+#pragma unroll
+                        for (int m = 0; m < D; ++m) ck[m] = Cn[m * D + k];
+                        convective_flux_vec<T, D>(uu, ck, gamm1, f);
+                    } else {
+                        convective_flux_dir<T, D>(uu, k, gamm1, f);
+                    }
                 }
                 if (NSV) {
                     T fv[NS];
@@ -165,6 +186,43 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
             T us[NS], un[NS], unt[NS];
             for (int s = 0; s < NS; ++s) us[s] = sU[fle][s][fi];
             bool own = fe < nb;
+            if constexpr (CURV) {
+                // the owner's outward physical normal C n_ref at its face node (metric_oper_int):
+                // this element's, or the neighbour's read from a.met
+                own = own || nb < 0;
+                T nv[D], area2 = T(0);
+#pragma unroll
+                for (int m = 0; m < D; ++m) {
+                    nv[m] = own ? sgn * sC[fle][m * D + K][fi]
+                                : -sgn * T(a.met[((long)nb * MS + m * D + K) * NP + fj]);
+                    area2 += nv[m] * nv[m];
+                }
+                const T area = dsqrt<T>(area2);
+                T nh[D];
+#pragma unroll
+                for (int m = 0; m < D; ++m) nh[m] = nv[m] / area;
+                if (nb >= 0) {
+                    for (int s = 0; s < NS; ++s) un[s] = T(a.u[((long)nb * NS + s) * NP + fj]);
+                } else {
+                    boundary_state_n<T, D>(-nb, us, nh, pc, un);
+                }
+                T ua[NS], ub[NS];
+                for (int s = 0; s < NS; ++s) {
+                    ua[s] = own ? us[s] : un[s];
+                    ub[s] = own ? un[s] : us[s];
+                }
+                const Prim<T, D> pa = to_prim<T, D>(ua, gamm1);
+                const Prim<T, D> pb = to_prim<T, D>(ub, gamm1);
+                T fs[NS];
+                numerical_flux_n<T, D>(ua, pa, ub, pb, nh, pc, fs);
+                const T fsgn = own ? T(1) : T(-1);
+                T ft[NS];
+                for (int s = 0; s < NS; ++s) ft[s] = -((area * fs[s]) * fsgn);
+                if (!SPLIT)
+                    for (int s = 0; s < NS; ++s) ft[s] += sgn * sF[fle][K][s][fi];
+                for (int s = 0; s < NS; ++s) sFace[fle][K][s][fi] = ft[s];
+                return;
+            }
             if constexpr (BND) {
                 if (nb >= 0) {
                     for (int s = 0; s < NS; ++s) un[s] = T(a.u[((long)nb * NS + s) * NP + fj]);
@@ -269,7 +327,15 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                 q1.p = sP[ple][D][na];
                 q2.p = sP[ple][D][nbn];
                 T F[NS];
-                two_point_flux_dir<T, D>(u1, q1, u2, q2, k, pc, F);
+                if constexpr (CURV) {
+                    // the pair's average cofactor column (strong_dg.cpp:1348-1372)
+                    T nv[D];
+#pragma unroll
+                    for (int m = 0; m < D; ++m) nv[m] = T(0.5) * (sC[ple][m * D + k][na] + sC[ple][m * D + k][nbn]);
+                    two_point_flux_vec<T, D>(u1, q1, u2, q2, nv, pc, F);
+                } else {
+                    two_point_flux_dir<T, D>(u1, q1, u2, q2, k, pc, F);
+                }
                 for (int s = 0; s < NS; ++s) sPF[ple][r][s] = F[s];
             }
             __syncthreads();
@@ -297,7 +363,10 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
     }
 
     // ---------------- phase E: FP64 epilogue -----------------------------
-    stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc);
+    if constexpr (CURV)
+        stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc, false, nullptr, nullptr, a.met[((long)e * MS + D * D) * NP + i]);
+    else
+        stage_epilogue<D, N, T>(a, ops, e, i, c, ud, acc);
 }
 
 } // namespace dgx
