@@ -77,13 +77,19 @@ typedef struct {
        +e_k): 0 = periodic (both faces of a direction), else a type of
        Euler::boundary_face_values (euler.cpp:1511-1567) applied in assemble_boundary_term_strong
        (strong_dg.cpp:1583-2364): 1001 wall, 1002 pressure outflow, 1003 inflow, 1004 Riemann
-       farfield, 1005 farfield, 1006 slip wall, 1007 extrapolation.  Euler only. */
+       farfield, 1005 farfield, 1006 slip wall, 1007 extrapolation; Navier-Stokes: 1001 no-slip
+       wall and 1006 slip wall (boundary_face_values_viscous_flux, navier_stokes.cpp:1354-1430). */
     int boundary_type[6];
     double angle_of_attack;     /* radians: free stream (cos a, sin a[, 0]) of the boundary states */
     /* 0: the affine box; 1: the box's nodes through the reference's curved periodic grid
        (NonsymmetricCurvedGridManifold, nonsymmetric_curved_periodic_grid.cpp:66-87, beta = 0.1,
        made for [-1,1]^2 and [0,2pi]^3) with curvilinear metric terms.  Euler only. */
     int grid_warp;
+    /* Navier-Stokes no-slip walls (1001): 0 adiabatic (the reference's default), 1 isothermal at
+       wall_temperature (thermal_boundary_condition_type, nondimensionalized_isothermal_wall_temperature,
+       parameters_navier_stokes.cpp:21-50).  Navier-Stokes takes the wall types 1001 and 1006 only. */
+    int thermal_bc;
+    double wall_temperature;
 } dgx_params;
 
 typedef struct dgx_ctx dgx_ctx;

@@ -551,6 +551,37 @@ void dgo_boundary_state(const dgo_ctx *c, int type, const double *u_int, const d
     }
 }
 
+/* NavierStokes::boundary_face_values_viscous_flux for the wall types (navier_stokes.cpp:1354-1430):
+   1001 no-slip wall -- momentum reversed, the gradient copied with the energy row zeroed on an
+   adiabatic wall, or the energy set so that the average temperature is the wall temperature on an
+   isothermal one; 1006 slip wall -- the interior state and gradient.  g/gb: [s*dim + k]. */
+static void visc_boundary_state(const dgo_ctx *c, int type, const double *ui, const double *gi, double *ub,
+                                double *gb)
+{
+    const int ns = c->ns, d = c->dim;
+    for (int s = 0; s < ns; ++s) {
+        ub[s] = ui[s];
+        for (int k = 0; k < d; ++k) gb[s * d + k] = gi[s * d + k];
+    }
+    if (type != 1001) return;
+    for (int k = 0; k < d; ++k) ub[1 + k] = -ui[1 + k];
+    if (c->P.thermal_bc == 0) {
+        for (int k = 0; k < d; ++k) gb[(ns - 1) * d + k] = 0.0;
+    } else {
+        double wi[NS_MAX], we[NS_MAX];
+        prim_of(c, ui, wi);
+        prim_of(c, ub, we);
+        const double T_int = c->gam * c->mach2 * (wi[ns - 1] / wi[0]);
+        const double T_ext = 2.0 * c->P.wall_temperature - T_int;
+        we[ns - 1] = we[0] * T_ext / (c->gam * c->mach2);
+        double v2 = 0.0;
+        for (int k = 0; k < d; ++k) v2 += we[1 + k] * we[1 + k];
+        ub[ns - 1] = we[ns - 1] / c->gamm1 + 0.5 * we[0] * v2;
+    }
+}
+
+static int wall_adiabatic(const dgo_ctx *c, int type) { return type == 1001 && c->P.thermal_bc == 0; }
+
 /* ------------------------------------------------------------------------ */
 /* geometry: node coordinates and metric terms                              */
 /* ------------------------------------------------------------------------ */
@@ -739,6 +770,8 @@ void dgo_default_params(dgo_params *p)
     for (int f = 0; f < 6; ++f) p->boundary_type[f] = 0;
     p->angle_of_attack = 0.0;
     p->grid_warp = 0;
+    p->thermal_bc = 0;
+    p->wall_temperature = 1.0;
 }
 
 dgo_ctx *dgo_create(const dgo_params *prm)
@@ -771,7 +804,11 @@ dgo_ctx *dgo_create(const dgo_params *prm)
     /* boundaries: Euler only, both faces of a direction periodic or both boundaries, known types */
     for (int f = 0; f < 2 * c->dim; ++f) {
         const int t = prm->boundary_type[f], o = prm->boundary_type[f ^ 1];
-        if ((t == 0) != (o == 0) || (t && (t < 1001 || t > 1007)) || (t && prm->pde != DGO_EULER)) {
+        /* Navier-Stokes: the wall types only (boundary_face_values_viscous_flux sets the viscous
+           boundary state for 1001 and 1006 and leaves it unset for the others) */
+        const int ns_ok = prm->pde == DGO_EULER || t == 1001 || t == 1006;
+        if ((t == 0) != (o == 0) || (t && (t < 1001 || t > 1007)) || (t && !ns_ok) ||
+            (prm->thermal_bc != 0 && prm->thermal_bc != 1) || !(prm->wall_temperature > 0.0)) {
             free(c);
             return NULL;
         }
@@ -1192,7 +1229,16 @@ void dgo_numerical_flux(const dgo_ctx *c, int which, const double *ui, const dou
 /* ------------------------------------------------------------------------ */
 /* Navier-Stokes viscous flux (src/physics/navier_stokes.cpp)               */
 /* ------------------------------------------------------------------------ */
+static void diss_flux_impl(const dgo_ctx *c, const double *u, const double *g, int adiabatic, double *f);
+
 void dgo_dissipative_flux(const dgo_ctx *c, const double *u, const double *g, double *f)
+{
+    diss_flux_impl(c, u, g, 0, f);
+}
+
+/* adiabatic: the heat flux set to zero (dissipative_flux_dot_normal_on_adiabatic_boundary,
+   navier_stokes.cpp:1266-1310: a no-slip wall with the adiabatic thermal condition) */
+static void diss_flux_impl(const dgo_ctx *c, const double *u, const double *g, int adiabatic, double *f)
 {
     /* dissipative_flux_templated (navier_stokes.cpp:1197-1220) ; g[s*dim+k], f[s*dim+k] */
     const int d = c->dim, ns = c->ns;
@@ -1238,7 +1284,7 @@ void dgo_dissipative_flux(const dgo_ctx *c, const double *u, const double *g, do
     double q[3];
     for (int k = 0; k < d; ++k) {
         const double dT = (c->gam * c->mach2 * pg[ns - 1][k] - T * pg[0][k]) / rho;
-        q[k] = -kappa * dT;
+        q[k] = adiabatic ? 0.0 : -kappa * dT;
     }
     /* Step 4 (navier_stokes.cpp:1321-1350) */
     for (int fd = 0; fd < d; ++fd) {
@@ -1250,24 +1296,26 @@ void dgo_dissipative_flux(const dgo_ctx *c, const double *u, const double *g, do
     }
 }
 
-static void diss_flux_dot_n(const dgo_ctx *c, const double *u, const double *g, const double *n, double *out)
+static void diss_flux_dot_n(const dgo_ctx *c, const double *u, const double *g, const double *n, double *out,
+                            int adiabatic)
 {
     double f[NS_MAX * 3];
-    dgo_dissipative_flux(c, u, g, f);
+    diss_flux_impl(c, u, g, adiabatic, f);
     for (int s = 0; s < c->ns; ++s) {
         out[s] = 0.0;
         for (int k = 0; k < c->dim; ++k) out[s] += f[s * c->dim + k] * n[k];
     }
 }
 
-/* SIPG / BR2 evaluate_auxiliary_flux (viscous_numerical_flux.cpp:144-328) */
+/* SIPG / BR2 evaluate_auxiliary_flux (viscous_numerical_flux.cpp:144-328); adiabatic: on an
+   adiabatic no-slip wall every dissipative flux of the formula drops its heat flux */
 static void viscous_num_flux(const dgo_ctx *c, const double *ui, const double *ue, const double *gi,
-                             const double *ge, const double *n, double penalty, double *out)
+                             const double *ge, const double *n, double penalty, double *out, int adiabatic)
 {
     const int ns = c->ns, d = c->dim;
     double a[NS_MAX], b[NS_MAX];
-    diss_flux_dot_n(c, ui, gi, n, a);
-    diss_flux_dot_n(c, ue, ge, n, b);
+    diss_flux_dot_n(c, ui, gi, n, a, adiabatic);
+    diss_flux_dot_n(c, ue, ge, n, b, adiabatic);
     double avg[NS_MAX];
     for (int s = 0; s < ns; ++s) avg[s] = 0.5 * (a[s] + b[s]);
     if (c->P.diss_flux == DGO_SIPG) {
@@ -1275,8 +1323,8 @@ static void viscous_num_flux(const dgo_ctx *c, const double *ui, const double *u
         for (int s = 0; s < ns; ++s)
             for (int k = 0; k < d; ++k) jump[s * d + k] = (ui[s] - ue[s]) * n[k];
         double ja[NS_MAX], jb[NS_MAX];
-        diss_flux_dot_n(c, ui, jump, n, ja);
-        diss_flux_dot_n(c, ue, jump, n, jb);
+        diss_flux_dot_n(c, ui, jump, n, ja, adiabatic);
+        diss_flux_dot_n(c, ue, jump, n, jb, adiabatic);
         for (int s = 0; s < ns; ++s) out[s] = avg[s] - penalty * (0.5 * (ja[s] + jb[s]));
     } else {
         for (int s = 0; s < ns; ++s) out[s] = avg[s];
@@ -1656,13 +1704,14 @@ static void face_term(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *
                 double si[NS_MAX], se[NS_MAX], gi[NS_MAX * 3], ge[NS_MAX * 3], vf[NS_MAX];
                 for (int s = 0; s < ns; ++s) {
                     si[s] = ssurf[0][s * nfp + f];
+                    for (int k = 0; k < d; ++k) gi[s * d + k] = asurf[0][(s * d + k) * nfp + f];
+                    if (boundary) continue;
                     se[s] = ssurf[1][s * nfp + f];
-                    for (int k = 0; k < d; ++k) {
-                        gi[s * d + k] = asurf[0][(s * d + k) * nfp + f];
-                        ge[s * d + k] = asurf[1][(s * d + k) * nfp + f];
-                    }
+                    for (int k = 0; k < d; ++k) ge[s * d + k] = asurf[1][(s * d + k) * nfp + f];
                 }
-                viscous_num_flux(c, si, se, gi, ge, nphys, penalty, vf);
+                /* a wall: the viscous boundary state and gradient (strong_dg.cpp:2276-2300) */
+                if (boundary) visc_boundary_state(c, -ce, si, gi, se, ge);
+                viscous_num_flux(c, si, se, gi, ge, nphys, penalty, vf, boundary && wall_adiabatic(c, -ce));
                 for (int s = 0; s < ns; ++s) dnf[s * nfp + f] = nrm * vf[s];
             }
         }
@@ -1729,18 +1778,30 @@ static void aux_volume(dgo_ctx *c, int cell, const double *u, double *laux /*[d]
     free(coef); free(sq); free(rg); free(pg); free(rhs);
 }
 
+/* ce < 0: boundary face of type -ce (assemble_boundary_term_auxiliary_equation, strong_dg.cpp:
+   790-870): the exterior state is the viscous wall state of the interior trace, only the interior
+   side is lifted */
 static void aux_face(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *u, double *laux)
 {
     const int ns = c->ns, Np = c->Np, nq = c->nq, n = c->n, d = c->dim, nfp = c->nfp;
+    const int boundary = ce < 0;
     double *tmp = (double *)malloc(sizeof(double) * Np);
     double *si = (double *)malloc(sizeof(double) * ns * nfp);
     double *se = (double *)malloc(sizeof(double) * ns * nfp);
     for (int s = 0; s < ns; ++s) {
         for (int i = 0; i < Np; ++i) tmp[i] = u[IDX(c, ci, s, i)];
         surf_apply(c, fi, c->chi_surf[fi % 2], c->chi, nq, n, tmp, si + s * nfp, 0, 1.0);
+        if (boundary) continue;
         for (int i = 0; i < Np; ++i) tmp[i] = u[IDX(c, ce, s, i)];
         surf_apply(c, fe, c->chi_surf[fe % 2], c->chi, nq, n, tmp, se + s * nfp, 0, 1.0);
     }
+    if (boundary)
+        for (int f = 0; f < nfp; ++f) {
+            double ui[NS_MAX], ub[NS_MAX], g0[NS_MAX * 3] = {0}, gb[NS_MAX * 3];
+            for (int s = 0; s < ns; ++s) ui[s] = si[s * nfp + f];
+            visc_boundary_state(c, -ce, ui, g0, ub, gb);
+            for (int s = 0; s < ns; ++s) se[s * nfp + f] = ub[s];
+        }
     double C[3][3];
     metric_cofactor(c, C);
     double nref[3] = {0, 0, 0};
@@ -1770,6 +1831,7 @@ static void aux_face(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *u
         for (int k = 0; k < d; ++k) {
             surf_inner(c, fi, vi + (s * d + k) * nfp, c->wface, r, c->chi_surf[fi % 2], c->chi, nq, n, 0, 1.0);
             for (int i = 0; i < Np; ++i) laux[k * ns * Np + s * Np + i] += r[i];
+            if (boundary) continue;
             surf_inner(c, fe, ve + (s * d + k) * nfp, c->wface, r, c->chi_surf[fe % 2], c->chi, nq, n, 0, 1.0);
             for (int i = 0; i < Np; ++i) {
 #ifdef _OPENMP
@@ -1794,7 +1856,7 @@ static void assemble_auxiliary(dgo_ctx *c, const double *u)
         aux_volume(c, e, u, laux);
         for (int f = 0; f < 2 * d; ++f) {
             const int nb = c->nbr[e * 2 * d + f];
-            if (e < nb) aux_face(c, e, f, nb, f ^ 1, u, laux);
+            if (e < nb || nb < 0) aux_face(c, e, f, nb, f ^ 1, u, laux);
         }
         for (int k = 0; k < d; ++k)
             for (int s = 0; s < ns; ++s)
@@ -1967,7 +2029,8 @@ void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
                     for (int s = 0; s < ns; ++s) acc[s] += area * 0.5 * lam * (fabs(ui[s]) + fabs(ue[s])) / c->wq[0];
                 }
                 if (visc) {
-                    viscous_num_flux(c, ui, ue, gi, ge, nrm, penalty, fl);
+                    if (nb < 0) visc_boundary_state(c, -nb, ui, gi, ue, ge);
+                    viscous_num_flux(c, ui, ue, gi, ge, nrm, penalty, fl, nb < 0 && wall_adiabatic(c, -nb));
                     dgo_dissipative_flux(c, ui, gi, F);
                     for (int s = 0; s < ns; ++s) acc[s] += (fabs(fl[s]) + fabs(F[s * d + k])) / c->wq[0];
                     if (c->P.diss_flux == DGO_SIPG) {

@@ -89,6 +89,9 @@ typedef struct {
     int grid_warp;           /* 0: the affine box; 1: the reference's curved periodic grid
                                 (NonsymmetricCurvedGridManifold, nonsymmetric_curved_periodic_grid.cpp:66-87,
                                 beta = 0.1) applied to the box's nodes; GLL, Euler */
+    int thermal_bc;          /* Navier-Stokes no-slip walls (1001): 0 adiabatic, 1 isothermal
+                                (thermal_boundary_condition_type, parameters_navier_stokes.cpp:24-50) */
+    double wall_temperature; /* nondimensionalized_isothermal_wall_temperature (default 1) */
 } dgo_params;
 
 typedef struct dgo_ctx dgo_ctx;

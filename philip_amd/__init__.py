@@ -55,6 +55,7 @@ class Params(ctypes.Structure):
         ("n_cells_1d", ctypes.c_int), ("domain_left", ctypes.c_double), ("domain_right", ctypes.c_double),
         ("cell_order", ctypes.c_int), ("precision", ctypes.c_int),
         ("boundary_type", ctypes.c_int * 6), ("angle_of_attack", ctypes.c_double), ("grid_warp", ctypes.c_int),
+        ("thermal_bc", ctypes.c_int), ("wall_temperature", ctypes.c_double),
     ]
 
 

@@ -264,6 +264,8 @@ void dgx_default_params(dgx_params *p)
     for (int f = 0; f < 6; ++f) p->boundary_type[f] = 0;
     p->angle_of_attack = 0.0;
     p->grid_warp = 0;
+    p->thermal_bc = 0;
+    p->wall_temperature = 1.0;
 }
 
 // DGFactory::create_discontinuous_galerkin + allocate_system: operators, device buffers (from
@@ -289,9 +291,12 @@ static int create_ctx(const dgx_params *p, dgx::Arena *arena, hipStream_t stream
         const int t = p->boundary_type[f], o = p->boundary_type[f ^ 1];
         if ((t == 0) != (o == 0)) return bad(DGX_ERR_INVALID, "both faces of a direction periodic or both boundaries");
         if (t && (t < 1001 || t > 1007)) return bad(DGX_ERR_UNSUPPORTED, "boundary types 1001..1007 only");
-        if (t && p->pde != DGX_EULER) return bad(DGX_ERR_UNSUPPORTED, "boundaries on the Euler path only");
+        if (t && p->pde != DGX_EULER && t != 1001 && t != 1006)
+            return bad(DGX_ERR_UNSUPPORTED, "Navier-Stokes boundaries: the walls 1001 and 1006 only");
     }
     if (p->grid_warp != 0 && p->grid_warp != 1) return bad(DGX_ERR_INVALID, "grid_warp is 0 or 1");
+    if (p->thermal_bc != 0 && p->thermal_bc != 1) return bad(DGX_ERR_INVALID, "thermal_bc is 0 or 1");
+    if (!(p->wall_temperature > 0.0)) return bad(DGX_ERR_INVALID, "wall_temperature must be positive");
     if (p->grid_warp && p->pde != DGX_EULER) return bad(DGX_ERR_UNSUPPORTED, "curvilinear grids on the Euler path only");
     dgx_ctx *c = new dgx_ctx();
     c->P = *p;
@@ -327,6 +332,8 @@ static int create_ctx(const dgx_params *p, dgx::Arena *arena, hipStream_t stream
     pc.vinf[0] = std::cos(p->angle_of_attack);
     pc.vinf[1] = std::sin(p->angle_of_attack);
     pc.vinf[2] = 0.0;
+    pc.thermal_bc = p->thermal_bc;
+    pc.wall_temperature = p->wall_temperature;
     for (int f = 0; f < 2 * c->dim; ++f) c->has_boundary = c->has_boundary || p->boundary_type[f] != 0;
     c->nbr_count = (int64_t)c->ncells * 2 * c->dim;
     const size_t bytes = sizeof(double) * (size_t)c->ndofs;

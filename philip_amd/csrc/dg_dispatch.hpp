@@ -42,7 +42,15 @@ void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
         return;
     }
     if constexpr (D == 3) {
-        if (split && nsv) {
+        if (nsv && bnd) {
+            // Navier-Stokes with walls: stage2_kernel at both precisions
+            if (split)
+                hipLaunchKernelGGL((stage2_kernel<D, N, true, true, T, true>), grid, block, 0, c->stream, a, c->ops,
+                                   c->pc);
+            else
+                hipLaunchKernelGGL((stage2_kernel<D, N, false, true, T, true>), grid, block, 0, c->stream, a,
+                                   c->ops, c->pc);
+        } else if (split && nsv) {
             // mixed: line kernel with the viscous terms; FP64: stage2_kernel (the FP64 line
             // variant needs ~85 KB of LDS per block at p=5 and ran slower at one block per CU)
             if constexpr (std::is_same<T, float>::value) {
@@ -136,10 +144,10 @@ int launch_aux_dn(dgx_ctx *c, const double *u)
     const int blocks = (c->ncells + S::EPB - 1) / S::EPB;
     if (c->P.precision == DGX_MIXED)
         hipLaunchKernelGGL((aux_kernel<D, N, float>), dim3(blocks), dim3(S::BLOCK), 0, c->stream, u, c->auxf_dev[0],
-                           c->auxf_dev[1], c->auxf_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops);
+                           c->auxf_dev[1], c->auxf_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops, c->pc);
     else
         hipLaunchKernelGGL((aux_kernel<D, N, double>), dim3(blocks), dim3(S::BLOCK), 0, c->stream, u, c->aux_dev[0],
-                           c->aux_dev[1], c->aux_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops);
+                           c->aux_dev[1], c->aux_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops, c->pc);
     return check_launch(c, "aux_kernel");
 }
 

@@ -217,7 +217,7 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs &a, const Ops1D &
 template <int D, int N, typename Q>
 __global__ void __launch_bounds__((Shape<D, N>::BLOCK))
 aux_kernel(const double *__restrict__ u, Q *__restrict__ q0, Q *__restrict__ q1,
-           Q *__restrict__ q2, const int *__restrict__ nbr, int ncells, double inv_h, Ops1D ops)
+           Q *__restrict__ q2, const int *__restrict__ nbr, int ncells, double inv_h, Ops1D ops, PhysConst pc)
 {
     using S = Shape<D, N>;
     constexpr int NP = S::NP, NS = S::NS, EPB = S::EPB;
@@ -250,8 +250,12 @@ aux_kernel(const double *__restrict__ u, Q *__restrict__ q0, Q *__restrict__ q1,
             const int nb = nbr[e * 2 * D + 2 * d + side];
             const int j = node_with<D, N>(i, d, side ? 0 : N - 1);
             const double sgn = side ? 1.0 : -1.0;
+            // a wall face (nb < 0: -type): the viscous wall state of the node
+            // (assemble_boundary_term_auxiliary_equation, strong_dg.cpp:790-870)
+            double ub[NS];
+            if (nb < 0) visc_wall_state<double, D>(-nb, ui, pc, ub);
             for (int s = 0; s < NS; ++s) {
-                const double un = u[((long)nb * NS + s) * NP + j];
+                const double un = (nb < 0) ? ub[s] : u[((long)nb * NS + s) * NP + j];
                 const double avg = 0.5 * (ui[s] + un);
                 g[s] += ops.inv_wb * ((avg - ui[s]) * sgn);
             }

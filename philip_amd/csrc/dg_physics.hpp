@@ -15,6 +15,8 @@ namespace dgx {
 struct PhysConst {
     double gam, gamm1, mach2, reynolds, prandtl, mu_const, temperature_ratio;
     double vinf[3];  // free-stream velocity (cos a, sin a, 0): farfield, Riemann and inflow states
+    double wall_temperature;  // Navier-Stokes isothermal no-slip walls
+    int thermal_bc;  // Navier-Stokes no-slip walls (1001): 0 adiabatic, 1 isothermal
     int use_constant_viscosity;
     int two_point;   // KG=0 IR=1 CH=2 RA=3
     int conv_flux;   // LXF=0 ROE=1 CENTRAL=2 TWO_POINT=3 TP+LXF=4 TP+ROE=5
@@ -209,6 +211,35 @@ __device__ __forceinline__ void boundary_state(int type, const T *ui, int K, T s
     for (int k = 0; k < D; ++k) ub[1 + k] = copy ? ui[1 + k] : rho_b * vel_b[k];
     ub[D + 1] = copy ? ui[D + 1] : p_b / gamm1 + T(0.5) * rho_b * w2;
     (void)NS;
+}
+
+// NavierStokes::boundary_face_values_viscous_flux for the wall types (navier_stokes.cpp:1354-1430): the
+// exterior state of the viscous flux and of the lifted gradient.  1001 no-slip wall: momentum
+// reversed, and on an isothermal wall the energy set so that the average temperature is the wall
+// temperature; 1006 slip wall: the interior state.  (The exterior gradient is the interior one,
+// with the energy row zeroed on an adiabatic no-slip wall.)
+template <typename T, int D>
+__device__ __forceinline__ void visc_wall_state(int type, const T *ui, const PhysConst &pc, T *ub)
+{
+    for (int s = 0; s < D + 2; ++s) ub[s] = ui[s];
+    if (type != 1001) return;
+#pragma unroll
+    for (int k = 0; k < D; ++k) ub[1 + k] = -ui[1 + k];
+    if (pc.thermal_bc == 1) {
+        const T gam = T(pc.gam), gamm1 = T(pc.gamm1), m2 = T(pc.mach2);
+        const T rho = ui[0];
+        T v2 = T(0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const T v = ub[1 + k] / rho;
+            v2 += v * v;
+        }
+        const T p_i = gamm1 * (ui[D + 1] - T(0.5) * rho * v2);
+        const T T_int = gam * m2 * (p_i / rho);
+        const T T_ext = T(2) * T(pc.wall_temperature) - T_int;
+        const T p_e = rho * T_ext / (gam * m2);
+        ub[D + 1] = p_e / gamm1 + T(0.5) * rho * v2;
+    }
 }
 
 // The same boundary states on a face of a curvilinear grid, n the outward unit normal (the
@@ -596,7 +627,8 @@ __device__ __forceinline__ void numerical_flux_n(const T *ui, const Prim<T, D> &
 // dissipative_flux_templated (navier_stokes.cpp:1197-1220), direction fd component.
 // g is the conservative gradient g[s*D + k].
 template <typename T, int D>
-__device__ __forceinline__ void viscous_flux_dir(const T *u, const T *g, int fd, const PhysConst &pc, T *f)
+__device__ __forceinline__ void viscous_flux_dir(const T *u, const T *g, int fd, const PhysConst &pc, T *f,
+                                                 bool adiabatic = false)
 {
     const T gam = T(pc.gam), gamm1 = T(pc.gamm1), mach2 = T(pc.mach2);
     const T rho = u[0];
@@ -631,7 +663,9 @@ __device__ __forceinline__ void viscous_flux_dir(const T *u, const T *g, int fd,
     }
     const T kappa = mu_hat / (gamm1 * mach2 * T(pc.prandtl));
     const T dT = (gam * mach2 * pg[D + 1][fd] - T_ * pg[0][fd]) / rho;
-    const T q = -kappa * dT;
+    // adiabatic: an adiabatic no-slip wall drops the heat flux
+    // (dissipative_flux_dot_normal_on_adiabatic_boundary, navier_stokes.cpp:1266-1310)
+    const T q = adiabatic ? T(0) : -kappa * dT;
     f[0] = T(0);
     for (int a = 0; a < D; ++a) f[1 + a] = -tau_fd[a];
     T fe = T(0);

@@ -254,30 +254,48 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
             if (!SPLIT)
                 for (int s = 0; s < NS; ++s) ft[s] += sgn * sF[fle][K][s][fi];
             if (NSV) {
-                T ui[NS], gi[NS * D], gn[NS * D], jump[NS * D];
+                T ui[NS], gi[NS * D], gn[NS * D], jump[NS * D], uv[NS];
+                bool wall = false, adia = false;
+                if constexpr (BND) wall = nb < 0;
                 for (int s = 0; s < NS; ++s) {
                     ui[s] = T(a.u[((long)fe * NS + s) * NP + fi]);
-                    for (int d = 0; d < D; ++d) {
-                        gi[s * D + d] = aux_at<T>(a, d, ((long)fe * NS + s) * NP + fi);
-                        gn[s * D + d] = aux_at<T>(a, d, ((long)nb * NS + s) * NP + fj);
-                        jump[s * D + d] = (d == K) ? (ui[s] - un[s]) * sgn : (ui[s] - un[s]) * T(0);
+                    for (int d = 0; d < D; ++d) gi[s * D + d] = aux_at<T>(a, d, ((long)fe * NS + s) * NP + fi);
+                }
+                if (wall) {
+                    // a wall (1001 no-slip, 1006 slip): the viscous wall state and the interior gradient,
+                    // its energy row zeroed and every flux without heat flux on an adiabatic no-slip wall
+                    // (boundary_face_values_viscous_flux, navier_stokes.cpp:1354-1430)
+                    visc_wall_state<T, D>(-nb, ui, pc, uv);
+                    adia = (-nb == 1001) && pc.thermal_bc == 0;
+                    for (int s = 0; s < NS; ++s)
+                        for (int d = 0; d < D; ++d) gn[s * D + d] = (adia && s == NS - 1) ? T(0) : gi[s * D + d];
+                } else {
+                    for (int s = 0; s < NS; ++s) {
+                        uv[s] = un[s];
+                        for (int d = 0; d < D; ++d) gn[s * D + d] = aux_at<T>(a, d, ((long)nb * NS + s) * NP + fj);
                     }
                 }
+                for (int s = 0; s < NS; ++s)
+                    for (int d = 0; d < D; ++d)
+                        jump[s * D + d] = (d == K) ? (ui[s] - uv[s]) * sgn : (ui[s] - uv[s]) * T(0);
                 T fa[NS], fb[NS];
-                viscous_flux_dir<T, D>(ui, gi, K, pc, fa);
-                viscous_flux_dir<T, D>(un, gn, K, pc, fb);
+                viscous_flux_dir<T, D>(ui, gi, K, pc, fa, adia);
+                viscous_flux_dir<T, D>(uv, gn, K, pc, fb, adia);
                 T fvs[NS];
                 for (int s = 0; s < NS; ++s) fvs[s] = T(0.5) * (fa[s] * sgn + fb[s] * sgn);
                 if (pc.diss_flux == 0) {
                     T ja[NS], jb[NS];
-                    viscous_flux_dir<T, D>(ui, jump, K, pc, ja);
-                    viscous_flux_dir<T, D>(un, jump, K, pc, jb);
+                    viscous_flux_dir<T, D>(ui, jump, K, pc, ja, adia);
+                    viscous_flux_dir<T, D>(uv, jump, K, pc, jb, adia);
                     const T pen = T(a.penalty);
                     for (int s = 0; s < NS; ++s) fvs[s] = fvs[s] - pen * (T(0.5) * (ja[s] * sgn + jb[s] * sgn));
                 }
                 for (int s = 0; s < NS; ++s) ft[s] -= fvs[s];
-                if (SPLIT)
+                if (SPLIT) {
+                    // the strong form's interior viscous flux (with its heat flux)
+                    if (adia) viscous_flux_dir<T, D>(ui, gi, K, pc, fa);
                     for (int s = 0; s < NS; ++s) ft[s] += sgn * fa[s];
+                }
             }
             for (int s = 0; s < NS; ++s) sFace[fle][K][s][fi] = ft[s];
         };

@@ -175,4 +175,6 @@ def test_invalid_boundaries_raise():
     with pytest.raises(P.DeviceError):
         P.DG(dim=2, poly_degree=2, boundary_type=[1000, 1000, 0, 0])
     with pytest.raises(P.DeviceError):
-        P.DG(dim=3, poly_degree=2, pde=P.NAVIER_STOKES, boundary_type=[1006, 1006, 0, 0, 0, 0])
+        P.DG(dim=3, poly_degree=2, pde=P.NAVIER_STOKES, boundary_type=[1004, 1004, 0, 0, 0, 0])
+    with pytest.raises(P.DeviceError):
+        P.DG(dim=3, poly_degree=2, pde=P.NAVIER_STOKES, boundary_type=[1001, 1001, 0, 0, 0, 0], thermal_bc=2)

@@ -25,6 +25,9 @@ namespace PHiLiP { namespace Parameters {
 struct EulerParam { double mach_inf = 0.1, gamma_gas = 1.4, angle_of_attack = 0.0; };
 struct NavierStokesParam {
     double prandtl_number = 0.72, reynolds_number_inf = 1600.0, temperature_inf = 273.15;
+    double nondimensionalized_isothermal_wall_temperature = 1.0;
+    enum ThermalBoundaryCondition { adiabatic, isothermal };
+    ThermalBoundaryCondition thermal_boundary_condition_type = adiabatic;
     bool use_constant_viscosity = false;
     double nondimensionalized_constant_viscosity = 1.0;
 };

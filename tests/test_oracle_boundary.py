@@ -233,7 +233,8 @@ def test_invalid_boundary_types_rejected(bad):
     with pytest.raises(ValueError):
         O.Oracle(dim=2, poly_degree=1, n_cells_1d=2, boundary_type=bad)
     with pytest.raises(ValueError):
-        O.Oracle(dim=2, poly_degree=1, n_cells_1d=2, pde=O.NAVIER_STOKES, boundary_type=[1006, 1006, 0, 0])
+        # Navier-Stokes takes the walls only (the reference sets no viscous boundary state for the others)
+        O.Oracle(dim=2, poly_degree=1, n_cells_1d=2, pde=O.NAVIER_STOKES, boundary_type=[1004, 1004, 0, 0])
 
 
 SCHEMES = [
