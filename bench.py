@@ -274,12 +274,15 @@ def ns_tgv_params():
                 two_point_flux=P.RA, conv_flux=P.TWO_POINT_ROE)
 
 
-def navier_stokes_tgv(precision="mixed", steps=3):
+def navier_stokes_tgv(precision="mixed", steps=3, curved=False):
     """Navier-Stokes TGV, Re 1600, Pr 0.71, M 0.1, p=5 on 16^3 hex (4,423,680 DOF), BR2, Ranocha
     split form + EC/Roe surface flux, constant viscosity, classical RK4 (auxiliary-gradient kernel +
-    stage kernel per stage).  Returns DOF-updates/s over the timed steps."""
+    stage kernel per stage).  curved: on the reference's curved periodic grid (beta 0.1), the
+    lifted gradient and the stage kernel with the metric terms.  Returns DOF-updates/s over the
+    timed steps."""
     import philip_amd as P
-    d = P.DG(precision=P.MIXED if precision == "mixed" else P.FP64, **ns_tgv_params())
+    d = P.DG(precision=P.MIXED if precision == "mixed" else P.FP64, grid_warp=1 if curved else 0,
+             **ns_tgv_params())
     d.set_initial_condition_tgv()
     dt = d.constant_time_step(0.01)
     d.allocate_ode_system("rk4")
@@ -291,8 +294,9 @@ def navier_stokes_tgv(precision="mixed", steps=3):
     t = time.perf_counter() - t0
     n = d.ndofs
     d.close()
+    grid = " on the curved periodic grid" if curved else ""
     return dict(value=n * 4 * steps / t, unit="DOF-updates/s", dof=n, ms_per_stage=1e3 * t / (4 * steps),
-                precision=precision, workload="NS TGV Re1600 p=5 16^3, BR2, Ranocha split + EC/Roe, RK4")
+                precision=precision, workload=f"NS TGV Re1600 p=5 16^3{grid}, BR2, Ranocha split + EC/Roe, RK4")
 
 
 def curvilinear_tgv(precision="fp64", steps=3):
@@ -410,6 +414,8 @@ def main():
                     help="only run the Navier-Stokes TGV stage rate at this precision (for profiling)")
     ap.add_argument("--curvilinear-only", choices=["fp64", "mixed"], default=None,
                     help="only run the curved-grid Euler TGV stage rate at this precision (for profiling)")
+    ap.add_argument("--curvilinear-ns-only", choices=["fp64", "mixed"], default=None,
+                    help="only run the curved-grid Navier-Stokes TGV stage rate at this precision (for profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances line")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream line")
     ap.add_argument("--serving-rate", type=float, default=8.0,
@@ -423,6 +429,9 @@ def main():
         return
     if args.curvilinear_only:
         print(json.dumps(curvilinear_tgv(args.curvilinear_only)), flush=True)
+        return
+    if args.curvilinear_ns_only:
+        print(json.dumps(navier_stokes_tgv(args.curvilinear_ns_only, curved=True)), flush=True)
         return
     if args.copy_calibration:
         copy_calibration()
@@ -468,7 +477,8 @@ def main():
         progress(f"batched: {line['batched']['value']:.3e} DOF-updates/s")
         line["navier_stokes"] = [navier_stokes_tgv("fp64"), navier_stokes_tgv("mixed")]
         progress("Navier-Stokes stage rates done")
-        line["curvilinear"] = [curvilinear_tgv("fp64"), curvilinear_tgv("mixed")]
+        line["curvilinear"] = [curvilinear_tgv("fp64"), curvilinear_tgv("mixed"),
+                               navier_stokes_tgv("fp64", curved=True), navier_stokes_tgv("mixed", curved=True)]
         progress("curvilinear-grid stage rates done")
     if not args.no_ns_history:
         line["tgv_1000_steps"] = tgv_history()

@@ -83,7 +83,7 @@ typedef struct {
     double angle_of_attack;     /* radians: free stream (cos a, sin a[, 0]) of the boundary states */
     /* 0: the affine box; 1: the box's nodes through the reference's curved periodic grid
        (NonsymmetricCurvedGridManifold, nonsymmetric_curved_periodic_grid.cpp:66-87, beta = 0.1,
-       made for [-1,1]^2 and [0,2pi]^3) with curvilinear metric terms.  Euler only. */
+       made for [-1,1]^2 and [0,2pi]^3) with curvilinear metric terms (Euler, and Navier-Stokes in 3D). */
     int grid_warp;
     /* Navier-Stokes no-slip walls (1001): 0 adiabatic (the reference's default), 1 isothermal at
        wall_temperature (thermal_boundary_condition_type, nondimensionalized_isothermal_wall_temperature,
@@ -131,8 +131,9 @@ int dgx_get_neighbors(const dgx_ctx *ctx, int32_t *host_nbr);
    host array [cell][node][3] in the reference's cell order -- what HighOrderGrid holds
    (high_order_grid.cpp) for grid_degree = poly_degree -- from which the device builds the metric
    terms of metric_operators (operators.cpp:2520-2884: Jacobian, det J, and in 3D the conservative
-   curl form of the cofactor).  The box topology (periodic or boundary faces) is kept.  Euler only;
-   the batched line kernel and the integrated quantities do not take curvilinear contexts. */
+   curl form of the cofactor) and each cell's SIPG penalty from its extent along each direction
+   (evaluate_penalty_scaling, dg_base.cpp:397-413).  The box topology (periodic or boundary faces) is
+   kept.  The batched line kernel and the integrated quantities do not take curvilinear contexts. */
 int dgx_set_grid_nodes(dgx_ctx *ctx, const double *host_xyz);
 /* node coordinates in use [cell][node][3] (the affine box if none were set) */
 int dgx_get_grid_nodes(dgx_ctx *ctx, double *host_xyz);

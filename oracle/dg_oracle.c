@@ -688,7 +688,7 @@ static void compute_metrics(dgo_ctx *c)
 
 int dgo_set_grid_nodes(dgo_ctx *c, const double *xyz)
 {
-    if (c->P.flux_nodes != DGO_GLL || c->P.pde != DGO_EULER) return -1;
+    if (c->P.flux_nodes != DGO_GLL) return -1;
     memcpy(c->xn, xyz, sizeof(double) * (size_t)c->ncells * c->Np * 3);
     compute_metrics(c);
     c->curv = 1;
@@ -813,8 +813,8 @@ dgo_ctx *dgo_create(const dgo_params *prm)
             return NULL;
         }
     }
-    /* curvilinear grids: collocated GLL nodes (grid degree = p), Euler */
-    if (prm->grid_warp && (prm->flux_nodes != DGO_GLL || prm->pde != DGO_EULER || prm->grid_warp != 1)) {
+    /* curvilinear grids: collocated GLL nodes (grid degree = p) */
+    if (prm->grid_warp && (prm->flux_nodes != DGO_GLL || prm->grid_warp != 1)) {
         free(c);
         return NULL;
     }
@@ -1512,12 +1512,38 @@ static void volume_term(dgo_ctx *c, int cell, const double *u, double *lrhs)
     free(Ssp); free(div); free(ddiv); free(rhs);
 }
 
-static double penalty_scaling(const dgo_ctx *c)
+/* cell->extent_in_direction(k): the longest of the cell's edges along reference direction k, from
+   its vertices (the corner nodes); h on the affine box */
+static double cell_extent(const dgo_ctx *c, int cell, int k)
 {
-    /* evaluate_penalty_scaling (dg_base.cpp:397-413) */
+    if (!c->curv) return c->h;
+    const int n = c->n, d = c->dim;
+    double ext = 0.0;
+    for (int a = 0; a < (1 << (d - 1)); ++a) {
+        int lo[3] = {0, 0, 0}, bit = 0;
+        for (int m = 0; m < d; ++m) {
+            if (m == k) continue;
+            lo[m] = ((a >> bit) & 1) ? n - 1 : 0;
+            ++bit;
+        }
+        int hi[3] = {lo[0], lo[1], lo[2]};
+        hi[k] = n - 1;
+        const int q0 = lo[0] + n * (lo[1] + n * lo[2]), q1 = hi[0] + n * (hi[1] + n * hi[2]);
+        const double *x0 = c->xn + ((long)cell * c->Np + q0) * 3, *x1 = c->xn + ((long)cell * c->Np + q1) * 3;
+        double l2 = 0.0;
+        for (int m = 0; m < 3; ++m) l2 += (x1[m] - x0[m]) * (x1[m] - x0[m]);
+        const double l = sqrt(l2);
+        if (l > ext) ext = l;
+    }
+    return ext;
+}
+
+/* evaluate_penalty_scaling (dg_base.cpp:397-413) of a cell's faces normal to direction k */
+static double penalty_scaling(const dgo_ctx *c, int cell, int k)
+{
     const unsigned deg = (unsigned)c->p;
     const unsigned degsq = (deg == 0) ? 1 : deg * (deg + 1);
-    return degsq / c->h * c->P.sipg_penalty_factor;
+    return degsq / cell_extent(c, cell, k) * c->P.sipg_penalty_factor;
 }
 
 /* face term (strong_dg.cpp:2368-3576); rhs_ext is the global vector */
@@ -1529,7 +1555,10 @@ static void face_term(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *
     const int ns = c->ns, Np = c->Np, nq = c->nq, n = c->n, d = c->dim, nfp = c->nfp;
     const int split = c->P.use_split_form;
     const int ns_visc = (c->P.pde == DGO_NAVIER_STOKES);
-    const double penalty = 0.5 * (penalty_scaling(c) + penalty_scaling(c));
+    /* a boundary face: the cell's own penalty (dg_base.cpp:563); an interior face: the average of both
+       cells' (dg_base.cpp:615-617) */
+    const double penalty = (ce < 0) ? penalty_scaling(c, ci, fi / 2)
+                                    : 0.5 * (penalty_scaling(c, ci, fi / 2) + penalty_scaling(c, ce, fe / 2));
     const int boundary = ce < 0, nsides = boundary ? 1 : 2;
     const int cells[2] = {ci, ce}, faces[2] = {fi, fe};
     double *svol[2] = {NULL, NULL}, *ssurf[2] = {NULL, NULL}, *avol[2] = {NULL, NULL}, *asurf[2] = {NULL, NULL};
@@ -1760,14 +1789,15 @@ static void aux_volume(dgo_ctx *c, int cell, const double *u, double *laux /*[d]
     double *rg = (double *)malloc(sizeof(double) * d * Np);
     double *pg = (double *)malloc(sizeof(double) * Np);
     double *rhs = (double *)malloc(sizeof(double) * Np);
-    double C[3][3];
-    metric_cofactor(c, C);
     for (int s = 0; s < ns; ++s) {
         for (int i = 0; i < Np; ++i) coef[i] = u[IDX(c, cell, s, i)];
         vol_apply(c, c->chi, nq, n, coef, sq, 0, 1.0);
         for (int k = 0; k < d; ++k) dir_apply(c, k, c->fb_grad, c->fb_vol, nq, nq, sq, rg + k * Np, 0, 1.0);
         for (int k = 0; k < d; ++k) {
             for (int q = 0; q < Np; ++q) {
+                /* the physical gradient C grad_ref u (strong_dg.cpp:700-712) */
+                double C[3][3];
+                metric_at(c, cell, q, C);
                 pg[q] = 0.0;
                 for (int j = 0; j < d; ++j) pg[q] += C[k][j] * rg[j * Np + q];
             }
@@ -1802,13 +1832,14 @@ static void aux_face(dgo_ctx *c, int ci, int fi, int ce, int fe, const double *u
             visc_boundary_state(c, -ce, ui, g0, ub, gb);
             for (int s = 0; s < ns; ++s) se[s * nfp + f] = ub[s];
         }
-    double C[3][3];
-    metric_cofactor(c, C);
     double nref[3] = {0, 0, 0};
     nref[fi / 2] = (fi % 2) ? 1.0 : -1.0;
     double *vi = (double *)malloc(sizeof(double) * ns * d * nfp);
     double *ve = (double *)malloc(sizeof(double) * ns * d * nfp);
     for (int f = 0; f < nfp; ++f) {
+        /* the interior cell's metric at the face node */
+        double C[3][3];
+        metric_at(c, ci, face_vol_node(c, fi, f), C);
         double nphys[3];
         for (int a = 0; a < d; ++a) {
             nphys[a] = 0.0;
@@ -1927,7 +1958,6 @@ void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
             Sk[a][b] = c->skew[a * n + b] / c->wq[a];
         }
     const double hd1 = (d == 3) ? c->h * c->h : c->h;
-    const double penalty = penalty_scaling(c);
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads)
 #endif
@@ -1980,13 +2010,22 @@ void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
                     }
                     if (visc) {
                         dgo_dissipative_flux(c, uj, gj, F);
-                        for (int s = 0; s < ns; ++s) acc[s] += fabs(D[ci[k]][b] * F[s * d + k]);
+                        for (int s = 0; s < ns; ++s) {
+                            double fk = F[s * d + k];
+                            if (c->curv) {
+                                fk = 0.0;
+                                for (int m = 0; m < d; ++m) fk += F[s * d + m] * Cj[m][k];
+                            }
+                            acc[s] += fabs(D[ci[k]][b] * fk);
+                        }
                     }
                 }
                 if (ci[k] != 0 && ci[k] != n - 1) continue;
                 /* face node: numerical fluxes and the interior fluxes of the strong form */
                 const int side = (ci[k] == n - 1), face = 2 * k + side;
                 const int nb = c->nbr[e * 2 * d + face];
+                const double penalty = (nb < 0) ? penalty_scaling(c, e, k)
+                                                : 0.5 * (penalty_scaling(c, e, k) + penalty_scaling(c, nb, k));
                 int cj[3] = {ci[0], ci[1], ci[2]};
                 cj[k] = side ? 0 : n - 1;
                 const int j = cj[0] + n * (cj[1] + n * cj[2]);
@@ -2032,7 +2071,14 @@ void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
                     if (nb < 0) visc_boundary_state(c, -nb, ui, gi, ue, ge);
                     viscous_num_flux(c, ui, ue, gi, ge, nrm, penalty, fl, nb < 0 && wall_adiabatic(c, -nb));
                     dgo_dissipative_flux(c, ui, gi, F);
-                    for (int s = 0; s < ns; ++s) acc[s] += (fabs(fl[s]) + fabs(F[s * d + k])) / c->wq[0];
+                    for (int s = 0; s < ns; ++s) {
+                        double fn = F[s * d + k];
+                        if (c->curv) {
+                            fn = 0.0;
+                            for (int m = 0; m < d; ++m) fn += F[s * d + m] * nrm[m];
+                        }
+                        acc[s] += area * (fabs(fl[s]) + fabs(fn)) / c->wq[0];
+                    }
                     if (c->P.diss_flux == DGO_SIPG) {
                         /* penalty {{F_v(u, (u_i - u_e) n)}}: the one-sided terms in u_i and u_e */
                         const double *side_u[2] = {ui, ue};
@@ -2042,7 +2088,14 @@ void dgo_residual_magnitude(dgo_ctx *c, const double *u, double *mag)
                                 for (int s = 0; s < ns; ++s)
                                     for (int kk = 0; kk < d; ++kk) g[s * d + kk] = side_u[b][s] * nrm[kk];
                                 dgo_dissipative_flux(c, side_u[a], g, Fv);
-                                for (int s = 0; s < ns; ++s) acc[s] += 0.5 * penalty * fabs(Fv[s * d + k]) / c->wq[0];
+                                for (int s = 0; s < ns; ++s) {
+                                    double fn = Fv[s * d + k];
+                                    if (c->curv) {
+                                        fn = 0.0;
+                                        for (int m = 0; m < d; ++m) fn += Fv[s * d + m] * nrm[m];
+                                    }
+                                    acc[s] += area * 0.5 * penalty * fabs(fn) / c->wq[0];
+                                }
                             }
                     }
                 }

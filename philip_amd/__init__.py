@@ -293,7 +293,7 @@ class DG:
         return a
 
     def set_grid_nodes(self, xyz):
-        """Curvilinear grid from node coordinates [cell][node][3] (grid degree = p, Euler)."""
+        """Curvilinear grid from node coordinates [cell][node][3] (grid degree = p)."""
         x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.ncells, -1, 3)
         self._chk(load().dgx_set_grid_nodes(self.h, _dp(x)), "set_grid_nodes")
 

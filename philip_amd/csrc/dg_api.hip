@@ -297,7 +297,6 @@ static int create_ctx(const dgx_params *p, dgx::Arena *arena, hipStream_t stream
     if (p->grid_warp != 0 && p->grid_warp != 1) return bad(DGX_ERR_INVALID, "grid_warp is 0 or 1");
     if (p->thermal_bc != 0 && p->thermal_bc != 1) return bad(DGX_ERR_INVALID, "thermal_bc is 0 or 1");
     if (!(p->wall_temperature > 0.0)) return bad(DGX_ERR_INVALID, "wall_temperature must be positive");
-    if (p->grid_warp && p->pde != DGX_EULER) return bad(DGX_ERR_UNSUPPORTED, "curvilinear grids on the Euler path only");
     dgx_ctx *c = new dgx_ctx();
     c->P = *p;
     c->arena = arena;
@@ -364,7 +363,8 @@ static int create_ctx(const dgx_params *p, dgx::Arena *arena, hipStream_t stream
         c->xn_dev = static_cast<double *>(dev_alloc(c, sizeof(double) * 3 * (size_t)c->ncells * c->Np));
         c->met_dev = static_cast<double *>(
             dev_alloc(c, sizeof(double) * (size_t)(c->dim * c->dim + 1) * c->ncells * c->Np));
-        if (!c->xn_dev || !c->met_dev || launch_coords(c, c->xn_dev, 1) != DGX_OK || launch_metrics(c) != DGX_OK) {
+        c->pen_dev = static_cast<double *>(dev_alloc(c, sizeof(double) * (size_t)c->dim * c->ncells));
+        if (!c->xn_dev || !c->met_dev || !c->pen_dev || launch_coords(c, c->xn_dev, 1) != DGX_OK || launch_metrics(c) != DGX_OK) {
             c->stream = nullptr;
             dgx_destroy(c);
             return arena ? DGX_ERR_STATE : bad(DGX_ERR_HIP, "curvilinear grid set-up failed");
@@ -398,6 +398,7 @@ int dgx_destroy(dgx_ctx *c)
     dev_free(c, c->status_dev);
     dev_free(c, c->xn_dev);
     dev_free(c, c->met_dev);
+    dev_free(c, c->pen_dev);
     for (int k = 0; k < 3; ++k) {
         dev_free(c, c->aux_dev[k]);
         dev_free(c, c->auxf_dev[k]);
@@ -475,10 +476,6 @@ int dgx_get_neighbors(const dgx_ctx *cc, int32_t *host_nbr)
 int dgx_set_grid_nodes(dgx_ctx *c, const double *host_xyz)
 {
     if (!c || !host_xyz) return DGX_ERR_INVALID;
-    if (c->P.pde != DGX_EULER) {
-        c->err = "curvilinear grids on the Euler path only";
-        return DGX_ERR_UNSUPPORTED;
-    }
     if (c->batch_locked) {
         c->err = "the context is captured in a batch graph";
         return DGX_ERR_STATE;
@@ -486,7 +483,8 @@ int dgx_set_grid_nodes(dgx_ctx *c, const double *host_xyz)
     const size_t nx = 3 * (size_t)c->ncells * c->Np, nm = (size_t)(c->dim * c->dim + 1) * c->ncells * c->Np;
     if (!c->xn_dev) c->xn_dev = static_cast<double *>(dev_alloc(c, sizeof(double) * nx));
     if (!c->met_dev) c->met_dev = static_cast<double *>(dev_alloc(c, sizeof(double) * nm));
-    if (!c->xn_dev || !c->met_dev) {
+    if (!c->pen_dev) c->pen_dev = static_cast<double *>(dev_alloc(c, sizeof(double) * (size_t)c->dim * c->ncells));
+    if (!c->xn_dev || !c->met_dev || !c->pen_dev) {
         c->err = "device allocation failed";
         return DGX_ERR_HIP;
     }
@@ -547,6 +545,7 @@ static StageArgs base_args(dgx_ctx *c, const double *u)
     a.resid_scale = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
     a.status = c->status_dev;
     a.met = c->curv ? c->met_dev : nullptr;
+    a.pen = c->curv ? c->pen_dev : nullptr;
     return a;
 }
 

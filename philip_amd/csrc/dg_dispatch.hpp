@@ -29,10 +29,21 @@ void stage2_dn(dgx_ctx *c, const StageArgs &a, dim3 grid, dim3 block)
 {
     const bool split = c->P.use_split_form != 0;
     const bool nsv = c->P.pde == DGX_NAVIER_STOKES;
-    const bool bnd = c->has_boundary;  // Euler only
+    const bool bnd = c->has_boundary;
     using LG = LineGeoS<D, N, T>;
     const dim3 lgrid((c->ncells + LG::EPB - 1) / LG::EPB), lblock(LG::BLOCK);
-    if (c->curv) {  // curvilinear grid (Euler): stage2_kernel with the metric terms, boundaries included
+    if (c->curv) {  // curvilinear grid: stage2_kernel with the metric terms, boundaries included
+        if constexpr (D == 3) {
+            if (nsv) {
+                if (split)
+                    hipLaunchKernelGGL((stage2_kernel<D, N, true, true, T, true, true>), grid, block, 0, c->stream, a,
+                                       c->ops, c->pc);
+                else
+                    hipLaunchKernelGGL((stage2_kernel<D, N, false, true, T, true, true>), grid, block, 0, c->stream,
+                                       a, c->ops, c->pc);
+                return;
+            }
+        }
         if (split)
             hipLaunchKernelGGL((stage2_kernel<D, N, true, false, T, true, true>), grid, block, 0, c->stream, a, c->ops,
                                c->pc);
@@ -144,10 +155,12 @@ int launch_aux_dn(dgx_ctx *c, const double *u)
     const int blocks = (c->ncells + S::EPB - 1) / S::EPB;
     if (c->P.precision == DGX_MIXED)
         hipLaunchKernelGGL((aux_kernel<D, N, float>), dim3(blocks), dim3(S::BLOCK), 0, c->stream, u, c->auxf_dev[0],
-                           c->auxf_dev[1], c->auxf_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops, c->pc);
+                           c->auxf_dev[1], c->auxf_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops, c->pc,
+                           c->curv ? c->met_dev : nullptr);
     else
         hipLaunchKernelGGL((aux_kernel<D, N, double>), dim3(blocks), dim3(S::BLOCK), 0, c->stream, u, c->aux_dev[0],
-                           c->aux_dev[1], c->aux_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops, c->pc);
+                           c->aux_dev[1], c->aux_dev[2], c->nbr_dev, c->ncells, 1.0 / c->h, c->ops, c->pc,
+                           c->curv ? c->met_dev : nullptr);
     return check_launch(c, "aux_kernel");
 }
 

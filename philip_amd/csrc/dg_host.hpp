@@ -67,6 +67,7 @@ struct dgx_ctx {
     bool curv = false;             // curvilinear grid: metric terms per node in met_dev
     double *xn_dev = nullptr;      // node coordinates [cell][node][3] (allocated on demand)
     double *met_dev = nullptr;     // [cell][dim*dim + 1][node]: cofactor C[m][n] (slot m*dim+n), det J
+    double *pen_dev = nullptr;     // [cell][dim]: SIPG penalty of the cell's faces normal to each direction
     int *nbr_dev = nullptr;
     double *u_dev = nullptr;       // DGBase::solution
     double *r_dev = nullptr;       // DGBase::right_hand_side
@@ -116,7 +117,7 @@ int launch_mesh(dgx_ctx *c);
 // node coordinates of the box's cells into xn [cell][node][3]; warp = 1: through the reference's
 // curved periodic grid mapping
 int launch_coords(dgx_ctx *c, double *xn, int warp);
-// metric terms of c->xn_dev into c->met_dev
+// metric terms of c->xn_dev into c->met_dev, and the cells' penalties into c->pen_dev
 int launch_metrics(dgx_ctx *c);
 int launch_ic(dgx_ctx *c, int kind, int isothermal, double strength, double u_inf, double v_inf);
 int launch_maxabs(dgx_ctx *c, const double *a, const double *b, int64_t n, double *out);

@@ -66,6 +66,8 @@ struct StageArgs {
     // curvilinear grid: [cell][dim*dim + 1][node], the metric cofactor C[m][n] = J dxi_n/dx_m in
     // slot m*dim + n and det J in slot dim*dim; null on the affine box
     const double *met;
+    // curvilinear grid: SIPG penalty of each cell's faces normal to each direction [cell][dim]
+    const double *pen;
 };
 
 // Non-physical state check of one node (the reference checks every density and pressure it
@@ -214,10 +216,14 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs &a, const Ops1D &
 //   q_d = (1/h) [ D_d u + (1/w_0) (u_hat - u) n_d ]  on faces normal to d, u_hat = {{u}}.
 // Q: storage type of q (double for the FP64 path, float for the mixed path, whose
 // kernels evaluate the viscous fluxes in float).
+// met: curvilinear grid (the metric terms [cell][dim*dim + 1][node]; null on the affine box):
+//   q_k = (1/J) [ sum_j C[k][j] D_j u + (1/w_0) (u_hat - u) N_k ] with N the face's physical normal
+//   C n_ref at the node of the face's owner (the lower cell index; this cell on a boundary).
 template <int D, int N, typename Q>
 __global__ void __launch_bounds__((Shape<D, N>::BLOCK))
 aux_kernel(const double *__restrict__ u, Q *__restrict__ q0, Q *__restrict__ q1,
-           Q *__restrict__ q2, const int *__restrict__ nbr, int ncells, double inv_h, Ops1D ops, PhysConst pc)
+           Q *__restrict__ q2, const int *__restrict__ nbr, int ncells, double inv_h, Ops1D ops, PhysConst pc,
+           const double *__restrict__ met)
 {
     using S = Shape<D, N>;
     constexpr int NP = S::NP, NS = S::NS, EPB = S::EPB;
@@ -238,6 +244,11 @@ aux_kernel(const double *__restrict__ u, Q *__restrict__ q0, Q *__restrict__ q1,
     int c[3];
     node_coords<D, N>(i, c);
     Q *qout[3] = {q0, q1, q2};
+    constexpr int MS = D * D + 1;
+    double qp[D][NS];  // curvilinear grid: J times the physical gradient
+    if (met)
+        for (int k = 0; k < D; ++k)
+            for (int s = 0; s < NS; ++s) qp[k][s] = 0.0;
     for (int d = 0; d < D; ++d) {
         double g[NS];
         for (int s = 0; s < NS; ++s) {
@@ -245,6 +256,11 @@ aux_kernel(const double *__restrict__ u, Q *__restrict__ q0, Q *__restrict__ q1,
             for (int l = 0; l < N; ++l) acc += ops.D[c[d]][l] * sU[le][s][node_with<D, N>(i, d, l)];
             g[s] = acc;
         }
+        if (met)
+            for (int k = 0; k < D; ++k) {
+                const double ckd = met[((long)e * MS + k * D + d) * NP + i];
+                for (int s = 0; s < NS; ++s) qp[k][s] += ckd * g[s];
+            }
         for (int side = 0; side < 2; ++side) {
             if (c[d] != (side ? N - 1 : 0)) continue;
             const int nb = nbr[e * 2 * D + 2 * d + side];
@@ -254,13 +270,32 @@ aux_kernel(const double *__restrict__ u, Q *__restrict__ q0, Q *__restrict__ q1,
             // (assemble_boundary_term_auxiliary_equation, strong_dg.cpp:790-870)
             double ub[NS];
             if (nb < 0) visc_wall_state<double, D>(-nb, ui, pc, ub);
+            double lift[NS];
             for (int s = 0; s < NS; ++s) {
                 const double un = (nb < 0) ? ub[s] : u[((long)nb * NS + s) * NP + j];
                 const double avg = 0.5 * (ui[s] + un);
-                g[s] += ops.inv_wb * ((avg - ui[s]) * sgn);
+                lift[s] = ops.inv_wb * (avg - ui[s]);
+            }
+            if (met) {
+                // the face's physical normal is the owner's (the lower cell index, or this cell on a
+                // boundary): sgn C[:, d] at the owner's face node
+                const bool own = nb < 0 || e < nb;
+                for (int k = 0; k < D; ++k) {
+                    const double nk = sgn * (own ? met[((long)e * MS + k * D + d) * NP + i]
+                                                 : met[((long)nb * MS + k * D + d) * NP + j]);
+                    for (int s = 0; s < NS; ++s) qp[k][s] += lift[s] * nk;
+                }
+            } else {
+                for (int s = 0; s < NS; ++s) g[s] += lift[s] * sgn;
             }
         }
-        for (int s = 0; s < NS; ++s) qout[d][((long)e * NS + s) * NP + i] = Q(g[s] * inv_h);
+        if (!met)
+            for (int s = 0; s < NS; ++s) qout[d][((long)e * NS + s) * NP + i] = Q(g[s] * inv_h);
+    }
+    if (met) {
+        const double inv_j = 1.0 / met[((long)e * MS + D * D) * NP + i];
+        for (int k = 0; k < D; ++k)
+            for (int s = 0; s < NS; ++s) qout[k][((long)e * NS + s) * NP + i] = Q(qp[k][s] * inv_j);
     }
 }
 

@@ -483,8 +483,46 @@ template <int D, int N> static void metric_dn(dgx_ctx *c)
                        0, c->stream, c->xn_dev, c->met_dev, c->ncells, c->ops);
 }
 
+// SIPG penalty of a cell's faces normal to direction k on a curvilinear grid, as
+// evaluate_penalty_scaling (dg_base.cpp:397-413): deg (deg + 1) / extent_in_direction(k) times the
+// penalty factor, the extent being the longest of the cell's edges along k (between corner nodes).
+__global__ void penalty_kernel(const double *__restrict__ xn, double *__restrict__ pen, int ncells, int n, int dim,
+                               double scale)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ncells * dim) return;
+    const int cell = (int)(t / dim), k = (int)(t % dim);
+    const long np = (dim == 3) ? (long)n * n * n : (long)n * n;
+    double ext = 0.0;
+    for (int a = 0; a < (1 << (dim - 1)); ++a) {
+        int lo[3] = {0, 0, 0}, bit = 0;
+        for (int m = 0; m < dim; ++m) {
+            if (m == k) continue;
+            lo[m] = ((a >> bit) & 1) ? n - 1 : 0;
+            ++bit;
+        }
+        int hi[3] = {lo[0], lo[1], lo[2]};
+        hi[k] = n - 1;
+        const long q0 = lo[0] + n * (lo[1] + n * lo[2]), q1 = hi[0] + n * (hi[1] + n * hi[2]);
+        const double *x0 = xn + ((long)cell * np + q0) * 3, *x1 = xn + ((long)cell * np + q1) * 3;
+        double l2 = 0.0;
+        for (int m = 0; m < 3; ++m) l2 += (x1[m] - x0[m]) * (x1[m] - x0[m]);
+        ext = fmax(ext, sqrt(l2));
+    }
+    pen[t] = scale / ext;
+}
+
 int launch_metrics(dgx_ctx *c)
 {
+    if (c->pen_dev) {
+        const unsigned deg = (unsigned)c->p;
+        const double scale = (double)(deg == 0 ? 1 : deg * (deg + 1)) * c->P.sipg_penalty_factor;
+        const long nt = (long)c->ncells * c->dim;
+        hipLaunchKernelGGL(penalty_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, c->xn_dev,
+                           c->pen_dev, c->ncells, c->n, c->dim, scale);
+        const int rc = check_launch(c, "penalty_kernel");
+        if (rc) return rc;
+    }
 #define MET_CASE(NN)                                                                                        \
     case NN:                                                                                                \
         if (c->dim == 3) metric_dn<3, NN>(c);                                                               \

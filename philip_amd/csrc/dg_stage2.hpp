@@ -136,6 +136,12 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
             if (NSV)
                 for (int s = 0; s < NS; ++s)
                     for (int d = 0; d < D; ++d) g[s * D + d] = aux_at<T>(a, d, ((long)e * NS + s) * NP + i);
+            // CURV with the viscous terms: the physical viscous flux components, contracted below
+            T fvm[(CURV && NSV) ? D : 1][NS];
+            if constexpr (CURV && NSV) {
+#pragma unroll
+                for (int m = 0; m < D; ++m) viscous_flux_dir<T, D>(ut, g, m, pc, fvm[m]);
+            }
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 T f[NS];
@@ -155,7 +161,16 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                 }
                 if (NSV) {
                     T fv[NS];
-                    viscous_flux_dir<T, D>(ut, g, k, pc, fv);
+                    if constexpr (CURV) {
+                        for (int s = 0; s < NS; ++s) {
+                            T acc = T(0);
+#pragma unroll
+                            for (int m = 0; m < D; ++m) acc += fvm[m][s] * Cn[m * D + k];
+                            fv[s] = acc;
+                        }
+                    } else {
+                        viscous_flux_dir<T, D>(ut, g, k, pc, fv);
+                    }
                     for (int s = 0; s < NS; ++s) f[s] += fv[s];
                 }
                 for (int s = 0; s < NS; ++s) sF[le][k][s][i] = f[s];
@@ -218,8 +233,59 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                 const T fsgn = own ? T(1) : T(-1);
                 T ft[NS];
                 for (int s = 0; s < NS; ++s) ft[s] = -((area * fs[s]) * fsgn);
-                if (!SPLIT)
+                // the strong form's interior contravariant flux: convective and viscous (conservative
+                // form), viscous only (split form, whose convective part is in the two-point sums)
+                if (!SPLIT || NSV)
                     for (int s = 0; s < NS; ++s) ft[s] += sgn * sF[fle][K][s][fi];
+                if constexpr (NSV) {
+                    // viscous numerical flux along this element's outward unit normal, scaled by the
+                    // face area; walls as on the affine box; the penalty of the face from the cells'
+                    // extents (evaluate_penalty_scaling, dg_base.cpp:397-413, 563, 615-617)
+                    T ne[D];
+#pragma unroll
+                    for (int m = 0; m < D; ++m) ne[m] = own ? nh[m] : -nh[m];
+                    T ui[NS], gi[NS * D], gn[NS * D], jump[NS * D], uv[NS];
+                    bool adia = false;
+                    for (int s = 0; s < NS; ++s) {
+                        ui[s] = T(a.u[((long)fe * NS + s) * NP + fi]);
+                        for (int d = 0; d < D; ++d) gi[s * D + d] = aux_at<T>(a, d, ((long)fe * NS + s) * NP + fi);
+                    }
+                    if (nb < 0) {
+                        visc_wall_state<T, D>(-nb, ui, pc, uv);
+                        adia = (-nb == 1001) && pc.thermal_bc == 0;
+                        for (int s = 0; s < NS; ++s)
+                            for (int d = 0; d < D; ++d) gn[s * D + d] = (adia && s == NS - 1) ? T(0) : gi[s * D + d];
+                    } else {
+                        for (int s = 0; s < NS; ++s) {
+                            uv[s] = un[s];
+                            for (int d = 0; d < D; ++d) gn[s * D + d] = aux_at<T>(a, d, ((long)nb * NS + s) * NP + fj);
+                        }
+                    }
+                    for (int s = 0; s < NS; ++s)
+                        for (int d = 0; d < D; ++d) jump[s * D + d] = (ui[s] - uv[s]) * ne[d];
+                    auto vdot = [&](const T *uu, const T *gg, bool ad, T *out) {
+                        for (int s = 0; s < NS; ++s) out[s] = T(0);
+#pragma unroll
+                        for (int m = 0; m < D; ++m) {
+                            T fm[NS];
+                            viscous_flux_dir<T, D>(uu, gg, m, pc, fm, ad);
+                            for (int s = 0; s < NS; ++s) out[s] += fm[s] * ne[m];
+                        }
+                    };
+                    T fa[NS], fb[NS], fvs[NS];
+                    vdot(ui, gi, adia, fa);
+                    vdot(uv, gn, adia, fb);
+                    for (int s = 0; s < NS; ++s) fvs[s] = T(0.5) * (fa[s] + fb[s]);
+                    if (pc.diss_flux == 0) {
+                        const double pe = a.pen[(long)fe * D + K];
+                        const T pen = T(nb < 0 ? pe : 0.5 * (pe + a.pen[(long)nb * D + K]));
+                        T ja[NS], jb[NS];
+                        vdot(ui, jump, adia, ja);
+                        vdot(uv, jump, adia, jb);
+                        for (int s = 0; s < NS; ++s) fvs[s] = fvs[s] - pen * (T(0.5) * (ja[s] + jb[s]));
+                    }
+                    for (int s = 0; s < NS; ++s) ft[s] -= area * fvs[s];
+                }
                 for (int s = 0; s < NS; ++s) sFace[fle][K][s][fi] = ft[s];
                 return;
             }

@@ -2,7 +2,9 @@
 curved periodic grid (NonsymmetricCurvedGridManifold, beta = 0.1) built on the device, caller-
 supplied node coordinates, metric terms (the 3D conservative curl form), and the curvilinear split
 and conservative forms with their face terms (stage2_kernel with the metric, strong_dg.cpp:1267-1380,
-3162-3350).
+3162-3350), and Navier-Stokes on the curved grid (the lifted gradient through the metric,
+strong_dg.cpp:700-870; viscous fluxes contracted with the cofactor and the face fluxes along the
+physical normal with each cell's penalty from its extent, dg_base.cpp:397-413).
 
 Bars: node coordinates within 1e-14 and det J within 1e-12 relative, the cofactor within 1e-11
 (the curl form's cancellation at p = 6); FP64 residual per DoF within 1e-12
@@ -219,8 +221,6 @@ def test_unsupported_combinations_raise():
     with pytest.raises(P.DeviceError):
         d.integrated()
     with pytest.raises(P.DeviceError):
-        P.DG(dim=3, poly_degree=2, pde=P.NAVIER_STOKES, grid_warp=1)
-    with pytest.raises(P.DeviceError):
         P.DG(dim=3, poly_degree=2, grid_warp=2)
     plain = P.DG(dim=3, poly_degree=2)
     with pytest.raises(P.DeviceError):
@@ -235,3 +235,82 @@ def test_initial_condition_at_the_curved_nodes():
     assert np.abs(u - uo).max() <= 1e-14 * np.abs(uo).max()
     # the curved nodes, not the box's: the state differs from the affine grid's
     assert np.abs(uo - O.Oracle(**dict(kw, grid_warp=0)).ic_tgv()).max() > 1e-3
+
+
+# ---------------------------------------------------------------------------------------------
+# Navier-Stokes on curvilinear grids
+# ---------------------------------------------------------------------------------------------
+
+def ns_kw(split, diss, p, n=3, **extra):
+    sch = dict(use_split_form=1, two_point_flux=O.RA, conv_flux=O.TWO_POINT_ROE) if split else dict(conv_flux=O.ROE)
+    kw = dict(dim=3, poly_degree=p, n_cells_1d=n, pde=O.NAVIER_STOKES, diss_flux=diss, reynolds_inf=100.0,
+              prandtl=0.71, mach_inf=0.3, domain_left=0.0, domain_right=2 * np.pi, grid_warp=1, **sch)
+    kw.update(extra)
+    return kw
+
+
+def check_ns(o, d, u, label):
+    r = check(o, d, u, label)
+    for k in range(3):
+        q, q_ref = d.auxiliary_solution(k), o.auxiliary(k)
+        eq = np.abs(q - q_ref).max() / np.abs(q_ref).max()
+        assert eq <= TOL, (label, k, eq)
+    return r
+
+
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("diss", [O.SIPG, O.BR2])
+@pytest.mark.parametrize("p", [2, 3, 5])
+def test_navier_stokes_on_the_curved_grid(split, diss, p):
+    kw = ns_kw(split, diss, p, n=3 if p < 5 else 2)
+    log(f"curved grid NS split={split} diss={diss} p={p}")
+    o, d = O.Oracle(**kw), P.DG(**kw)
+    r = check_ns(o, d, random_field(o, seed=40 + p, vel=0.3), f"NS split={split} diss={diss} p={p}")
+    assert np.array_equal(r, d.assemble_residual())
+
+
+WALL_SETS = [("noslip_adiabatic", [1001] * 6, 0, 1.0), ("noslip_isothermal", [1001] * 6, 1, 1.2),
+             ("channel", [0, 0, 1001, 1001, 1006, 1006], 1, 0.9)]
+
+
+@pytest.mark.parametrize("name,bcs,thermal,tw", WALL_SETS, ids=[w[0] for w in WALL_SETS])
+@pytest.mark.parametrize("split", [0, 1])
+def test_navier_stokes_walls_on_caller_nodes(name, bcs, thermal, tw, split):
+    # curved walls: the quadratic map bends every face of the unit box
+    kw = ns_kw(split, O.SIPG, 3, grid_warp=0, domain_right=1.0, boundary_type=bcs, thermal_bc=thermal,
+               wall_temperature=tw)
+    log(f"curved walls {name} split={split}")
+    o, d = O.Oracle(**kw), P.DG(**kw)
+    x = quadratic_map(3, o.node_coords())
+    o.set_grid_nodes(x)
+    d.set_grid_nodes(x)
+    check_ns(o, d, random_field(o, seed=6, vel=0.3), f"curved walls {name} split={split}")
+
+
+@pytest.mark.parametrize("split", [0, 1])
+def test_navier_stokes_stepping_on_the_curved_grid(split):
+    kw = ns_kw(split, O.SIPG, 3)
+    o, d = O.Oracle(**kw), P.DG(**kw)
+    u = random_field(o, seed=12, vel=0.2)
+    d.set_solution(u)
+    dt = 1e-3
+    d.allocate_ode_system("rk4")
+    d.step_in_time(dt, 3)
+    for _ in range(3):
+        o.step("rk4", dt, u)
+    e = np.abs(d.solution() - u).max() / np.abs(u).max()
+    log(f"  NS stepping split={split}: rel err {e:.2e} (bar {TOL:.0e})")
+    assert e <= TOL
+
+
+def test_navier_stokes_mixed_precision_on_the_curved_grid():
+    for split in (0, 1):
+        kw = ns_kw(split, O.SIPG, 3)
+        o = O.Oracle(**kw)
+        d = P.DG(precision=P.MIXED, **kw)
+        u = random_field(o, seed=13, vel=0.3)
+        d.set_solution(u)
+        r, r_ref = d.assemble_residual(), o.residual(u)
+        e = max(np.abs(r - r_ref).max() / np.abs(r_ref).max(), np.linalg.norm(r - r_ref) / np.linalg.norm(r_ref))
+        log(f"  NS mixed split={split}: rel err {e:.2e} (bar 5e-4)")
+        assert e <= 5e-4

@@ -278,3 +278,63 @@ def test_smooth_residual_converges_to_the_flux_divergence(name, kw):
         errs.append(err)
     rate = np.log2(errs[0] / errs[1])
     assert errs[1] < 1e-2 and rate > 3.5, (errs, rate)
+
+
+# --- Navier-Stokes on curvilinear grids: the lifted gradient C grad_ref u / J and the viscous
+# fluxes contracted with the cofactor (strong_dg.cpp:690-870 with the metric terms) ---
+
+def ns_kw(split=1, diss=O.SIPG, **extra):
+    sch = dict(use_split_form=1, two_point_flux=O.RA, conv_flux=O.TWO_POINT_ROE) if split else dict(conv_flux=O.ROE)
+    kw = dict(dim=3, poly_degree=3, n_cells_1d=3, pde=O.NAVIER_STOKES, diss_flux=diss, reynolds_inf=100.0,
+              prandtl=0.71, mach_inf=0.3, **sch)
+    kw.update(extra)
+    return kw
+
+
+def test_lifted_gradient_of_a_linear_field_is_exact_on_a_resolved_map():
+    # multilinear map: the curl form is the exact cofactor, the node values of a linear field are its
+    # interpolant, so C grad_ref u / J is the field's gradient at every node of the cells away from
+    # the periodic wrap (where the linear field jumps)
+    o = O.Oracle(**ns_kw(domain_left=0.0, domain_right=1.0, n_cells_1d=4))
+    x = quadratic_map(3, o.node_coords())[0]
+    o.set_grid_nodes(x)
+    a = np.array([[0.3, -0.2, 0.1], [0.05, 0.2, -0.1], [-0.1, 0.04, 0.3], [0.2, 0.1, -0.05], [1.0, -0.5, 0.25]])
+    base = np.array([1.0, 0.1, -0.1, 0.05, 9.0])
+    u = (base[None, :, None] + np.einsum("sk,cnk->csn", a, x)).ravel()
+    o.residual(u)
+    inner = np.all((o.cell_ijk() >= 1) & (o.cell_ijk() <= 2), axis=1)
+    assert inner.sum() == 8
+    for k in range(3):
+        q = o.auxiliary(k).reshape(o.ncells, o.ns, o.Np)[inner]
+        assert np.abs(q - a[None, :, k, None]).max() < 1e-12
+
+
+@pytest.mark.parametrize("split,diss", [(0, O.SIPG), (1, O.SIPG), (1, O.BR2)])
+def test_navier_stokes_free_stream_and_conservation_on_the_curved_grid(split, diss):
+    o = O.Oracle(**ns_kw(split, diss, domain_left=0.0, domain_right=2 * np.pi, grid_warp=1))
+    uinf = cons(1.2, [0.3, -0.2, 0.1], 1.0 / (GAM * 0.09))
+    u = np.tile(uinf[:, None], (o.ncells, 1, o.Np)).ravel()
+    r = o.residual(u)
+    assert np.linalg.norm(r) < 1e-13 * uinf[-1] * np.sqrt(r.size)
+    assert max(np.abs(o.auxiliary(k)).max() for k in range(3)) < 1e-12
+    v = random_field(o, 6)
+    r = o.residual(v).reshape(o.ncells, o.ns, o.Np)
+    assert np.abs(r.sum(axis=(0, 2))).max() / (np.abs(r).max() * r.size / o.ns) < 1e-14
+
+
+@pytest.mark.parametrize("split,diss", [(0, O.SIPG), (1, O.BR2)])
+def test_navier_stokes_rotated_grid_rotates_the_residual(split, diss):
+    base = ns_kw(split, diss, domain_left=0.0, domain_right=1.0)
+    o, orot = O.Oracle(**base), O.Oracle(**base)
+    x = quadratic_map(3, o.node_coords())[0]
+    o.set_grid_nodes(x)
+    R = rotation(3)
+    orot.set_grid_nodes(x @ R.T)
+    u = random_field(o, 7).reshape(o.ncells, o.ns, o.Np)
+    ur = u.copy()
+    ur[:, 1:4] = np.einsum("ij,cjn->cin", R, u[:, 1:4])
+    r = o.residual(u.ravel()).reshape(u.shape)
+    rr = orot.residual(ur.ravel()).reshape(u.shape)
+    expect = r.copy()
+    expect[:, 1:4] = np.einsum("ij,cjn->cin", R, r[:, 1:4])
+    assert np.abs(rr - expect).max() < 1e-12 * np.abs(r).max()
