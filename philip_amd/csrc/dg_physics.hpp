@@ -675,9 +675,10 @@ __device__ __forceinline__ void viscous_flux_dir(const T *u, const T *g, int fd,
 
 // Dissipative flux in every direction at once, f[k] = F_v(u, grad u) . e_k: the primitive
 // gradient, viscosity and divergence are shared between the directions; each direction's
-// arithmetic is that of viscous_flux_dir.
+// arithmetic is that of viscous_flux_dir (adiabatic: without the heat flux, as there).
 template <typename T, int D>
-__device__ __forceinline__ void viscous_flux_all(const T *u, const T *g, const PhysConst &pc, T (&f)[D][D + 2])
+__device__ __forceinline__ void viscous_flux_all(const T *u, const T *g, const PhysConst &pc, T (&f)[D][D + 2],
+                                                 bool adiabatic = false)
 {
     const T gam = T(pc.gam), gamm1 = T(pc.gamm1), mach2 = T(pc.mach2);
     const T rho = u[0];
@@ -713,7 +714,7 @@ __device__ __forceinline__ void viscous_flux_all(const T *u, const T *g, const P
             if (a == fd) tau_fd[a] += mu2 * divv;
         }
         const T dT = (gam * mach2 * pg[D + 1][fd] - T_ * pg[0][fd]) / rho;
-        const T q = -kappa * dT;
+        const T q = adiabatic ? T(0) : -kappa * dT;
         f[fd][0] = T(0);
         for (int a = 0; a < D; ++a) f[fd][1 + a] = -tau_fd[a];
         T fe = T(0);

@@ -137,11 +137,8 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                 for (int s = 0; s < NS; ++s)
                     for (int d = 0; d < D; ++d) g[s * D + d] = aux_at<T>(a, d, ((long)e * NS + s) * NP + i);
             // CURV with the viscous terms: the physical viscous flux components, contracted below
-            T fvm[(CURV && NSV) ? D : 1][NS];
-            if constexpr (CURV && NSV) {
-#pragma unroll
-                for (int m = 0; m < D; ++m) viscous_flux_dir<T, D>(ut, g, m, pc, fvm[m]);
-            }
+            T fvm[D][NS];
+            if constexpr (CURV && NSV) viscous_flux_all<T, D>(ut, g, pc, fvm);
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 T f[NS];
@@ -244,7 +241,7 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                     T ne[D];
 #pragma unroll
                     for (int m = 0; m < D; ++m) ne[m] = own ? nh[m] : -nh[m];
-                    T ui[NS], gi[NS * D], gn[NS * D], jump[NS * D], uv[NS];
+                    T ui[NS], gi[NS * D], gn[NS * D], uv[NS];
                     bool adia = false;
                     for (int s = 0; s < NS; ++s) {
                         ui[s] = T(a.u[((long)fe * NS + s) * NP + fi]);
@@ -261,28 +258,28 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                             for (int d = 0; d < D; ++d) gn[s * D + d] = aux_at<T>(a, d, ((long)nb * NS + s) * NP + fj);
                         }
                     }
-                    for (int s = 0; s < NS; ++s)
-                        for (int d = 0; d < D; ++d) jump[s * D + d] = (ui[s] - uv[s]) * ne[d];
-                    auto vdot = [&](const T *uu, const T *gg, bool ad, T *out) {
-                        for (int s = 0; s < NS; ++s) out[s] = T(0);
-#pragma unroll
-                        for (int m = 0; m < D; ++m) {
-                            T fm[NS];
-                            viscous_flux_dir<T, D>(uu, gg, m, pc, fm, ad);
-                            for (int s = 0; s < NS; ++s) out[s] += fm[s] * ne[m];
-                        }
-                    };
-                    T fa[NS], fb[NS], fvs[NS];
-                    vdot(ui, gi, adia, fa);
-                    vdot(uv, gn, adia, fb);
-                    for (int s = 0; s < NS; ++s) fvs[s] = T(0.5) * (fa[s] + fb[s]);
+                    // the viscous flux is linear in the gradient: central flux and SIPG penalty
+                    // 1/2 [F(u_i, g_i - pen J) + F(u_v, g_v - pen J)] . n in two evaluations, J the
+                    // jump (u_i - u_v) n
+                    T pen = T(0);
                     if (pc.diss_flux == 0) {
                         const double pe = a.pen[(long)fe * D + K];
-                        const T pen = T(nb < 0 ? pe : 0.5 * (pe + a.pen[(long)nb * D + K]));
-                        T ja[NS], jb[NS];
-                        vdot(ui, jump, adia, ja);
-                        vdot(uv, jump, adia, jb);
-                        for (int s = 0; s < NS; ++s) fvs[s] = fvs[s] - pen * (T(0.5) * (ja[s] + jb[s]));
+                        pen = T(nb < 0 ? pe : 0.5 * (pe + a.pen[(long)nb * D + K]));
+                    }
+                    for (int s = 0; s < NS; ++s)
+                        for (int d = 0; d < D; ++d) {
+                            const T jd = pen * ((ui[s] - uv[s]) * ne[d]);
+                            gi[s * D + d] -= jd;
+                            gn[s * D + d] -= jd;
+                        }
+                    T Fa[D][NS], Fb[D][NS], fvs[NS];
+                    viscous_flux_all<T, D>(ui, gi, pc, Fa, adia);
+                    viscous_flux_all<T, D>(uv, gn, pc, Fb, adia);
+                    for (int s = 0; s < NS; ++s) {
+                        T acc = T(0);
+#pragma unroll
+                        for (int m = 0; m < D; ++m) acc += (Fa[m][s] + Fb[m][s]) * ne[m];
+                        fvs[s] = T(0.5) * acc;
                     }
                     for (int s = 0; s < NS; ++s) ft[s] -= area * fvs[s];
                 }
@@ -317,10 +314,12 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
             const T fsgn = own ? T(1) : T(-1);
             T ft[NS];
             for (int s = 0; s < NS; ++s) ft[s] = -(fs[s] * fsgn);
-            if (!SPLIT)
+            // the strong form's interior flux: convective and viscous (conservative form), viscous
+            // only (split form), with its heat flux also on an adiabatic wall
+            if (!SPLIT || NSV)
                 for (int s = 0; s < NS; ++s) ft[s] += sgn * sF[fle][K][s][fi];
             if (NSV) {
-                T ui[NS], gi[NS * D], gn[NS * D], jump[NS * D], uv[NS];
+                T ui[NS], gi[NS * D], gn[NS * D], uv[NS];
                 bool wall = false, adia = false;
                 if constexpr (BND) wall = nb < 0;
                 for (int s = 0; s < NS; ++s) {
@@ -341,27 +340,19 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
                         for (int d = 0; d < D; ++d) gn[s * D + d] = aux_at<T>(a, d, ((long)nb * NS + s) * NP + fj);
                     }
                 }
-                for (int s = 0; s < NS; ++s)
-                    for (int d = 0; d < D; ++d)
-                        jump[s * D + d] = (d == K) ? (ui[s] - uv[s]) * sgn : (ui[s] - uv[s]) * T(0);
+                // the viscous flux is linear in the gradient: central flux and SIPG penalty
+                // 1/2 [F(u_i, g_i - pen J) + F(u_v, g_v - pen J)] . n in two evaluations, the jump
+                // J = (u_i - u_v) n nonzero in column K only
+                const T pen = (pc.diss_flux == 0) ? T(a.penalty) : T(0);
+                for (int s = 0; s < NS; ++s) {
+                    const T jk = pen * ((ui[s] - uv[s]) * sgn);
+                    gi[s * D + K] -= jk;
+                    gn[s * D + K] -= jk;
+                }
                 T fa[NS], fb[NS];
                 viscous_flux_dir<T, D>(ui, gi, K, pc, fa, adia);
                 viscous_flux_dir<T, D>(uv, gn, K, pc, fb, adia);
-                T fvs[NS];
-                for (int s = 0; s < NS; ++s) fvs[s] = T(0.5) * (fa[s] * sgn + fb[s] * sgn);
-                if (pc.diss_flux == 0) {
-                    T ja[NS], jb[NS];
-                    viscous_flux_dir<T, D>(ui, jump, K, pc, ja, adia);
-                    viscous_flux_dir<T, D>(uv, jump, K, pc, jb, adia);
-                    const T pen = T(a.penalty);
-                    for (int s = 0; s < NS; ++s) fvs[s] = fvs[s] - pen * (T(0.5) * (ja[s] * sgn + jb[s] * sgn));
-                }
-                for (int s = 0; s < NS; ++s) ft[s] -= fvs[s];
-                if (SPLIT) {
-                    // the strong form's interior viscous flux (with its heat flux)
-                    if (adia) viscous_flux_dir<T, D>(ui, gi, K, pc, fa);
-                    for (int s = 0; s < NS; ++s) ft[s] += sgn * fa[s];
-                }
+                for (int s = 0; s < NS; ++s) ft[s] -= T(0.5) * (fa[s] * sgn + fb[s] * sgn);
             }
             for (int s = 0; s < NS; ++s) sFace[fle][K][s][fi] = ft[s];
         };
