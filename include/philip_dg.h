@@ -133,7 +133,7 @@ int dgx_get_neighbors(const dgx_ctx *ctx, int32_t *host_nbr);
    terms of metric_operators (operators.cpp:2520-2884: Jacobian, det J, and in 3D the conservative
    curl form of the cofactor) and each cell's SIPG penalty from its extent along each direction
    (evaluate_penalty_scaling, dg_base.cpp:397-413).  The box topology (periodic or boundary faces) is
-   kept.  The batched line kernel and the integrated quantities do not take curvilinear contexts. */
+   kept.  The batched line kernel does not take curvilinear contexts. */
 int dgx_set_grid_nodes(dgx_ctx *ctx, const double *host_xyz);
 /* node coordinates in use [cell][node][3] (the affine box if none were set) */
 int dgx_get_grid_nodes(dgx_ctx *ctx, double *host_xyz);
@@ -204,7 +204,9 @@ double dgx_relaxation_parameter(const dgx_ctx *ctx);
 double dgx_constant_time_step(const dgx_ctx *ctx, double cfl);
 /* PeriodicTurbulence::compute_and_update_integrated_quantities (periodic_turbulence.cpp:439-671)
    with QGauss(p+1+overint): out[0]=kinetic energy, [1]=enstrophy, [2]=incompressible KE,
-   [3]=vorticity-based dissipation rate; reductions in FP64 on the device. */
+   [3]=vorticity-based dissipation rate; reductions in FP64 on the device.  On a curvilinear grid
+   the metric terms are built at the QGauss points as the reference's metric operators do
+   (periodic_turbulence.cpp:487-492). */
 int dgx_integrated_quantities(dgx_ctx *ctx, int overint, double *out4);
 /* Allocation-free variant of dgx_integrated_quantities.  No counterpart in the reference: it is
    for callers that cannot allocate device memory (a time loop captured in a HIP graph that

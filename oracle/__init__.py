@@ -431,7 +431,7 @@ def set_threads(n):
 
 
 (MUT_NONE, MUT_LXF_MIN, MUT_LXF_NO_HALF, MUT_ROE_NO_HALF, MUT_ROE_NO_HARTEN, MUT_ROE_NO_SHEAR,
- MUT_NO_METRIC_AVERAGE, MUT_CROSS_PRODUCT_METRIC) = range(8)
+ MUT_NO_METRIC_AVERAGE, MUT_CROSS_PRODUCT_METRIC, MUT_TRANSPOSED_METRIC, MUT_CELL_CONSTANT_DETJ) = range(10)
 
 
 def set_mutation(m):

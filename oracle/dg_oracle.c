@@ -98,8 +98,10 @@ static void legendre(int n, double x, double *pn, double *dpn)
 
 void dgo_gauss_legendre(int n, double *x, double *w)
 {
-    /* Newton on P_n from Chebyshev guesses; mapped to [0,1] (weights sum 1) */
-    double xt[MAXN], wt[MAXN];
+    /* Newton on P_n from Chebyshev guesses; mapped to [0,1] (weights sum 1); n <= 64 (the
+       over-integration rules QGauss(p + 1 + overint) exceed the basis size) */
+    double xt[64], wt[64];
+    if (n > 64) { fprintf(stderr, "dgo_gauss_legendre: at most 64 points\n"); abort(); }
     for (int i = 0; i < n; ++i) {
         double z = cos(M_PI * (i + 0.75) / (n + 0.5));
         for (int it = 0; it < 100; ++it) {
@@ -2478,13 +2480,83 @@ double dgo_vortex_time_step(const dgo_ctx *c, double cfl, double speed)
 /* ------------------------------------------------------------------------ */
 /* integrated quantities (periodic_turbulence.cpp:439-671)                   */
 /* ------------------------------------------------------------------------ */
+/* The metric terms of curvilinear cell e at the QGauss points (metric_operators::
+   build_volume_metric_operators with the points as flux nodes, operators.cpp:2520-2884, as
+   periodic_turbulence.cpp:487-492 builds them): the Jacobian of the interpolated mapping at the points
+   (B, G: 1D values and derivatives of the grid-node Lagrange basis there), det J, and the cofactor --
+   in 2D the adjugate of that Jacobian, in 3D the conservative curl form: the grid-node products
+   X_l dX_m/dxi_j (their derivatives taken with the nodal derivative matrix) differentiated through
+   their interpolant at the points.  Cq[(a*3+b)*nqv + q], Jq[q]. */
+static void metrics_at_points(const dgo_ctx *c, int e, const double *B, const double *G, int nx, double *Cq,
+                              double *Jq)
+{
+    const int d = c->dim, n = c->n, Np = c->Np;
+    const int nqv = (d == 3) ? nx * nx * nx : nx * nx;
+    double *X = (double *)malloc(sizeof(double) * 3 * Np);
+    double *Jm = (double *)malloc(sizeof(double) * 9 * Np);
+    double *P = (double *)malloc(sizeof(double) * 9 * Np);
+    double *Jp = (double *)malloc(sizeof(double) * 9 * nqv);
+    for (int i = 0; i < d; ++i)
+        for (int q = 0; q < Np; ++q) X[i * Np + q] = c->xn[((long)e * Np + q) * 3 + i];
+    /* the Jacobian at the points: dX_i/dxi_j of the interpolant */
+    for (int i = 0; i < d; ++i)
+        for (int j = 0; j < d; ++j)
+            dgo_tensor_apply(d, j == 0 ? G : B, nx, n, j == 1 ? G : B, nx, n, j == 2 ? G : B, nx, n, X + i * Np,
+                             Jp + (i * 3 + j) * nqv, 0, 1.0);
+    memset(Cq, 0, sizeof(double) * 9 * nqv);
+    if (d == 2) {
+        for (int q = 0; q < nqv; ++q) {
+            Cq[(0 * 3 + 0) * nqv + q] = Jp[(1 * 3 + 1) * nqv + q];
+            Cq[(1 * 3 + 0) * nqv + q] = -Jp[(0 * 3 + 1) * nqv + q];
+            Cq[(0 * 3 + 1) * nqv + q] = -Jp[(1 * 3 + 0) * nqv + q];
+            Cq[(1 * 3 + 1) * nqv + q] = Jp[(0 * 3 + 0) * nqv + q];
+        }
+    } else {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) dir_apply(c, j, c->chi_grad, c->chi, n, n, X + i * Np, Jm + (i * 3 + j) * Np, 0, 1.0);
+        const int lm[3][2] = {{2, 1}, {0, 2}, {1, 0}};
+        for (int r = 0; r < 3; ++r)
+            for (int j = 0; j < 3; ++j)
+                for (int q = 0; q < Np; ++q)
+                    P[(r * 3 + j) * Np + q] = X[lm[r][0] * Np + q] * Jm[(lm[r][1] * 3 + j) * Np + q];
+        /* directional derivative of a grid-node field at the points */
+        #define DPT(dir, in, out, add, f) \
+            dgo_tensor_apply(3, (dir) == 0 ? G : B, nx, n, (dir) == 1 ? G : B, nx, n, (dir) == 2 ? G : B, nx, n, \
+                             (in), (out), (add), (f))
+        for (int r = 0; r < 3; ++r) {
+            const double *Pxi = P + (r * 3 + 0) * Np, *Peta = P + (r * 3 + 1) * Np, *Pzeta = P + (r * 3 + 2) * Np;
+            double *C0 = Cq + (r * 3 + 0) * nqv, *C1 = Cq + (r * 3 + 1) * nqv, *C2 = Cq + (r * 3 + 2) * nqv;
+            DPT(1, Pzeta, C0, 0, -1.0);
+            DPT(2, Peta, C0, 1, 1.0);
+            DPT(0, Pzeta, C1, 0, 1.0);
+            DPT(2, Pxi, C1, 1, -1.0);
+            DPT(0, Peta, C2, 0, -1.0);
+            DPT(1, Pxi, C2, 1, 1.0);
+        }
+        #undef DPT
+    }
+    for (int q = 0; q < nqv; ++q) {
+        double J[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) J[i][j] = (i < d && j < d) ? Jp[(i * 3 + j) * nqv + q] : 0.0;
+        if (d == 2) Jq[q] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        else Jq[q] = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0])
+                   + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        if (g_mutation == DGO_MUT_TRANSPOSED_METRIC)
+            /* planted: the cofactor transposed in the gradient transform */
+            for (int a = 0; a < 3; ++a)
+                for (int b = a + 1; b < 3; ++b) {
+                    const double t = Cq[(a * 3 + b) * nqv + q];
+                    Cq[(a * 3 + b) * nqv + q] = Cq[(b * 3 + a) * nqv + q];
+                    Cq[(b * 3 + a) * nqv + q] = t;
+                }
+        if (g_mutation == DGO_MUT_CELL_CONSTANT_DETJ) Jq[q] = c->detJ;  /* planted: the box's det J */
+    }
+    free(X); free(Jm); free(P); free(Jp);
+}
+
 void dgo_integrated_quantities(const dgo_ctx *c, const double *u, int overint, double *out)
 {
-    if (c->curv) {
-        /* over-integration on a curvilinear grid needs the metric at the QGauss points: not restated */
-        for (int q = 0; q < 4; ++q) out[q] = NAN;
-        return;
-    }
     const int d = c->dim, ns = c->ns, n = c->n, Np = c->Np;
     const int nx = c->p + 1 + overint;
     double xq[64], wq[64];
@@ -2508,7 +2580,10 @@ void dgo_integrated_quantities(const dgo_ctx *c, const double *u, int overint, d
     double *coef = (double *)malloc(sizeof(double) * Np);
     double C[3][3];
     metric_cofactor(c, C);
+    double *Cq = c->curv ? (double *)malloc(sizeof(double) * 9 * nqv) : NULL;
+    double *Jq = c->curv ? (double *)malloc(sizeof(double) * nqv) : NULL;
     for (int e = 0; e < c->ncells; ++e) {
+        if (c->curv) metrics_at_points(c, e, B, G, nx, Cq, Jq);
         for (int s = 0; s < ns; ++s) {
             for (int i = 0; i < Np; ++i) coef[i] = u[IDX(c, e, s, i)];
             dgo_tensor_apply(d, B, nx, n, B, nx, n, B, nx, n, coef, sq + s * nqv, 0, 1.0);
@@ -2518,11 +2593,15 @@ void dgo_integrated_quantities(const dgo_ctx *c, const double *u, int overint, d
         }
         for (int q = 0; q < nqv; ++q) {
             double st[NS_MAX], g[NS_MAX * 3];
+            const double jq = c->curv ? Jq[q] : c->detJ;
+            if (c->curv)
+                for (int a = 0; a < d; ++a)
+                    for (int b = 0; b < d; ++b) C[a][b] = Cq[(a * 3 + b) * nqv + q];
             for (int s = 0; s < ns; ++s) {
                 st[s] = sq[s * nqv + q];
                 for (int k = 0; k < d; ++k) {
                     double acc = 0.0;
-                    for (int j = 0; j < d; ++j) acc += C[k][j] * gq[(s * d + j) * nqv + q] / c->detJ;
+                    for (int j = 0; j < d; ++j) acc += C[k][j] * gq[(s * d + j) * nqv + q] / jq;
                     g[s * d + k] = acc;
                 }
             }
@@ -2540,7 +2619,7 @@ void dgo_integrated_quantities(const dgo_ctx *c, const double *u, int overint, d
             double om2 = 0.0;
             for (int k = 0; k < 3; ++k) om2 += om[k] * om[k];
             const double ENS = 0.5 * st[0] * om2;
-            const double w = W[q] * c->detJ;
+            const double w = W[q] * jq;
             ke += KE * w;
             ens += ENS * w;
             ike += 0.5 * vel2 * w;
@@ -2552,7 +2631,7 @@ void dgo_integrated_quantities(const dgo_ctx *c, const double *u, int overint, d
     out[1] = ens / vol;
     out[2] = ike / vol;
     out[3] = 2.0 * out[1] / c->P.reynolds_inf;
-    free(B); free(G); free(W); free(sq); free(gq); free(coef);
+    free(B); free(G); free(W); free(sq); free(gq); free(coef); free(Cq); free(Jq);
 }
 
 double dgo_entropy_rate(dgo_ctx *c, const double *u)

@@ -50,13 +50,16 @@ enum { DGO_ORDER_DEALII = 0, DGO_ORDER_LEXICOGRAPHIC = 1 };
 /* planted mistakes (tests of the tests only): LxF wave speed min instead of max, LxF or Roe
    dissipation without its factor 1/2, Roe without Harten's entropy fix, Roe without the shear
    (tangential velocity jump) wave, the curvilinear two-point flux without its metric average, the 3D
-   cofactor without the conservative curl form */
+   cofactor without the conservative curl form, the QGauss-point metric of the integrated quantities
+   transposed or replaced by the box's det J */
 enum {
     DGO_MUT_NONE = 0, DGO_MUT_LXF_MIN = 1, DGO_MUT_LXF_NO_HALF = 2, DGO_MUT_ROE_NO_HALF = 3,
     DGO_MUT_ROE_NO_HARTEN = 4, DGO_MUT_ROE_NO_SHEAR = 5,
     DGO_MUT_NO_METRIC_AVERAGE = 6,  /* curvilinear split form: the node's own cofactor in place of the
                                        two-point average */
-    DGO_MUT_CROSS_PRODUCT_METRIC = 7 /* 3D cofactor det(J) J^-T pointwise instead of the curl form */
+    DGO_MUT_CROSS_PRODUCT_METRIC = 7, /* 3D cofactor det(J) J^-T pointwise instead of the curl form */
+    DGO_MUT_TRANSPOSED_METRIC = 8,    /* integrated quantities on curved cells: cofactor transposed */
+    DGO_MUT_CELL_CONSTANT_DETJ = 9    /* integrated quantities on curved cells: the box's det J */
 };
 
 typedef struct {

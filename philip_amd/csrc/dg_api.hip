@@ -1157,24 +1157,15 @@ int dgx_set_initial_condition_vortex(dgx_ctx *c, double strength, double u_inf, 
     return DGX_OK;
 }
 
-static int curv_unsupported(dgx_ctx *c)
-{
-    c->err = "integrated quantities on a curvilinear grid need the metric at the over-integration points: "
-             "not built";
-    return DGX_ERR_UNSUPPORTED;
-}
-
 int dgx_integrated_quantities(dgx_ctx *c, int overint, double *out4)
 {
     if (!c || !out4) return DGX_ERR_INVALID;
-    if (c->curv) return curv_unsupported(c);
     return launch_integrated(c, c->u_dev, overint, out4);
 }
 
 int dgx_integrated_quantities_static(dgx_ctx *c, const double *u_dev, int overint, double *out4_dev)
 {
     if (!c || !out4_dev) return DGX_ERR_INVALID;
-    if (c->curv) return curv_unsupported(c);
     return launch_integrated_static(c, u_dev ? u_dev : c->u_dev, overint, out4_dev);
 }
 

@@ -693,6 +693,197 @@ integrated_kernel(const double *__restrict__ u, const double *__restrict__ ops, 
         for (int f = 0; f < 3; ++f) partial[(long)e * 3 + f] = red[f * IQ_THREADS];
 }
 
+// Integrated quantities on a curvilinear cell (periodic_turbulence.cpp:439-671 with the metric
+// operators built at the QGauss points, :487-492 / operators.cpp:2520-2884): one block per cell,
+// density, momentum, node coordinates and, in 3D, the conservative curl form's grid-node products
+// P[r][j] = X_l dX_m/dxi_j ((l, m) = (z, y), (x, z), (y, x)) in LDS; each thread evaluates its
+// points directly from the tensor-product Lagrange basis: values and reference gradients of the
+// state, the Jacobian of the interpolated mapping (det J), and the cofactor (2D: the adjugate of
+// that Jacobian; 3D: C[r][0] = d_zeta P[r][1] - d_eta P[r][2], C[r][1] = d_xi P[r][2] -
+// d_zeta P[r][0], C[r][2] = d_eta P[r][0] - d_xi P[r][1] of the interpolant).  Physical gradient
+// C grad_ref / det J, weights W_q det J; per-cell partial sums as integrated_kernel.
+template <int D, int N>
+__global__ void __launch_bounds__(IQ_THREADS)
+integrated_curv_kernel(const double *__restrict__ u, const double *__restrict__ tab, int nx,
+                       const double *__restrict__ xn, Ops1D ops, double *__restrict__ partial)
+{
+    constexpr int NS = D + 2, NV = D + 1, n = N;
+    constexpr int Np = (D == 3) ? n * n * n : n * n;
+    constexpr int NPF = (D == 3) ? 9 : 1;
+    extern __shared__ double smem[];
+    double *sU = smem;               // [NV][Np]
+    double *sX = sU + NV * Np;       // [D][Np]
+    double *sP = sX + D * Np;        // [9][Np] (3D)
+    double *sB = sP + NPF * Np;      // [nx][n] values, then [nx][n] derivatives, then [nx] weights
+    double *sG = sB + nx * n;
+    double *sW = sG + nx * n;
+    double *red = sW + nx;           // [3][IQ_THREADS]
+    const int e = blockIdx.x;
+    for (int t = threadIdx.x; t < NV * Np; t += blockDim.x) {
+        const int sv = t / Np, i = t % Np;
+        sU[t] = u[((long)e * NS + sv) * Np + i];
+    }
+    for (int t = threadIdx.x; t < D * Np; t += blockDim.x) {
+        const int a = t / Np, i = t % Np;
+        sX[t] = xn[((long)e * Np + i) * 3 + a];
+    }
+    for (int t = threadIdx.x; t < 2 * nx * n + nx; t += blockDim.x) sB[t] = tab[t];
+    __syncthreads();
+    if constexpr (D == 3) {
+        constexpr int LM[3][2] = {{2, 1}, {0, 2}, {1, 0}};
+        for (int i = threadIdx.x; i < Np; i += blockDim.x) {
+            int c[3];
+            node_coords<D, N>(i, c);
+            for (int r = 0; r < 3; ++r)
+                for (int j = 0; j < 3; ++j) {
+                    double dx = 0.0;
+                    for (int l = 0; l < n; ++l) dx += ops.D[c[j]][l] * sX[LM[r][1] * Np + node_with<D, N>(i, j, l)];
+                    sP[(r * 3 + j) * Np + i] = sX[LM[r][0] * Np + i] * dx;
+                }
+        }
+        __syncthreads();
+    }
+    const int nqv = (D == 3) ? nx * nx * nx : nx * nx;
+    double ke = 0.0, ens = 0.0, ike = 0.0;
+    for (int q = threadIdx.x; q < nqv; q += blockDim.x) {
+        const int qx = q % nx, qy = (q / nx) % nx, qz = (D == 3) ? q / (nx * nx) : 0;
+        double val[NV], gr[NV][D], jac[D][D], C[D][D];
+        for (int v = 0; v < NV; ++v) {
+            val[v] = 0.0;
+            for (int k = 0; k < D; ++k) gr[v][k] = 0.0;
+        }
+        for (int a = 0; a < D; ++a)
+            for (int k = 0; k < D; ++k) jac[a][k] = C[a][k] = 0.0;
+        constexpr int nz = (D == 3) ? n : 1;
+        for (int iz = 0; iz < nz; ++iz) {
+            const double bz = (D == 3) ? sB[qz * n + iz] : 1.0, gz = (D == 3) ? sG[qz * n + iz] : 0.0;
+            for (int iy = 0; iy < n; ++iy) {
+                const double by = sB[qy * n + iy], gy = sG[qy * n + iy];
+                const double byz = by * bz, gyz = gy * bz, bgz = by * gz;
+                for (int ix = 0; ix < n; ++ix) {
+                    const double bx = sB[qx * n + ix], gx = sG[qx * n + ix];
+                    const int i = ix + n * (iy + n * iz);
+                    const double wb = bx * byz;
+                    double w[3] = {gx * byz, bx * gyz, bx * bgz};
+                    for (int v = 0; v < NV; ++v) {
+                        const double f = sU[v * Np + i];
+                        val[v] += wb * f;
+                        for (int k = 0; k < D; ++k) gr[v][k] += w[k] * f;
+                    }
+                    for (int a = 0; a < D; ++a) {
+                        const double f = sX[a * Np + i];
+                        for (int k = 0; k < D; ++k) jac[a][k] += w[k] * f;
+                    }
+                    if constexpr (D == 3) {
+                        for (int r = 0; r < 3; ++r) {
+                            const double p0 = sP[(r * 3 + 0) * Np + i], p1 = sP[(r * 3 + 1) * Np + i],
+                                         p2 = sP[(r * 3 + 2) * Np + i];
+                            C[r][0] += w[2] * p1 - w[1] * p2;
+                            C[r][1] += w[0] * p2 - w[2] * p0;
+                            C[r][2] += w[1] * p0 - w[0] * p1;
+                        }
+                    }
+                }
+            }
+        }
+        double det;
+        if constexpr (D == 2) {
+            C[0][0] = jac[1][1];
+            C[1][0] = -jac[0][1];
+            C[0][1] = -jac[1][0];
+            C[1][1] = jac[0][0];
+            det = jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0];
+        } else {
+            det = jac[0][0] * (jac[1][1] * jac[2][2] - jac[1][2] * jac[2][1]) -
+                  jac[0][1] * (jac[1][0] * jac[2][2] - jac[1][2] * jac[2][0]) +
+                  jac[0][2] * (jac[1][0] * jac[2][1] - jac[1][1] * jac[2][0]);
+        }
+        double g[NV][D];
+        for (int v = 0; v < NV; ++v)
+            for (int k = 0; k < D; ++k) {
+                double acc = 0.0;
+                for (int j = 0; j < D; ++j) acc += C[k][j] * gr[v][j] / det;
+                g[v][k] = acc;
+            }
+        const double rho = val[0];
+        double vel[D], vel2 = 0.0;
+        for (int k = 0; k < D; ++k) {
+            vel[k] = val[1 + k] / rho;
+            vel2 = vel2 + vel[k] * vel[k];
+        }
+        double vg[D][D];
+        for (int a2 = 0; a2 < D; ++a2)
+            for (int b2 = 0; b2 < D; ++b2) vg[a2][b2] = (g[1 + a2][b2] - vel[a2] * g[0][b2]) / rho;
+        double om2;
+        if constexpr (D == 2) {
+            const double oz = vg[1][0] - vg[0][1];
+            om2 = oz * oz;
+        } else {
+            const double ox = vg[2][1] - vg[1][2];
+            const double oy = vg[0][2] - vg[2][0];
+            const double oz = vg[1][0] - vg[0][1];
+            om2 = ox * ox + oy * oy + oz * oz;
+        }
+        double wq = sW[qx] * sW[qy];
+        if (D == 3) wq *= sW[qz];
+        wq *= det;
+        ke += 0.5 * rho * vel2 * wq;
+        ens += 0.5 * rho * om2 * wq;
+        ike += 0.5 * vel2 * wq;
+    }
+    red[threadIdx.x] = ke;
+    red[IQ_THREADS + threadIdx.x] = ens;
+    red[2 * IQ_THREADS + threadIdx.x] = ike;
+    __syncthreads();
+    for (int sh = IQ_THREADS / 2; sh > 0; sh >>= 1) {
+        if ((int)threadIdx.x < sh)
+            for (int f = 0; f < 3; ++f) red[f * IQ_THREADS + threadIdx.x] += red[f * IQ_THREADS + threadIdx.x + sh];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int f = 0; f < 3; ++f) partial[(long)e * 3 + f] = red[f * IQ_THREADS];
+}
+
+// Launches the per-cell integrated-quantity kernel (affine box or curvilinear cells) with the
+// QGauss table `tab` (values, derivatives, weights) into partial[cell][3].
+static int launch_iq_cells(dgx_ctx *c, const double *u, const double *tab, int nx, double *partial)
+{
+    const int n = c->n;
+    if (c->curv) {
+        const int npf = (c->dim == 3) ? 9 : 1;
+        const size_t shmem = sizeof(double) * ((size_t)(c->dim + 1 + c->dim + npf) * c->Np + 2 * nx * n + nx +
+                                               3 * IQ_THREADS);
+        switch (c->dim * 16 + n) {
+#define IQC_CASE(DD, NN)                                                                                          \
+    case DD * 16 + NN:                                                                                             \
+        hipLaunchKernelGGL((integrated_curv_kernel<DD, NN>), dim3(c->ncells), dim3(IQ_THREADS), shmem, c->stream, \
+                           u, tab, nx, c->xn_dev, c->ops, partial);                                                \
+        break;
+            IQC_CASE(3, 2) IQC_CASE(3, 3) IQC_CASE(3, 4) IQC_CASE(3, 5) IQC_CASE(3, 6) IQC_CASE(3, 7)
+            IQC_CASE(2, 2) IQC_CASE(2, 3) IQC_CASE(2, 4) IQC_CASE(2, 5) IQC_CASE(2, 6) IQC_CASE(2, 7)
+#undef IQC_CASE
+        default: c->err = "unsupported polynomial degree"; return DGX_ERR_UNSUPPORTED;
+        }
+        return check_launch(c, "integrated_curv_kernel");
+    }
+    const int nr = (c->dim == 3) ? n * n : n;
+    const size_t shmem = sizeof(double) * ((size_t)(c->dim + 1) * c->Np + 2 * nx * n + nx +
+                                           (size_t)(c->dim + 1) * 2 * nx * nr + 3 * IQ_THREADS);
+    const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
+    switch (c->dim * 16 + n) {
+#define IQ_CASE(DD, NN)                                                                                          \
+    case DD * 16 + NN:                                                                                            \
+        hipLaunchKernelGGL((integrated_kernel<DD, NN>), dim3(c->ncells), dim3(IQ_THREADS), shmem, c->stream, u,   \
+                           tab, nx, c->ncells, 1.0 / c->h, detJ, partial);                                        \
+        break;
+        IQ_CASE(3, 2) IQ_CASE(3, 3) IQ_CASE(3, 4) IQ_CASE(3, 5) IQ_CASE(3, 6) IQ_CASE(3, 7)
+        IQ_CASE(2, 2) IQ_CASE(2, 3) IQ_CASE(2, 4) IQ_CASE(2, 5) IQ_CASE(2, 6) IQ_CASE(2, 7)
+#undef IQ_CASE
+    default: c->err = "unsupported polynomial degree"; return DGX_ERR_UNSUPPORTED;
+    }
+    return check_launch(c, "integrated_kernel");
+}
+
 // maximum that keeps a NaN (fmax drops it): a non-physical state reaches the host
 __device__ __forceinline__ double nan_max(double a, double b) { return (a != a || b != b) ? a + b : (a > b ? a : b); }
 
@@ -1015,25 +1206,10 @@ int launch_wave_speed(dgx_ctx *c, const double *u, int overint, double *out)
 
 int launch_integrated(dgx_ctx *c, const double *u, int overint, double *out4)
 {
-    const int n = c->n, nx = c->p + 1 + overint;
-    int rc0 = iq_operators(c, overint);
-    if (rc0) return rc0;
-    const int nr = (c->dim == 3) ? n * n : n;
-    const size_t shmem = sizeof(double) * ((size_t)(c->dim + 1) * c->Np + 2 * nx * n + nx +
-                                           (size_t)(c->dim + 1) * 2 * nx * nr + 3 * IQ_THREADS);
-    const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
-    switch (c->dim * 16 + n) {
-#define IQ_CASE(DD, NN)                                                                                          \
-    case DD * 16 + NN:                                                                                            \
-        hipLaunchKernelGGL((integrated_kernel<DD, NN>), dim3(c->ncells), dim3(IQ_THREADS), shmem, c->stream, u,   \
-                           c->iq_ops, nx, c->ncells, 1.0 / c->h, detJ, c->iq_partial);                            \
-        break;
-        IQ_CASE(3, 2) IQ_CASE(3, 3) IQ_CASE(3, 4) IQ_CASE(3, 5) IQ_CASE(3, 6) IQ_CASE(3, 7)
-        IQ_CASE(2, 2) IQ_CASE(2, 3) IQ_CASE(2, 4) IQ_CASE(2, 5) IQ_CASE(2, 6) IQ_CASE(2, 7)
-#undef IQ_CASE
-    default: c->err = "unsupported polynomial degree"; return DGX_ERR_UNSUPPORTED;
-    }
-    int rc = check_launch(c, "integrated_kernel");
+    const int nx = c->p + 1 + overint;
+    int rc = iq_operators(c, overint);
+    if (rc) return rc;
+    rc = launch_iq_cells(c, u, c->iq_ops, nx, c->iq_partial);
     if (rc) return rc;
     double *sums = c->iq_partial + 3 * (size_t)c->ncells;
     hipLaunchKernelGGL(integrated_sum_kernel, dim3(1), dim3(IQ_THREADS), 0, c->stream, c->iq_partial, c->ncells, sums);
@@ -1144,22 +1320,7 @@ int launch_integrated_static(dgx_ctx *c, const double *u, int overint, double *o
     hipLaunchKernelGGL(iq_table_kernel, dim3(1), dim3(256), 0, c->stream, t, count, table);
     rc = check_launch(c, "iq_table_kernel");
     if (rc) return rc;
-    const int nr = (c->dim == 3) ? n * n : n;
-    const size_t shmem = sizeof(double) * ((size_t)(c->dim + 1) * c->Np + 2 * nx * n + nx +
-                                           (size_t)(c->dim + 1) * 2 * nx * nr + 3 * IQ_THREADS);
-    const double detJ = (c->dim == 3) ? c->h * c->h * c->h : c->h * c->h;
-    switch (c->dim * 16 + n) {
-#define IQ_CASE(DD, NN)                                                                                          \
-    case DD * 16 + NN:                                                                                            \
-        hipLaunchKernelGGL((integrated_kernel<DD, NN>), dim3(c->ncells), dim3(IQ_THREADS), shmem, c->stream, u,   \
-                           table, nx, c->ncells, 1.0 / c->h, detJ, partial);                                      \
-        break;
-        IQ_CASE(3, 2) IQ_CASE(3, 3) IQ_CASE(3, 4) IQ_CASE(3, 5) IQ_CASE(3, 6) IQ_CASE(3, 7)
-        IQ_CASE(2, 2) IQ_CASE(2, 3) IQ_CASE(2, 4) IQ_CASE(2, 5) IQ_CASE(2, 6) IQ_CASE(2, 7)
-#undef IQ_CASE
-    default: c->err = "unsupported polynomial degree"; return DGX_ERR_UNSUPPORTED;
-    }
-    rc = check_launch(c, "integrated_kernel");
+    rc = launch_iq_cells(c, u, table, nx, partial);
     if (rc) return rc;
     hipLaunchKernelGGL(integrated_sum_kernel, dim3(1), dim3(IQ_THREADS), 0, c->stream, partial, c->ncells, sums);
     rc = check_launch(c, "integrated_sum_kernel");

@@ -338,3 +338,114 @@ def test_navier_stokes_rotated_grid_rotates_the_residual(split, diss):
     expect = r.copy()
     expect[:, 1:4] = np.einsum("ij,cjn->cin", R, r[:, 1:4])
     assert np.abs(rr - expect).max() < 1e-12 * np.abs(r).max()
+
+
+# ---------------------------------------------------------------------------------------------
+# Integrated quantities on curved cells (periodic_turbulence.cpp:439-671 with the metric at the
+# QGauss points)
+# ---------------------------------------------------------------------------------------------
+
+def linear_state(o, y, a=0.3, b=-0.2, c=0.25):
+    """rho = 1 + 0.2 y0, velocity linear in the physical coordinates: (a y1, c y0) in 2D,
+    (a y1, b y2, c y0) in 3D, uniform pressure; sampled at the nodes y [cell][node][3]."""
+    dim = o.dim
+    rho = 1.0 + 0.2 * y[..., 0]
+    vel = [a * y[..., 1], c * y[..., 0]] if dim == 2 else [a * y[..., 1], b * y[..., 2], c * y[..., 0]]
+    pinf = 1.0 / (1.4 * 0.3 ** 2)
+    u = o.vec().reshape(o.ncells, o.ns, o.Np)
+    u[:, 0] = rho
+    v2 = 0.0
+    for k, v in enumerate(vel):
+        u[:, 1 + k] = rho * v
+        v2 = v2 + v * v
+    u[:, -1] = pinf / 0.4 + 0.5 * rho * v2
+    return u.ravel()
+
+
+def analytic_integrals(dim, n_cells, a=0.3, b=-0.2, c=0.25, npts=14):
+    """The same integrals of linear_state over the mapped unit box, by Gauss-Legendre quadrature of
+    the analytic map and its analytic Jacobian (quadratic_map): KE, enstrophy, incompressible KE
+    per unit box volume."""
+    xg, wg = np.polynomial.legendre.leggauss(npts)
+    xg, wg = 0.5 * (xg + 1.0), 0.5 * wg
+    h = 1.0 / n_cells
+    grids = np.meshgrid(*([xg] * dim), indexing="ij")
+    wts = np.prod(np.meshgrid(*([wg] * dim), indexing="ij"), axis=0).ravel()
+    om2 = (c - a) ** 2 if dim == 2 else a * a + b * b + c * c
+    ke = ens = ike = 0.0
+    for cell in np.ndindex(*([n_cells] * dim)):
+        x = np.zeros((wts.size, 3))
+        for k in range(dim):
+            x[:, k] = h * (cell[k] + grids[k].ravel())
+        y, Jx = quadratic_map(dim, x)
+        det = np.linalg.det(Jx[:, :dim, :dim] * h)
+        rho = 1.0 + 0.2 * y[:, 0]
+        vel = [a * y[:, 1], c * y[:, 0]] if dim == 2 else [a * y[:, 1], b * y[:, 2], c * y[:, 0]]
+        v2 = sum(v * v for v in vel)
+        ke += np.sum(wts * det * 0.5 * rho * v2)
+        ens += np.sum(wts * det * 0.5 * rho * om2)
+        ike += np.sum(wts * det * 0.5 * v2)
+    return np.array([ke, ens, ike])
+
+
+def integrated_vec(q):
+    return np.array([q["kinetic_energy"], q["enstrophy"], q["incompressible_kinetic_energy"]])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_integrated_quantities_on_a_resolved_map_match_the_analytic_integrals(dim):
+    # the map and the state are resolved at p = 3 (2D: the map is quadratic, the momentum cubic in
+    # x; 3D: the map is multilinear, the curl form's products quadratic), so the metric at the
+    # QGauss points, the physical velocity gradient and the integrands are exact polynomials
+    o = O.Oracle(dim=dim, poly_degree=3, n_cells_1d=2, domain_left=0.0, domain_right=1.0)
+    y = quadratic_map(dim, o.node_coords())[0]
+    o.set_grid_nodes(y)
+    u = linear_state(o, y)
+    ref = analytic_integrals(dim, 2)
+    got = integrated_vec(o.integrated(u))
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), (got, ref)
+    # planted mistakes: the cofactor transposed in the gradient transform; the box's det J
+    for mut, k in ((O.MUT_TRANSPOSED_METRIC, 1), (O.MUT_CELL_CONSTANT_DETJ, 0)):
+        O.set_mutation(mut)
+        try:
+            bad = integrated_vec(o.integrated(u))
+        finally:
+            O.set_mutation(O.MUT_NONE)
+        assert abs(bad[k] - ref[k]) > 1e-6 * abs(ref[k]), (mut, bad, ref)
+
+
+@pytest.mark.parametrize("warp", [0, 1])
+@pytest.mark.parametrize("dim,p,n", [(2, 3, 3), (3, 3, 3), (3, 6, 2)])
+def test_integrated_quantities_of_a_uniform_state_on_the_curved_periodic_grid(dim, p, n, warp):
+    # the periodic warp is a periodic displacement of the box: its det J integrates to the box
+    # volume, so a uniform state gives 1/2 rho |v|^2 and no enstrophy (p = 6: QGauss(17), more
+    # points than the basis holds, which the Gauss rule's scratch once overflowed)
+    lo, hi = BOX[dim]
+    o = O.Oracle(dim=dim, poly_degree=p, n_cells_1d=n, domain_left=lo, domain_right=hi, grid_warp=warp)
+    rho, vel = 1.1, np.array([0.3, -0.2, 0.1])[:dim]
+    u = o.vec().reshape(o.ncells, o.ns, o.Np)
+    u[:, 0] = rho
+    u[:, 1:1 + dim] = rho * vel[None, :, None]
+    u[:, -1] = 10.0 + 0.5 * rho * vel @ vel
+    q = integrated_vec(o.integrated(u.ravel()))
+    assert abs(q[0] - 0.5 * rho * vel @ vel) <= 1e-13
+    assert abs(q[1]) <= 1e-20
+    assert abs(q[2] - 0.5 * vel @ vel) <= 1e-13
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_integrated_quantities_of_the_affine_box_given_as_nodes(dim):
+    lo, hi = BOX[dim]
+    kw = dict(dim=dim, poly_degree=3, n_cells_1d=3, domain_left=lo, domain_right=hi)
+    o, oc = O.Oracle(**kw), O.Oracle(**kw)
+    oc.set_grid_nodes(o.node_coords())
+    rng = np.random.default_rng(5)
+    u = o.ic_tgv().reshape(o.ncells, o.ns, o.Np) if dim == 3 else None
+    if u is None:
+        u = o.vec().reshape(o.ncells, o.ns, o.Np)
+        u[:, 0] = 1.0
+        u[:, 1:3] = 0.1 * np.sin(o.node_coords()[..., :2]).transpose(0, 2, 1)
+        u[:, 3] = 20.0
+    u = (u * (1.0 + 1e-2 * rng.standard_normal(u.shape))).ravel()
+    a, b = integrated_vec(o.integrated(u)), integrated_vec(oc.integrated(u))
+    assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
