@@ -68,7 +68,15 @@ template <int D, int N, typename T, int ES = 1, bool V = false, bool XJ = false>
     // the stage kernel ran 3 % (mixed) and 1.5 % (FP64) faster; at N = 6 the skew was slower.
     static constexpr bool QSKEW = (D == 3 && N == 4);
     static constexpr int pad16(int x) { return x + (48 - x % 32) % 32; }
-    static constexpr int ESQ = (QSKEW && FAST) ? pad16(NQ * NP) : pad1(NQ * NP);
+    // 2D, float: the items of a wave are consecutive lines of consecutive elements, so a line
+    // phase along y reads node line + N pos of elements e, e+1, ..: with a stride of 1 mod 32
+    // words five to seven lanes met in one bank.  The element stride's residue mod 64 is chosen
+    // per N (a search over the 32- and 64-bank layouts and the three elements-per-block tiles)
+    // so that no more than two lanes share a bank along x or y.
+    static constexpr int res2d(int n) { return n == 3 ? 41 : n == 4 ? 41 : n == 5 ? 25 : n == 6 ? 53 : n == 7 ? 8 : 1; }
+    static constexpr int pad2d(int x) { return x + ((res2d(N) - x % 64) % 64 + 64) % 64; }
+    static constexpr bool P2D = (D == 2 && FAST && N >= 3);
+    static constexpr int ESQ = (QSKEW && FAST) ? pad16(NQ * NP) : (P2D ? pad2d(NQ * NP) : pad1(NQ * NP));
     // Long lines are shared by LS items (the pairs and the viscous columns split evenly), so
     // that the line phase keeps most of the block busy: from p = 4 in 2D and for Navier-Stokes;
     // for 3D Euler at p = 5, and at p = 6 on the FP64 path only (at p = 4 the extra partials
@@ -76,7 +84,7 @@ template <int D, int N, typename T, int ES = 1, bool V = false, bool XJ = false>
     // kernel's item loop was no longer unrolled, spilling the line sums to scratch)
     static constexpr int LS = (V || D == 2) ? (N >= 7 ? 3 : (N >= 5 ? 2 : 1))
                                             : (N >= 7 ? (FAST ? 1 : 3) : (N >= 6 ? 2 : 1));
-    static constexpr int ESV = pad1(LS * NS * NP);   // line partials summed over directions [part][state][node]
+    static constexpr int ESV = P2D ? pad2d(LS * NS * NP) : pad1(LS * NS * NP);   // line partials [part][state][node]
     static constexpr int NFN = 2 * D * NL;           // face nodes per element
     // neighbour traces' states [state][face][face node]; each slot then takes that face node's
     // surface term
