@@ -60,6 +60,21 @@ def test_batch_matches_single_solves():
     for p_, prec in ((4, P.FP64), (5, P.FP64), (5, P.MIXED), (6, P.FP64), (6, P.MIXED)):
         cases.append((dict(dim=3, poly_degree=p_, n_cells_1d=4, use_split_form=1, two_point_flux=P.CH,
                            conv_flux=P.TWO_POINT_ROE, precision=prec), None))
+    # solves outside the batched line kernel, run through their own kernels inside the graph: the
+    # curved periodic grid (Euler and Navier-Stokes), boundary faces, Navier-Stokes walls
+    tgv_box = dict(domain_left=0.0, domain_right=2 * np.pi)
+    cases.append((dict(dim=3, poly_degree=3, n_cells_1d=4, grid_warp=1, use_split_form=1, two_point_flux=P.CH,
+                       conv_flux=P.TWO_POINT_ROE, precision=P.FP64, **tgv_box), None))
+    cases.append((dict(dim=3, poly_degree=2, n_cells_1d=4, grid_warp=1, pde=P.NAVIER_STOKES, reynolds_inf=1600.0,
+                       use_split_form=1, two_point_flux=P.RA, conv_flux=P.TWO_POINT_ROE, precision=P.MIXED,
+                       **tgv_box), None))
+    cases.append((dict(dim=2, poly_degree=3, n_cells_1d=6, domain_left=-5.0, domain_right=5.0, mach_inf=MACH_VORTEX,
+                       boundary_type=[1004, 1004, 1006, 1006], use_split_form=1, two_point_flux=P.CH,
+                       conv_flux=P.TWO_POINT_ROE, precision=P.MIXED), (3.0, 0.7, 0.8)))
+    cases.append((dict(dim=3, poly_degree=3, n_cells_1d=4, pde=P.NAVIER_STOKES, reynolds_inf=1600.0,
+                       boundary_type=[0, 0, 1001, 1001, 1006, 1006], thermal_bc=1, wall_temperature=1.1,
+                       use_split_form=1, two_point_flux=P.RA, conv_flux=P.TWO_POINT_ROE, precision=P.FP64,
+                       **tgv_box), None))
     log(f"batch of {len(cases)} instances")
     b = P.Batch([kw for kw, _ in cases])
     dts = []
