@@ -40,6 +40,13 @@ template <> __device__ __forceinline__ float dlog<float>(float x) { return __log
 template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 #endif
+// An integer the optimizer may not reason about past this point (an empty asm that takes and
+// returns it in a register): used where gfx950 -O3 mis-folded a chain of comparisons.
+#ifdef DGX_HOST_TEST
+inline void opaque_int(int &) {}
+#else
+__device__ __forceinline__ void opaque_int(int &x) { asm volatile("" : "+v"(x)); }
+#endif
 
 // compute_pressure_templated (euler.cpp:486-500)
 template <typename T, int D>
@@ -127,7 +134,13 @@ __device__ __forceinline__ void boundary_state(int type, const T *ui, int K, T s
 #pragma unroll
     for (int k = 0; k < D; ++k) vel_b[k] = vel_i[k];
     bool copy = false;  // exterior = interior conservative state
-    if (type == 1001 || type == 1006) {
+    // 1007 is tested first, by equality, with the type opaque to the optimizer: inlined into the
+    // curvilinear stage kernel, gfx950 -O3 turned this chain into range tests that sent 1007 down
+    // the slip-wall branch (its code held no comparison against 1007)
+    opaque_int(type);
+    if (type == 1007) {
+        copy = true;
+    } else if (type == 1001 || type == 1006) {
 #pragma unroll
         for (int k = 0; k < D; ++k)
             if (k == K) vel_b[k] = -vel_i[k];  // v - 2 (v.n) n with n = +-e_K
@@ -264,7 +277,11 @@ __device__ __forceinline__ void boundary_state_n(int type, const T *ui, const T 
 #pragma unroll
     for (int k = 0; k < D; ++k) vel_b[k] = vel_i[k];
     bool copy = false;
-    if (type == 1001 || type == 1006) {
+    // 1007 first, by equality, the type opaque to the optimizer (see boundary_state)
+    opaque_int(type);
+    if (type == 1007) {
+        copy = true;
+    } else if (type == 1001 || type == 1006) {
         // v - 2 (v.n) n
 #pragma unroll
         for (int k = 0; k < D; ++k) vel_b[k] = vel_i[k] - T(2) * vn_i * n[k];

@@ -139,6 +139,19 @@ def test_boundaries_on_the_curved_grid(scheme):
     check(o, d, random_field(o, seed=5), f"curved grid with boundaries {scheme}")
 
 
+@pytest.mark.parametrize("btype", [1001, 1002, 1003, 1004, 1005, 1006, 1007])
+@pytest.mark.parametrize("scheme", ["roe", "ra_lxf"])
+def test_every_boundary_type_on_curved_caller_nodes(btype, scheme):
+    # every face of the unit box bent by the quadratic map, all six of one type (Euler)
+    kw = kw_for(3, 3, scheme, warp=0, boundary_type=[btype] * 6)
+    kw.update(domain_left=0.0, domain_right=1.0)
+    o, d = O.Oracle(**kw), P.DG(**kw)
+    x = quadratic_map(3, o.node_coords())
+    o.set_grid_nodes(x)
+    d.set_grid_nodes(x)
+    check(o, d, random_field(o, seed=btype, vel=0.3), f"curved caller nodes, all faces {btype}, {scheme}")
+
+
 @pytest.mark.parametrize("method", ["rk4", "rk4_3_5_3sstar", "rk3_2_5f_3sstarplus"])
 @pytest.mark.parametrize("scheme", ["roe", "ch_roe"])
 def test_time_stepping_on_the_curved_grid(method, scheme):
@@ -272,11 +285,12 @@ WALL_SETS = [("noslip_adiabatic", [1001] * 6, 0, 1.0), ("noslip_isothermal", [10
 
 @pytest.mark.parametrize("name,bcs,thermal,tw", WALL_SETS, ids=[w[0] for w in WALL_SETS])
 @pytest.mark.parametrize("split", [0, 1])
-def test_navier_stokes_walls_on_caller_nodes(name, bcs, thermal, tw, split):
+@pytest.mark.parametrize("diss", [O.SIPG, O.BR2])
+def test_navier_stokes_walls_on_caller_nodes(name, bcs, thermal, tw, split, diss):
     # curved walls: the quadratic map bends every face of the unit box
-    kw = ns_kw(split, O.SIPG, 3, grid_warp=0, domain_right=1.0, boundary_type=bcs, thermal_bc=thermal,
+    kw = ns_kw(split, diss, 3, grid_warp=0, domain_right=1.0, boundary_type=bcs, thermal_bc=thermal,
                wall_temperature=tw)
-    log(f"curved walls {name} split={split}")
+    log(f"curved walls {name} split={split} diss={diss}")
     o, d = O.Oracle(**kw), P.DG(**kw)
     x = quadratic_map(3, o.node_coords())
     o.set_grid_nodes(x)
