@@ -282,7 +282,7 @@ aux_kernel(const double *__restrict__ u, Q *__restrict__ q0, Q *__restrict__ q1,
                 const bool own = nb < 0 || e < nb;
                 for (int k = 0; k < D; ++k) {
                     const double nk = sgn * (own ? met[((long)e * MS + k * D + d) * NP + i]
-                                                 : met[((long)nb * MS + k * D + d) * NP + j]);
+                                                 : met[((long)(nb < 0 ? 0 : nb) * MS + k * D + d) * NP + j]);
                     for (int s = 0; s < NS; ++s) qp[k][s] += lift[s] * nk;
                 }
             } else {

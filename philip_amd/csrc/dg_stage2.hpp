@@ -206,7 +206,7 @@ stage2_kernel(StageArgs a, Ops1D ops, PhysConst pc)
 #pragma unroll
                 for (int m = 0; m < D; ++m) {
                     nv[m] = own ? sgn * sC[fle][m * D + K][fi]
-                                : -sgn * T(a.met[((long)nb * MS + m * D + K) * NP + fj]);
+                                : -sgn * T(a.met[((long)(nb < 0 ? 0 : nb) * MS + m * D + K) * NP + fj]);
                     area2 += nv[m] * nv[m];
                 }
                 const T area = dsqrt<T>(area2);

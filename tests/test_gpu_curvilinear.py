@@ -141,15 +141,16 @@ def test_boundaries_on_the_curved_grid(scheme):
 
 @pytest.mark.parametrize("btype", [1001, 1002, 1003, 1004, 1005, 1006, 1007])
 @pytest.mark.parametrize("scheme", ["roe", "ra_lxf"])
-def test_every_boundary_type_on_curved_caller_nodes(btype, scheme):
-    # every face of the unit box bent by the quadratic map, all six of one type (Euler)
-    kw = kw_for(3, 3, scheme, warp=0, boundary_type=[btype] * 6)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_every_boundary_type_on_curved_caller_nodes(btype, scheme, dim):
+    # every face of the unit box bent by the quadratic map, all of one type (Euler)
+    kw = kw_for(dim, 3, scheme, warp=0, boundary_type=[btype] * (2 * dim))
     kw.update(domain_left=0.0, domain_right=1.0)
     o, d = O.Oracle(**kw), P.DG(**kw)
-    x = quadratic_map(3, o.node_coords())
+    x = quadratic_map(dim, o.node_coords())
     o.set_grid_nodes(x)
     d.set_grid_nodes(x)
-    check(o, d, random_field(o, seed=btype, vel=0.3), f"curved caller nodes, all faces {btype}, {scheme}")
+    check(o, d, random_field(o, seed=btype, vel=0.3), f"curved caller nodes {dim}d, all faces {btype}, {scheme}")
 
 
 @pytest.mark.parametrize("method", ["rk4", "rk4_3_5_3sstar", "rk3_2_5f_3sstarplus"])
