@@ -449,3 +449,26 @@ def test_integrated_quantities_of_the_affine_box_given_as_nodes(dim):
     u = (u * (1.0 + 1e-2 * rng.standard_normal(u.shape))).ravel()
     a, b = integrated_vec(o.integrated(u)), integrated_vec(oc.integrated(u))
     assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
+
+
+@pytest.mark.parametrize("diss", [O.SIPG, O.BR2])
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("walls", [0, 1])
+def test_navier_stokes_affine_box_as_node_coordinates_is_the_cartesian_residual(diss, split, walls):
+    # the curvilinear Navier-Stokes path (lifted gradient through the metric, contracted viscous
+    # fluxes, per-cell penalty from the cell extents) reproduces the Cartesian one on the box:
+    # every extent is h there, so the SIPG penalty is the box's deg (deg + 1) / h
+    extra = dict(boundary_type=[1001, 1001, 0, 0, 1006, 1006], thermal_bc=1, wall_temperature=1.1) if walls else {}
+    kw = ns_kw(split, diss, domain_left=0.0, domain_right=1.0, **extra)
+    o, oc = O.Oracle(**kw), O.Oracle(**kw)
+    oc.set_grid_nodes(o.node_coords())
+    u = random_field(o, 4, vel=0.3)
+    r, rc = o.residual(u), oc.residual(u)
+    assert np.abs(rc - r).max() < 1e-12 * np.abs(r).max()
+    for k in range(3):
+        assert np.abs(oc.auxiliary(k) - o.auxiliary(k)).max() < 1e-12 * np.abs(o.auxiliary(k)).max()
+    if diss == O.SIPG:
+        # the penalty matters: a different penalty factor changes the residual
+        rp = O.Oracle(**dict(kw, sipg_penalty_factor=2.0))
+        rp.set_grid_nodes(o.node_coords())
+        assert np.abs(rp.residual(u) - rc).max() > 1e-6 * np.abs(r).max()
