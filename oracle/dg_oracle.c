@@ -17,7 +17,12 @@
  * Parity of this restatement is pinned by the reference's own golden values
  * (viscous TGV kinetic energy and dissipation rate at 1e-10, see
  * tests/test_oracle_golden.py) and by closed-form / invariant checks
- * (tests/test_oracle_*.py).
+ * (tests/test_oracle_*.py).  Parity unpinned: penalty_scaling's cell extent on
+ * curved cells (the longest cell edge along the direction, deal.II's
+ * extent_in_direction) -- pinned only on parallelepipeds, where it is h.
+ *
+ * Curvilinear grids (node coordinates set by dgo_set_grid_nodes or the curved
+ * periodic grid) replace the affine metric by per-node metric terms.
  */
 #include "dg_oracle.h"
 
