@@ -249,7 +249,8 @@ class DG:
 
     def set_solution(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
-        assert u.size == self.ndofs
+        if u.size != self.ndofs:
+            raise ValueError(f"solution: expected {self.ndofs} values, got {u.size}")
         self._chk(load().dgx_set_solution(self.h, _dp(u)), "set_solution")
 
     def set_initial_condition_tgv(self, isothermal=False):
@@ -294,7 +295,11 @@ class DG:
 
     def set_grid_nodes(self, xyz):
         """Curvilinear grid from node coordinates [cell][node][3] (grid degree = p)."""
-        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.ncells, -1, 3)
+        npn = self.ndofs // (self.ncells * (self.dim + 2))
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        if x.size != self.ncells * npn * 3:
+            raise ValueError(f"node coordinates: expected {self.ncells} x {npn} x 3 values, got {x.size}")
+        x = x.reshape(self.ncells, npn, 3)
         self._chk(load().dgx_set_grid_nodes(self.h, _dp(x)), "set_grid_nodes")
 
     def grid_nodes(self):
@@ -507,7 +512,8 @@ class Batch:
     def allocate(self, method, dts):
         m = METHODS[method] if isinstance(method, str) else method
         dts = np.ascontiguousarray(dts, dtype=np.float64)
-        assert dts.size == len(self.instances)
+        if dts.size != len(self.instances):
+            raise ValueError(f"time steps: expected one per instance ({len(self.instances)}), got {dts.size}")
         self._chk(load().dgx_batch_allocate(self.h, m, _dp(dts)), "batch_allocate")
 
     def step_in_time(self, n_steps=1):
